@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Frames/s of the rebin-then-rasterize path on one MI355X.
+
+A step is one rebin (preprocess -> scan -> emit -> radix sort -> identify ->
+rasterize) of the 1M-splat SH3 scene at 1920x1080 with 16x16 tiles, splats
+already resident in HBM. Prints ONE JSON line (rank 0).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1]
+                  [--no-cpu-baseline] [--cpu-seconds S]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "webgpu-3dgs_amd"))
+
+METRIC = json.loads((REPO / "BASELINE.json").read_text())["metric"]
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+RASTER_FLOP_PER_PAIR = 23  # ~22 flop + 1 exp per evaluated (pixel, splat) pair
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(n, k, tiles, pixels, passes, deg):
+    """Bytes each stage must move at minimum, on the canonical layouts."""
+    pre_in = 64 + (12 * (deg + 1) ** 2 if deg else 0)
+    return {
+        "preprocess-splats": n * (pre_in + 68),
+        "prefix-scan": 8 * n + 4,
+        "emit-tile-refs": 68 * n + 12 * k,
+        "radix-sort": passes * 24 * k,
+        "identify-tile-ranges": 8 * k + 4 * (tiles + 1),
+        "rasterize-tiles": 68 * k + 8 * (tiles + 1) + 8 * pixels,
+    }
+
+
+def cpu_baseline(cam, splats, W, H, tile, seconds):
+    """The oracle's fp32 pipeline (a port of the reference WGSL passes) on the
+    host cores, whole frames, until `seconds` of work or 3 frames."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_lib as O
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    O.set_threads(threads)
+    frames, t0, visited = 0, time.perf_counter(), 0
+    while True:
+        _, k, visited, blended = O.render(cam, splats, W, H, tile, "f32")
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or frames >= 3:
+            break
+    return {"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} whole frames of the same workload (oracle fp32 pipeline, "
+                      f"{threads} OpenMP threads, {el:.1f} s)"}, visited, blended
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", type=int, default=1)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this path runs on a single MI355X "
+              "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+        return 2
+
+    from gs3d import camera, scene
+    from gs3d.renderer import GaussianSplatRenderer
+
+    t = time.perf_counter()
+    wl = scene.workload(args.workload)
+    splats = scene.activate(wl.raw)
+    eye, W, H = wl.views[0]
+    cu = camera.view_camera(eye, W, H, wl.fov_y)
+    log(f"[bench] workload {wl.name}: {splats.n} splats SH{splats.sh_degree}, {W}x{H} "
+        f"(generated in {time.perf_counter() - t:.1f} s)")
+
+    r = GaussianSplatRenderer(0)
+    r.set_splat_data(splats)
+    r.set_tile_size(args.tile, args.tile)
+    for _ in range(args.warmup):
+        r.rebin(cu, W, H)
+    r.sync()
+    k = r.ref_count()
+    log(f"[bench] warmup done: K={k} keys, passes at bits {r.sort_passes()}")
+
+    # timed region: K back-to-back rebins, hipEvent pair per stage on the
+    # context's stream (the stream every kernel is launched on)
+    r.profiler_reset()
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.rebin(cu, W, H, 1)  # GS3D_REBIN_PROFILE
+    r.sync()
+    el = time.perf_counter() - t0
+    stages, frames = r.stage_times_avg_us()
+    ms = el * 1000.0 / args.steps
+    log(f"[bench] {args.steps} steps in {el:.3f} s: {ms:.3f} ms/frame; stages (us) "
+        + ", ".join(f"{n}={v:.1f}" for n, v in stages.items()))
+
+    tiles_x, tiles_y = r.get_tile_count()
+    T = tiles_x * tiles_y
+    passes = len(r.sort_passes())
+    algo = algorithmic_bytes(splats.n, k, T, W * H, passes, splats.sh_degree)
+
+    cpu = None
+    visited = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle port) ...")
+        cpu, visited, _ = cpu_baseline(cu, splats, W, H, args.tile, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu['value']:.3f} frames/s on {cpu['cores']} cores")
+
+    per_stage = {}
+    for name, b in algo.items():
+        us = stages.get(name, 0.0)
+        gbs = b / (us * 1e-6) / 1e9 if us > 0 else 0.0
+        per_stage[name] = {"us": round(us, 2), "algo_bytes": int(b), "GB/s": round(gbs, 1),
+                           "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if visited:
+        us = stages["rasterize-tiles"]
+        tf = visited * RASTER_FLOP_PER_PAIR / (us * 1e-6) / 1e12
+        per_stage["rasterize-tiles"].update({"pairs": visited, "TFLOP/s": round(tf, 2),
+                                             "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
+    dominant = max(algo, key=lambda n: stages.get(n, 0.0))
+    d = per_stage[dominant]
+    roofline = {"kernel": dominant, "bound": "hbm", "achieved": d["GB/s"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": d["hbm_frac"], "traffic": None}
+    fps = 1000.0 / ms
+    out = {
+        "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "splats": splats.n, "sh_degree": splats.sh_degree,
+                   "width": W, "height": H, "tile": args.tile, "keys": k, "tiles": T,
+                   "radix_passes": passes, "digit_bits": 8},
+        "mkeys_per_s": round(k / (stages["radix-sort"] * 1e-6) / 1e6, 1) if stages["radix-sort"] else None,
+        "stages_us": {n: round(v, 2) for n, v in stages.items()},
+        "profiled_frames": frames,
+        "per_stage": per_stage,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    r.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

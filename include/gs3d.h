@@ -1,0 +1,228 @@
+/*
+ * gs3d.h — C ABI of libgs3d.so, the MI355X (gfx950) rebin-then-rasterize
+ * 3D Gaussian Splatting path.
+ *
+ * Every entry point returns GS3D_OK (0) or a negative GS3D_E* code; the message
+ * of the last failure on a context is gs3d_last_error(ctx). No C++ exception
+ * crosses this boundary. Host pointers are borrowed for the duration of the
+ * call only (copied host->device or device->host inside the call). A context
+ * owns all of its device memory and one HIP stream; it is not thread-safe
+ * (one caller thread per context, as the reference's single JS thread).
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   IRenderer.init / render ............ src/renderers/IRenderer.ts:10-13
+ *   GaussianSplatRenderer.rebin ........ src/renderers/GaussianSplatRenderer.ts:1104-1266
+ *   setTileSize / getTileCount / Size .. GaussianSplatRenderer.ts:301-317, 1268-1274
+ *   readTileCounts ..................... GaussianSplatRenderer.ts:1278-1310
+ *   SceneSyncer.setSplatData ........... src/SceneSyncer.ts:25-43
+ *   BufferManager named buffers ........ src/BufferManager.ts:32-154
+ *   Profiler per-pass timings .......... src/Profiler.ts:135-251
+ *   WGSL passes (stage entry points) ... src/shaders/compute/ (one per pass)
+ */
+#ifndef GS3D_H
+#define GS3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS3D_ABI_VERSION 1
+
+/* ---- error codes -------------------------------------------------------- */
+#define GS3D_OK               0
+#define GS3D_E_INVALID_ARG   -1   /* null pointer, zero/oversize dimension      */
+#define GS3D_E_NO_DEVICE     -2   /* HIP device missing or not gfx950-capable   */
+#define GS3D_E_HIP           -3   /* a HIP runtime call failed                  */
+#define GS3D_E_OOM           -4   /* device allocation failed                   */
+#define GS3D_E_STATE         -5   /* call order: e.g. rebin before set_splats   */
+#define GS3D_E_UNKNOWN_NAME  -6   /* registry miss (buffer / kernel variant)    */
+#define GS3D_E_TOO_SMALL     -7   /* destination buffer smaller than the data   */
+
+/* ---- byte-exact data contract ------------------------------------------- */
+
+/* CameraUniforms, 72 x f32 = 288 B: view, projection, inverse view, inverse
+ * projection (column-major mat4), position.xyz1, viewport (w, h, w/h, 0).
+ * Reference: preprocess_splats.wgsl:1-8, Camera.getUniformData Camera.ts:172-191. */
+typedef struct gs3d_camera {
+    float view[16];
+    float projection[16];
+    float inverse_view[16];
+    float inverse_projection[16];
+    float position[4];
+    float viewport[4];
+} gs3d_camera;
+
+/* ProjectedSplat, 64 B. Reference: preprocess_splats.wgsl:18-31.
+ * depth is the view-space z (negative in front of the camera). */
+typedef struct gs3d_projected_splat {
+    float    mean_px[2];
+    float    depth;
+    float    radius_px;
+    float    conic[3];
+    float    opacity;
+    float    color[3];
+    uint32_t valid;
+    uint32_t tile_min[2];
+    uint32_t tile_max[2];
+} gs3d_projected_splat;
+
+/* A sort key is the little-endian u64 (tile_id << 32) | bits(-depth), i.e. the
+ * interleaved {hi, lo} u32 pair of emit_tile_refs.wgsl:41-46 read as one u64. */
+
+/* Buffers readable with gs3d_read (the reference's named buffers,
+ * GaussianSplatRenderer.ts:338-465). */
+enum gs3d_buffer_id {
+    GS3D_BUF_PROJECTED_SPLATS  = 0,  /* N x gs3d_projected_splat                    */
+    GS3D_BUF_SPLAT_REF_COUNTS  = 1,  /* N x u32                                     */
+    GS3D_BUF_SPLAT_REF_OFFSETS = 2,  /* (N+1) x u32, [N] = total (sentinel)         */
+    GS3D_BUF_REF_COUNTER       = 3,  /* 1 x u32 = K                                 */
+    GS3D_BUF_SORT_KEYS         = 4,  /* K x u64, sorted by (tile, depth)            */
+    GS3D_BUF_SORT_VALUES       = 5,  /* K x u32 splat ids, in sorted order          */
+    GS3D_BUF_TILE_OFFSETS      = 6,  /* (T+1) x u32, trailing tiles stay 0          */
+    GS3D_BUF_IMAGE_RGBA16F     = 7,  /* W*H x 4 x f16, premultiplied (C, 1-T)       */
+    GS3D_BUF_IMAGE_RGBA32F     = 8,  /* same image widened to f32 on readback       */
+    GS3D_BUF_UNSORTED_KEYS     = 9,  /* K x u64 emit order (before the sort)        */
+    GS3D_BUF_UNSORTED_VALUES   = 10, /* K x u32 emit order                          */
+    GS3D_BUF__COUNT            = 11
+};
+
+/* rebin flags */
+#define GS3D_REBIN_PROFILE     1u   /* record a hipEvent pair per stage           */
+#define GS3D_REBIN_KEEP_UNSORTED 2u /* keep a copy of the emit-order keys/values  */
+
+/* Stage labels of gs3d_stage_times_us (reference Profiler labels where a
+ * reference pass exists, GaussianSplatRenderer.ts:1143-1265). */
+enum gs3d_stage {
+    GS3D_STAGE_PREPROCESS = 0,  /* "preprocess-splats"                       */
+    GS3D_STAGE_SCAN       = 1,  /* "prefix-scan"  (local+blocks+add fused)   */
+    GS3D_STAGE_EMIT       = 2,  /* "emit-tile-refs" (+ digit histograms)     */
+    GS3D_STAGE_SORT       = 3,  /* "radix-sort"  (all onesweep passes)       */
+    GS3D_STAGE_IDENTIFY   = 4,  /* "identify-tile-ranges"                    */
+    GS3D_STAGE_RASTER     = 5,  /* "rasterize-tiles"                         */
+    GS3D_STAGE_TOTAL      = 6,  /* whole rebin, first launch to last         */
+    GS3D_STAGE__COUNT     = 7
+};
+
+typedef struct gs3d_ctx gs3d_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on HIP device `device` (IRenderer.init, IRenderer.ts:10-11). */
+int gs3d_create(int device, gs3d_ctx** out);
+int gs3d_destroy(gs3d_ctx* ctx);
+const char* gs3d_last_error(const gs3d_ctx* ctx);
+int gs3d_abi_version(void);
+/* Block until all work queued on the context's stream has finished. */
+int gs3d_sync(gs3d_ctx* ctx);
+
+/* Device limits in the reference's WorkgroupLimits order
+ * (types.ts:49-56, WorkgroupManager.ts:15-24): maxTotalThreads, maxSizeX,
+ * maxSizeY, maxSizeZ, maxDispatch, maxSharedMemory. */
+int gs3d_device_limits(gs3d_ctx* ctx, uint32_t out[6]);
+
+/* ---- scene upload (SceneSyncer.setSplatData, SceneSyncer.ts:25-43) ---------
+ * pos4/scale4/rot4/color4: N x vec4 f32 in the loader's layouts
+ * (SceneWorker.ts:407-464): (x,y,z,1), (e^s0,e^s1,e^s2,0), normalised (w,x,y,z),
+ * (clamp01(0.5+C0 f_dc), clamp01(sigmoid(opacity))).
+ * sh_degree 0 uses color4 directly (the reference behaviour). For sh_degree
+ * 1..3, f_dc (N x 3 raw PLY values) and sh_rest (N x 3*((d+1)^2-1) f32 in
+ * f_rest_k index order: all R coefficients, then G, then B) are required. */
+int gs3d_set_splats(gs3d_ctx* ctx,
+                    const float* pos4, const float* scale4,
+                    const float* rot4, const float* color4,
+                    const float* f_dc, const float* sh_rest,
+                    int sh_degree, uint32_t n);
+uint32_t gs3d_splat_count(const gs3d_ctx* ctx);
+
+/* ---- renderer API (GaussianSplatRenderer.ts) ----------------------------- */
+
+/* Clamp to device limits and set the tile size; writes the size in use
+ * (setTileSize, GaussianSplatRenderer.ts:301-317, clampTileSize :162-167).
+ * Compiled tile variants are 8x8, 16x16 and 32x32 (1 thread per pixel up to
+ * 1024 threads); other requests are rejected with GS3D_E_UNKNOWN_NAME. */
+int gs3d_set_tile_size(gs3d_ctx* ctx, uint32_t x, uint32_t y, uint32_t out_xy[2]);
+int gs3d_get_tile_size(const gs3d_ctx* ctx, uint32_t out_xy[2]);
+int gs3d_get_tile_count(const gs3d_ctx* ctx, uint32_t out_xy[2]);
+
+/* Stages 1-6A on the context's stream: preprocess, scan, emit, sort, identify,
+ * rasterize (rebin, GaussianSplatRenderer.ts:1104-1266). `camera` is the 288-B
+ * CameraUniforms. Returns once the work is queued; gs3d_read / gs3d_sync wait. */
+int gs3d_rebin(gs3d_ctx* ctx, const gs3d_camera* camera,
+               uint32_t width, uint32_t height, uint32_t flags);
+
+/* Number of keys K emitted by the last rebin (ref_counter[0]). */
+int gs3d_ref_count(gs3d_ctx* ctx, uint64_t* out);
+
+/* Copy a named result of the last rebin into caller memory. `written` gets the
+ * number of bytes the buffer holds; GS3D_E_TOO_SMALL if `bytes` is short. */
+int gs3d_read(gs3d_ctx* ctx, int buffer_id, void* dst, size_t bytes, size_t* written);
+
+/* Per-tile splat counts with the underflow guard of readTileCounts
+ * (GaussianSplatRenderer.ts:1278-1310): counts[t] = off[t+1] >= off[t] ?
+ * off[t+1]-off[t] : 0 for t < T. */
+int gs3d_read_tile_counts(gs3d_ctx* ctx, uint32_t* counts, size_t n, size_t* written);
+
+/* Per-stage GPU times (microseconds) of the last rebin with GS3D_REBIN_PROFILE,
+ * indexed by enum gs3d_stage (Profiler.ts:197-228 sums repeated labels). */
+int gs3d_stage_times_us(gs3d_ctx* ctx, float* out, int n);
+const char* gs3d_stage_label(int stage);
+
+/* Average per-stage times over every profiled rebin since the last reset
+ * (the Profiler's averaging window, Profiler.ts:231-251); `frames` gets the
+ * number of frames averaged. */
+int gs3d_profiler_reset(gs3d_ctx* ctx);
+int gs3d_stage_times_avg_us(gs3d_ctx* ctx, float* out, int n, uint64_t* frames);
+
+/* Number of radix passes the last rebin's sort ran and the bit offset of each
+ * (passes whose digit is constant over all keys are skipped). */
+int gs3d_sort_passes(gs3d_ctx* ctx, uint32_t* n_passes, uint32_t bit_offsets[16]);
+
+/* ---- stage entry points (each WGSL pass fed caller data) ---------------- */
+
+/* preprocess_splats.wgsl:75-184 over caller splats. projected: N x 64 B,
+ * counts: N x u32. tile_count/inv_tile are SplatBinningUniforms fields
+ * (GaussianSplatRenderer.ts:1119-1131). Uses the uploaded SH if sh_degree>0. */
+int gs3d_stage_preprocess(gs3d_ctx* ctx, const gs3d_camera* camera,
+                          uint32_t tiles_x, uint32_t tiles_y,
+                          float inv_tile_x, float inv_tile_y,
+                          gs3d_projected_splat* projected, uint32_t* counts);
+
+/* Exclusive prefix scan of u32 counts (prefix_scan_local/blocks/add.wgsl):
+ * offsets[i] = sum_{j<i} counts[j] mod 2^32, offsets[n] = total. `offsets`
+ * holds n+1 entries. */
+int gs3d_stage_scan(gs3d_ctx* ctx, const uint32_t* counts, uint32_t n,
+                    uint32_t* offsets, uint32_t* total);
+
+/* emit_tile_refs.wgsl:56-79: one (key, splat_id) per overlapped tile, written
+ * at offsets[id]. `offsets` has n+1 entries; keys/values hold offsets[n]. */
+int gs3d_stage_emit(gs3d_ctx* ctx, const gs3d_projected_splat* projected,
+                    const uint32_t* offsets, uint32_t n, uint32_t tiles_x,
+                    uint64_t* keys, uint32_t* values);
+
+/* Stable LSD radix sort of (u64 key, u32 value) pairs — the result of
+ * radix_histogram/scan_local/scan_add/radix_scatter.wgsl over all 64 bits:
+ * pairs ordered by (key, input position). digit_bits is 4 or 8. */
+int gs3d_stage_sort(gs3d_ctx* ctx, const uint64_t* keys_in, const uint32_t* values_in,
+                    uint32_t n, uint32_t digit_bits,
+                    uint64_t* keys_out, uint32_t* values_out);
+
+/* identify_tile_ranges.wgsl:30-52 over caller sorted keys; tile_offsets holds
+ * tile_count+1 entries, zero-initialised inside the call. */
+int gs3d_stage_identify(gs3d_ctx* ctx, const uint64_t* sorted_keys, uint32_t n,
+                        uint32_t tile_count, uint32_t* tile_offsets);
+
+/* rasterize_tiles.wgsl:43-110 with tile size tile_x x tile_y (8, 16 or 32).
+ * image: W*H*4 f32 premultiplied (C, 1-T), rounded through f16 like the
+ * reference's rgba16float target. */
+int gs3d_stage_rasterize(gs3d_ctx* ctx, const gs3d_projected_splat* projected,
+                         uint32_t n_splats, const uint32_t* sorted_values, uint32_t n_refs,
+                         const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
+                         uint32_t width, uint32_t height, float* image);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GS3D_H */

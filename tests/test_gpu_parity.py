@@ -1,0 +1,269 @@
+"""HIP path vs the CPU oracle, through the C ABI (libgs3d.so).
+
+Integer stages (scan, emit, sort, identify) are bit-exact; preprocess is
+bit-exact against the fp32 oracle (same operation order, no contraction);
+the image is within 1/255 max-abs per channel of the fp32 oracle, and its
+error against the fp64 render is reported.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gs3d import _lib as L
+from gs3d import camera as cam_mod
+from gs3d import scene as sc
+
+pytestmark = pytest.mark.gpu
+
+TOL_IMAGE = 1.0 / 255.0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------------ scan ----
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 65_537, 1_000_000])
+def test_scan_bit_exact(renderer, n):
+    say(f"scan n={n}")
+    rng = np.random.default_rng(n)
+    counts = rng.integers(0, 64, n, dtype=np.uint32)
+    counts[rng.random(n) < 0.25] = 0
+    got, tot = renderer.stage_scan(counts)
+    exp, etot = O.scan(counts)
+    assert np.array_equal(got, exp) and tot == etot
+
+
+def test_scan_wraps(renderer):
+    counts = np.full(5000, 0x00F00000, np.uint32)
+    counts[17] = 0xFFFFFFFF
+    got, tot = renderer.stage_scan(counts)
+    exp, etot = O.scan(counts)
+    assert np.array_equal(got, exp) and tot == etot
+
+
+# ------------------------------------------------------------------ sort ----
+def make_keys(n, seed, tiles=8160, dup=0.3):
+    rng = np.random.default_rng(seed)
+    tile = rng.integers(0, tiles, n, dtype=np.uint64)
+    depth = rng.uniform(0.2, 30.0, n).astype(np.float32).view(np.uint32).astype(np.uint64)
+    keys = (tile << np.uint64(32)) | depth
+    if n:
+        idx = rng.random(n) < dup
+        keys[idx] = keys[rng.integers(0, n, idx.sum())]
+    vals = rng.permutation(n).astype(np.uint32)
+    return keys, vals
+
+
+@pytest.mark.parametrize("bits", [8, 4])
+@pytest.mark.parametrize("n", [0, 1, 63, 4095, 4096, 4097, 100_003, 3_400_000])
+def test_sort_bit_exact(renderer, n, bits):
+    say(f"sort n={n} bits={bits}")
+    keys, vals = make_keys(n, n + bits)
+    gk, gv = renderer.stage_sort(keys, vals, bits)
+    order = np.lexsort((np.arange(n), keys))
+    assert np.array_equal(gk, keys[order])
+    assert np.array_equal(gv, vals[order])
+
+
+def test_sort_full_64bit_keys(renderer):
+    rng = np.random.default_rng(77)
+    keys = rng.integers(0, 2 ** 63, 200_000, dtype=np.uint64) * np.uint64(2) + rng.integers(
+        0, 2, 200_000, dtype=np.uint64)
+    keys[::7] = keys[3]
+    vals = np.arange(keys.size, dtype=np.uint32)
+    gk, gv = renderer.stage_sort(keys, vals, 8)
+    ok, ov = O.radix_sort(keys, vals, 8)
+    assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
+
+
+# ------------------------------------------------------ emit / identify ----
+def test_emit_bit_exact(renderer):
+    from test_oracle import random_projected
+
+    for n, seed in ((1, 1), (255, 2), (256, 3), (257, 4), (20_000, 5)):
+        say(f"emit n={n}")
+        proj, counts = random_projected(n, 120, 68, seed=seed)
+        u = proj.view(np.uint32)
+        big = np.random.default_rng(seed).random(n) < 0.01  # a few very wide splats
+        u[big, 12], u[big, 13], u[big, 14], u[big, 15] = 0, 0, 119, 67
+        u[big, 11] = 1
+        counts = np.where(u[:, 11] == 1, (u[:, 14] - u[:, 12] + 1) * (u[:, 15] - u[:, 13] + 1),
+                          0).astype(np.uint32)
+        offs, _ = O.scan(counts)
+        gk, gv = renderer.stage_emit(proj, offs, 120)
+        ek, ev = O.emit(proj, offs, 120)
+        assert np.array_equal(gk, ek) and np.array_equal(gv, ev)
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "dense", "sparse", "last_tile", "gap_start"])
+def test_identify_bit_exact(renderer, case):
+    T = 8160
+    rng = np.random.default_rng(len(case))
+    tiles = {"empty": np.array([], np.int64), "one": np.array([4000]),
+             "dense": rng.integers(0, T, 300_000), "sparse": rng.choice([3, 900, 5000], 3000),
+             "last_tile": rng.choice([0, T - 1], 50), "gap_start": rng.choice([8000, 8001], 9)}[case]
+    keys = np.sort((tiles.astype(np.uint64) << np.uint64(32)) | rng.integers(
+        0, 1 << 31, tiles.size, dtype=np.uint64))
+    got = renderer.stage_identify(keys, T)
+    assert np.array_equal(got, O.identify(keys, T))
+
+
+# ------------------------------------------------------------ preprocess ----
+@pytest.mark.parametrize("deg", [0, 1, 2, 3])
+def test_preprocess_bit_exact(renderer, deg):
+    say(f"preprocess deg={deg}")
+    raw = sc.clustered(20 + deg, 40_000, 16, sh_degree=deg) if deg else sc.uniform_cube(20, 40_000)
+    s = sc.activate(raw)
+    renderer.set_splat_data(s)
+    W, H = 1280, 720
+    cu = cam_mod.view_camera(sc.orbit_eye(6.0, 30.0, 15.0) if deg else (0.0, 0.0, 3.0), W, H)
+    tx, ty = 80, 45
+    inv = float(np.float32(1 / 16))
+    gp, gc = renderer.stage_preprocess(cu, tx, ty, inv, inv)
+    ep, ec = O.preprocess(cu, s, tx, ty, inv, inv)
+    assert np.array_equal(gc, ec)
+    valid = ep.view(np.uint32)[:, 11] == 1
+    assert np.array_equal(gp.view(np.uint32)[:, 11], ep.view(np.uint32)[:, 11])
+    diff = gp.view(np.uint32)[valid] != ep.view(np.uint32)[valid]
+    assert not diff.any(), f"{diff.any(axis=1).sum()} splats differ; fields {np.nonzero(diff.any(axis=0))}"
+
+
+# --------------------------------------------------------------- raster ----
+@pytest.mark.parametrize("tile", [16, 8, 32])
+def test_rasterize_stage(renderer, tile):
+    say(f"rasterize tile={tile}")
+    raw = sc.uniform_cube(30, 60_000, log_scale_mean=-4.0)
+    s = sc.activate(raw)
+    W, H = 640, 360
+    cu = cam_mod.view_camera((0.0, 0.0, 3.0), W, H)
+    ref = O.pipeline_f32(cu, s, W, H, tile)
+    img = renderer.stage_rasterize(ref["projected"], ref["sorted_values"], ref["tile_offsets"],
+                                   tile, W, H)
+    err = np.abs(img - ref["image"]).max()
+    say(f"  max abs vs oracle f32: {err:.3g}")
+    assert err <= TOL_IMAGE
+
+
+# ------------------------------------------------------------- pipeline ----
+def rebin_and_compare(renderer, s, eye, W, H, tile=16, fov=math.pi / 3):
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(tile, tile)
+    cu = cam_mod.view_camera(eye, W, H, fov)
+    renderer.rebin(cu, W, H, L.REBIN_KEEP_UNSORTED)
+    ref = O.pipeline_f32(cu, s, W, H, tile)
+    k = renderer.ref_count()
+    assert k == ref["k"]
+    gp = renderer.read(L.BUF_PROJECTED_SPLATS, np.uint32).reshape(-1, 16)
+    valid = ref["projected"].view(np.uint32)[:, 11] == 1
+    assert np.array_equal(gp[valid], ref["projected"].view(np.uint32)[valid])
+    assert np.array_equal(renderer.read(L.BUF_SPLAT_REF_COUNTS, np.uint32), ref["counts"])
+    assert np.array_equal(renderer.read(L.BUF_SPLAT_REF_OFFSETS, np.uint32), ref["offsets"])
+    assert int(renderer.read(L.BUF_REF_COUNTER, np.uint32)[0]) == ref["k"]
+    assert np.array_equal(renderer.read(L.BUF_UNSORTED_KEYS, np.uint64), ref["keys"])
+    assert np.array_equal(renderer.read(L.BUF_UNSORTED_VALUES, np.uint32), ref["values"])
+    assert np.array_equal(renderer.read(L.BUF_SORT_KEYS, np.uint64), ref["sorted_keys"])
+    assert np.array_equal(renderer.read(L.BUF_SORT_VALUES, np.uint32), ref["sorted_values"])
+    assert np.array_equal(renderer.read(L.BUF_TILE_OFFSETS, np.uint32), ref["tile_offsets"])
+    img = renderer.read(L.BUF_IMAGE_RGBA32F, np.float32).reshape(H, W, 4)
+    err = float(np.abs(img - ref["image"]).max())
+    return img, ref, err
+
+
+def test_pipeline_uniform_100k_sh0(renderer):
+    """The 100k SH0 workload at 1280x720: every buffer vs the oracle."""
+    say("pipeline uniform-100k-sh0-720p")
+    wl = sc.workload(0)
+    s = sc.activate(wl.raw)
+    (eye, W, H) = wl.views[0]
+    img, ref, err = rebin_and_compare(renderer, s, eye, W, H)
+    f64, _, _, _ = O.render(cam_mod.view_camera(eye, W, H), s, W, H, 16, "f64")
+    err64 = float(np.abs(img - f64).max())
+    say(f"  K={ref['k']} image max abs: vs oracle f32 {err:.3g}, vs fp64 render {err64:.3g}")
+    assert err <= TOL_IMAGE
+    assert err64 <= 2 * TOL_IMAGE
+
+
+@pytest.mark.parametrize("tile", [8, 32])
+def test_pipeline_tile_sizes(renderer, tile):
+    say(f"pipeline tile={tile}")
+    s = sc.activate(sc.clustered(41, 50_000, 16, sh_degree=3))
+    _, _, err = rebin_and_compare(renderer, s, sc.orbit_eye(7.0, 45.0, 20.0), 960, 540, tile)
+    assert err <= TOL_IMAGE
+
+
+def test_pipeline_capacity_regrow(renderer):
+    """Splats that cover many tiles make K exceed the initial 8N key budget:
+    the frame is re-run with grown buffers and still matches the oracle."""
+    say("pipeline regrow")
+    s = sc.activate(sc.uniform_cube(42, 2_000, log_scale_mean=-1.5, log_scale_std=0.3))
+    img, ref, err = rebin_and_compare(renderer, s, (0.0, 0.0, 3.0), 1280, 720)
+    assert ref["k"] > 8 * 2_000
+    assert err <= TOL_IMAGE
+
+
+def test_pipeline_empty_and_all_culled(renderer):
+    s = sc.activate(sc.uniform_cube(43, 100))
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    cu = cam_mod.view_camera((0.0, 0.0, -30.0), 64, 48, target=(0.0, 0.0, -60.0))  # facing away
+    renderer.rebin(cu, 64, 48)
+    assert renderer.ref_count() == 0
+    assert not renderer.read(L.BUF_TILE_OFFSETS, np.uint32).any()
+    assert not renderer.read(L.BUF_IMAGE_RGBA32F, np.float32).any()
+    assert renderer.read_tile_counts().sum() == 0
+
+
+def test_full_size_1m_sh3_properties(renderer):
+    """1M SH3 splats at 1920x1080: size-independent properties + determinism."""
+    say("full-size clusters-1M-sh3-1080p")
+    wl = sc.workload(1)
+    s = sc.activate(wl.raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    eye, W, H = wl.views[0]
+    cu = cam_mod.view_camera(eye, W, H)
+    renderer.rebin(cu, W, H)
+    k = renderer.ref_count()
+    counts = renderer.read(L.BUF_SPLAT_REF_COUNTS, np.uint32)
+    offs = renderer.read(L.BUF_SPLAT_REF_OFFSETS, np.uint32)
+    keys = renderer.read(L.BUF_SORT_KEYS, np.uint64)
+    vals = renderer.read(L.BUF_SORT_VALUES, np.uint32)
+    toff = renderer.read(L.BUF_TILE_OFFSETS, np.uint32)
+    img1 = renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16)
+    say(f"  K={k}")
+    assert k == int(counts.sum(dtype=np.uint64)) == int(offs[-1])
+    assert np.array_equal(offs[:-1], np.concatenate([[0], np.cumsum(counts, dtype=np.uint64)[:-1]]))
+    assert np.all(keys[1:] >= keys[:-1])
+    same = keys[1:] == keys[:-1]
+    assert np.all(vals[1:][same] > vals[:-1][same])  # stable: ties in id order
+    assert np.array_equal(np.bincount(vals, minlength=s.n), counts)  # each splat k times
+    T = 120 * 68
+    assert np.array_equal(toff, O.identify(keys, T))
+    # determinism: a second rebin reproduces every byte
+    renderer.rebin(cu, W, H)
+    assert np.array_equal(renderer.read(L.BUF_SORT_KEYS, np.uint64), keys)
+    assert np.array_equal(renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16), img1)
+
+
+def test_errors_through_abi(renderer):
+    import gs3d.renderer as R
+
+    with pytest.raises(L.Gs3dError) as e:
+        renderer.set_tile_size(12, 12)
+    assert e.value.code == -6
+    fresh = R.GaussianSplatRenderer(0)
+    with pytest.raises(L.Gs3dError) as e:
+        fresh.rebin(np.zeros(72, np.float32), 64, 64)
+    assert e.value.code == -5
+    with pytest.raises(L.Gs3dError) as e:
+        fresh.read(L.BUF_SORT_KEYS, np.uint64)
+    assert e.value.code == -5
+    assert fresh.read_tile_counts() is None  # readTileCounts: null on failure
+    fresh.close()
+    lim = renderer.device_limits()
+    assert lim["maxTotalThreads"] >= 1024

@@ -1,0 +1,103 @@
+// gs3d_internal.h — shared definitions of the HIP kernels and the runtime.
+//
+// Device memory layout (all buffers owned by a gs3d_ctx, see runtime.cpp):
+//   splat_positions/scales/rotations/colors  N x float4 (the loader's layouts)
+//   splat_sh          SoA f32, coefficient-major: sh[(k*3 + ch)*N + i],
+//                     k = 0 (f_dc) .. (deg+1)^2-1; one coalesced load per
+//                     (thread, coefficient)
+//   projected_splats  N x 64 B gs3d_projected_splat
+//   splat_ref_counts  N x u32;  splat_ref_offsets (N+1) x u32
+//   sort_keys_a/b     cap x u64 (tile << 32 | bits(-depth)); values cap x u32
+//   tile_offsets      (T+1) x u32
+//   splat_target      W*H x 4 x f16 (premultiplied colour, 1 - T)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gs3d.h"
+
+namespace gs3d {
+
+constexpr int kWave = 64;
+
+// Lookback status words (scan and radix passes): [epoch:30][flag:2][value:32].
+// A word whose epoch differs from the launch's epoch reads as "not ready", so
+// the status arrays never need clearing between launches.
+constexpr uint64_t kFlagAggregate = 1ull;
+constexpr uint64_t kFlagPrefix = 2ull;
+__host__ __device__ inline uint64_t status_word(uint32_t epoch, uint64_t flag, uint32_t value) {
+    return ((uint64_t)epoch << 34) | (flag << 32) | (uint64_t)value;
+}
+
+// Scan: 256 threads x 8 counts per thread.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Radix: 256 threads (4 waves) x 16 keys per thread.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kMaxPasses = 16;
+
+// Counters buffer slots (u32), cleared at the start of every rebin.
+constexpr int kCtrScanTicket = 0;
+constexpr int kCtrSortTicket = 1;   // + pass index (up to kMaxPasses)
+constexpr int kCtrWords = 64;
+
+struct PreprocessArgs {
+    gs3d_camera cam;
+    const float4* pos;
+    const float4* scale;
+    const float4* rot;
+    const float4* color;
+    const float* sh;  // SoA, see above; null when sh_degree == 0
+    gs3d_projected_splat* out;
+    uint32_t* counts;
+    uint32_t n;
+    int sh_degree;
+    uint32_t tiles_x, tiles_y;
+    float inv_tile_x, inv_tile_y;
+};
+
+struct EmitArgs {
+    const gs3d_projected_splat* projected;
+    const uint32_t* offsets;  // n + 1 entries
+    uint64_t* keys;
+    uint32_t* values;
+    uint32_t* hist;           // [passes][radix] digit histograms, or null
+    uint32_t n;
+    uint32_t tiles_x;
+    uint64_t capacity;        // keys/values entries allocated
+    uint32_t n_passes;
+    uint32_t digit_bits;
+    uint32_t shifts[kMaxPasses];
+};
+
+// ---- launchers (each enqueues on `stream`, returns the HIP status) --------
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+hipError_t launch_sh_relayout(const float* f_dc, const float* rest, float* sh, uint32_t n, int deg,
+                              hipStream_t s);
+hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
+hipError_t launch_scan(const uint32_t* counts, uint32_t* offsets, uint32_t* ref_counter,
+                       uint64_t* status, uint32_t* ticket, uint32_t n, uint32_t epoch, hipStream_t s);
+hipError_t launch_emit(const EmitArgs& a, hipStream_t s);
+hipError_t launch_digit_histogram(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
+                                  uint32_t* hist, uint32_t digit_bits, uint32_t n_passes,
+                                  const uint32_t* shifts, hipStream_t s);
+// One stable LSD pass over n = *n_dev keys (grid covers n_max; n > n_max: no-op).
+hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                             uint32_t* vout, const uint32_t* n_dev, uint32_t n_max,
+                             const uint32_t* hist_pass, uint64_t* status, uint32_t* ticket,
+                             uint32_t shift, uint32_t digit_bits, uint32_t epoch, hipStream_t s);
+// tile_offsets[0..tile_count] from n = *n_dev sorted keys (n > n_max: all 0).
+hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
+                                 uint32_t* tile_offsets, uint32_t tile_count, hipStream_t s);
+hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_t* values,
+                            const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
+                            uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
+                            uint16_t* image_rgba16f, hipStream_t s);
+hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
+
+}  // namespace gs3d

@@ -1,0 +1,245 @@
+// k_preprocess.hip — per-splat EWA projection, SH colour and tile-overlap count.
+//
+// Follows preprocess_splats.wgsl:75-184 (reference repo, src/shaders/compute/):
+// clip/NDC centre cull, screen mapping, cov3d = (R S)(R S)^T, J W Sigma W^T J^T
+// without a frustum clamp, +0.3 low-pass on the diagonal, det <= 0 cull,
+// conic = (c, -b, a)/det, radius = max(2, ceil(3 sqrt(lambda_max))), tile rect
+// clamped to the grid, count = w*h. Colour: degree 0 takes the loader's
+// activated colour (the reference behaviour); degree 1-3 evaluates the real SH
+// basis at normalize(pos - camera.position) and clamps 0.5 + sum to [0, 1].
+//
+// One thread per splat; HBM-bound: 64 B of vec4 input (+ 4*(deg+1)^2*3 B of
+// SH, coefficient-major SoA so every load is coalesced) in, 64 + 4 B out.
+// FMA contraction is off in this file so the float results equal the CPU
+// restatement operation for operation (same association as the WGSL).
+#include "gs3d_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gs3d {
+
+namespace {
+
+constexpr float kC0 = 0.28209479177387814f;
+constexpr float kC1 = 0.4886025119029199f;
+constexpr float kC2_0 = 1.0925484305920792f;
+constexpr float kC2_1 = -1.0925484305920792f;
+constexpr float kC2_2 = 0.31539156525252005f;
+constexpr float kC2_3 = -1.0925484305920792f;
+constexpr float kC2_4 = 0.5462742152960396f;
+constexpr float kC3_0 = -0.5900435899266435f;
+constexpr float kC3_1 = 2.890611442640554f;
+constexpr float kC3_2 = -0.4570457994644658f;
+constexpr float kC3_3 = 0.3731763325901154f;
+constexpr float kC3_4 = -0.4570457994644658f;
+constexpr float kC3_5 = 1.445305721320277f;
+constexpr float kC3_6 = -0.5900435899266435f;
+
+// Basis values b[0..15] at unit direction (x, y, z) for degree DEG.
+template <int DEG>
+__device__ __forceinline__ void sh_basis(float x, float y, float z, float* b) {
+    b[0] = kC0;
+    if (DEG > 0) {
+        b[1] = -kC1 * y;
+        b[2] = kC1 * z;
+        b[3] = -kC1 * x;
+    }
+    if (DEG > 1) {
+        float xx = x * x, yy = y * y, zz = z * z;
+        float xy = x * y, yz = y * z, xz = x * z;
+        b[4] = kC2_0 * xy;
+        b[5] = kC2_1 * yz;
+        b[6] = kC2_2 * (2.0f * zz - xx - yy);
+        b[7] = kC2_3 * xz;
+        b[8] = kC2_4 * (xx - yy);
+        if (DEG > 2) {
+            b[9] = kC3_0 * y * (3.0f * xx - yy);
+            b[10] = kC3_1 * xy * z;
+            b[11] = kC3_2 * y * (4.0f * zz - xx - yy);
+            b[12] = kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+            b[13] = kC3_4 * x * (4.0f * zz - xx - yy);
+            b[14] = kC3_5 * z * (xx - yy);
+            b[15] = kC3_6 * x * (xx - 3.0f * yy);
+        }
+    }
+}
+
+template <int DEG>
+__device__ __forceinline__ void sh_color(const PreprocessArgs& a, uint32_t i, float4 p, float* rgb) {
+    float dx = p.x - a.cam.position[0];
+    float dy = p.y - a.cam.position[1];
+    float dz = p.z - a.cam.position[2];
+    float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    float inv = 1.0f / len;
+    float b[16];
+    sh_basis<DEG>(dx * inv, dy * inv, dz * inv, b);
+    constexpr int kCoef = (DEG + 1) * (DEG + 1);
+    const size_t n = a.n;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        float acc = b[0] * a.sh[(size_t)ch * n + i];
+#pragma unroll
+        for (int k = 1; k < kCoef; k++) acc = acc + b[k] * a.sh[(size_t)(k * 3 + ch) * n + i];
+        float v = acc + 0.5f;
+        rgb[ch] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    }
+}
+
+template <int DEG>
+__global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.n) return;
+    float4* out = reinterpret_cast<float4*>(a.out + i);
+
+    const float4 p = a.pos[i];
+    const float* V = a.cam.view;
+    const float* P = a.cam.projection;
+    float vp[4], cp[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) vp[r] = V[r] * p.x + V[4 + r] * p.y + V[8 + r] * p.z + V[12 + r] * 1.0f;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        cp[r] = P[r] * vp[0] + P[4 + r] * vp[1] + P[8 + r] * vp[2] + P[12 + r] * vp[3];
+
+    bool ok = cp[3] > 0.0f;
+    float ndc0 = cp[0] / cp[3], ndc1 = cp[1] / cp[3], ndc2 = cp[2] / cp[3];
+    ok = ok && !(ndc0 < -1.0f || ndc0 > 1.0f || ndc1 < -1.0f || ndc1 > 1.0f || ndc2 < 0.0f ||
+                 ndc2 > 1.0f);
+    const float W = a.cam.viewport[0], H = a.cam.viewport[1];
+    uint32_t count = 0;
+    float4 o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0, o3 = o0;
+    if (ok) {
+        const float sx = (ndc0 * 0.5f + 0.5f) * W;
+        const float sy = (1.0f - (ndc1 * 0.5f + 0.5f)) * H;
+
+        const float4 q = a.rot[i];
+        const float4 s = a.scale[i];
+        const float qw = q.x, qx = q.y, qy = q.z, qz = q.w;
+        const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
+        const float xy = qx * qy, xz = qx * qz, yz = qy * qz;
+        const float wx = qw * qx, wy = qw * qy, wz = qw * qz;
+        // R[col][row], columns as in quat_to_mat3 (:60-64)
+        const float R[3][3] = {
+            {1.0f - 2.0f * (yy + zz), 2.0f * (xy + wz), 2.0f * (xz - wy)},
+            {2.0f * (xy - wz), 1.0f - 2.0f * (xx + zz), 2.0f * (yz + wx)},
+            {2.0f * (xz + wy), 2.0f * (yz - wx), 1.0f - 2.0f * (xx + yy)},
+        };
+        const float sc[3] = {s.x, s.y, s.z};
+        float M[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) M[c][r] = R[c][r] * sc[c];
+        float S[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) S[c][r] = M[0][r] * M[0][c] + M[1][r] * M[1][c] + M[2][r] * M[2][c];
+
+        const float fx = P[0] * W * 0.5f;
+        const float fy = P[5] * H * 0.5f;
+        const float zc = -vp[2];
+        const float iz = 1.0f / zc;
+        const float iz2 = iz * iz;
+        const float J[3][3] = {
+            {fx * iz, 0.0f, 0.0f},
+            {0.0f, -fy * iz, 0.0f},
+            {fx * vp[0] * iz2, -fy * vp[1] * iz2, 0.0f},
+        };
+        float T[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                T[c][r] = J[0][r] * V[c * 4 + 0] + J[1][r] * V[c * 4 + 1] + J[2][r] * V[c * 4 + 2];
+        float TS[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) TS[c][r] = T[0][r] * S[c][0] + T[1][r] * S[c][1] + T[2][r] * S[c][2];
+        const float c00 = TS[0][0] * T[0][0] + TS[1][0] * T[1][0] + TS[2][0] * T[2][0];
+        const float c01 = TS[0][1] * T[0][0] + TS[1][1] * T[1][0] + TS[2][1] * T[2][0];
+        const float c11 = TS[0][1] * T[0][1] + TS[1][1] * T[1][1] + TS[2][1] * T[2][1];
+        const float ca = c00 + 0.3f;
+        const float cb = c01;
+        const float cc = c11 + 0.3f;
+        const float det = ca * cc - cb * cb;
+        if (det > 0.0f) {
+            const float idet = 1.0f / det;
+            const float mid = 0.5f * (ca + cc);
+            const float disc = mid * mid - det;
+            const float lam = mid + sqrtf(disc > 0.1f ? disc : 0.1f);
+            float rad = ceilf(3.0f * sqrtf(lam));
+            if (rad < 2.0f) rad = 2.0f;
+            float minx = sx - rad, miny = sy - rad;
+            minx = minx < 0.0f ? 0.0f : minx;
+            miny = miny < 0.0f ? 0.0f : miny;
+            float maxx = sx + rad, maxy = sy + rad;
+            maxx = maxx > W - 1.0f ? W - 1.0f : maxx;
+            maxy = maxy > H - 1.0f ? H - 1.0f : maxy;
+            uint32_t t0x = (uint32_t)floorf(minx * a.inv_tile_x);
+            uint32_t t0y = (uint32_t)floorf(miny * a.inv_tile_y);
+            uint32_t t1x = (uint32_t)floorf(maxx * a.inv_tile_x);
+            uint32_t t1y = (uint32_t)floorf(maxy * a.inv_tile_y);
+            t0x = min(t0x, a.tiles_x - 1u);
+            t0y = min(t0y, a.tiles_y - 1u);
+            t1x = min(t1x, a.tiles_x - 1u);
+            t1y = min(t1y, a.tiles_y - 1u);
+            if (t0x <= t1x && t0y <= t1y) {
+                count = (t1x - t0x + 1u) * (t1y - t0y + 1u);
+                const float4 col = a.color[i];
+                float rgb[3] = {col.x, col.y, col.z};
+                if (DEG > 0) sh_color<DEG>(a, i, p, rgb);
+                o0 = make_float4(sx, sy, vp[2], rad);
+                o1 = make_float4(cc * idet, -cb * idet, ca * idet, col.w);
+                o2 = make_float4(rgb[0], rgb[1], rgb[2], __uint_as_float(1u));
+                o3 = make_float4(__uint_as_float(t0x), __uint_as_float(t0y), __uint_as_float(t1x),
+                                 __uint_as_float(t1y));
+            }
+        }
+    }
+    out[0] = o0;
+    out[1] = o1;
+    out[2] = o2;
+    out[3] = o3;
+    a.counts[i] = count;
+}
+
+// f_dc (N x 3) + rest (N x 3R, channel-major per splat) -> coefficient-major SoA.
+__global__ __launch_bounds__(256) void k_sh_relayout(const float* f_dc, const float* rest, float* sh,
+                                                     uint32_t n, int ncoef) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const int R = ncoef - 1;
+    for (int ch = 0; ch < 3; ch++) {
+        sh[(size_t)ch * n + i] = f_dc[(size_t)i * 3 + ch];
+        for (int k = 1; k < ncoef; k++)
+            sh[(size_t)(k * 3 + ch) * n + i] = rest[(size_t)i * 3 * R + (size_t)ch * R + (k - 1)];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const uint32_t grid = (a.n + 255u) / 256u;
+    switch (a.sh_degree) {
+        case 0: hipLaunchKernelGGL(k_preprocess<0>, dim3(grid), dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_preprocess<1>, dim3(grid), dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_preprocess<2>, dim3(grid), dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_preprocess<3>, dim3(grid), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sh_relayout(const float* f_dc, const float* rest, float* sh, uint32_t n, int deg,
+                              hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const int ncoef = (deg + 1) * (deg + 1);
+    hipLaunchKernelGGL(k_sh_relayout, dim3((n + 255u) / 256u), dim3(256), 0, s, f_dc, rest, sh, n,
+                       ncoef);
+    return hipGetLastError();
+}
+
+}  // namespace gs3d

@@ -1,0 +1,202 @@
+// k_sort.hip — one stable LSD radix pass over (u64 key, u32 value) pairs.
+//
+// The reference sorts with 8 passes x {per-workgroup LDS-atomic histogram,
+// recursive Blelloch scan of a digit-major table, lane-0 serial ranking +
+// scatter} (radix_histogram.wgsl:35-61, scan_local.wgsl, scan_add.wgsl,
+// radix_scatter.wgsl:49-88; GaussianSplatRenderer.ts:1216-1247). Each pass is a
+// stable counting sort on one digit, so the final order is (key, input index).
+//
+// Here a pass is ONE kernel (a "onesweep" pass):
+//   1. a ticket gives the workgroup its 4096-key tile in launch order;
+//   2. each wavefront ranks its 1024 keys with a wave-level multisplit: per
+//      digit bit a 64-lane ballot narrows the mask of lanes holding the same
+//      digit; popcount below the lane = stable rank, the lowest lane bumps the
+//      wave's private LDS histogram;
+//   3. per digit: the tile's count is published as an aggregate, the workgroup
+//      looks back over predecessor tiles' status words (decoupled lookback)
+//      for its exclusive prefix, then publishes the inclusive prefix; the
+//      digit's global start comes from the digit histograms fused into emit;
+//   4. keys/values are placed digit-sorted in LDS, then written out in LDS
+//      order so runs of equal digits leave as contiguous stores.
+// HBM traffic per pass: 12 B read + 12 B written per key. Tiles beyond
+// n = *n_dev exit at once, so a graph-captured launch can size the grid to the
+// buffer capacity while the key count lives on the device.
+#include "gs3d_internal.h"
+
+namespace gs3d {
+
+namespace {
+
+__device__ __forceinline__ uint64_t load_status(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan over the 256 threads of the block; all threads must call.
+__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t x, uint32_t* s_tmp) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    uint32_t v = x;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, kWave);
+        if (lane >= d) v += t;
+    }
+    if (lane == kWave - 1) s_tmp[wave] = v;
+    __syncthreads();
+    uint32_t prefix = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / kWave; w++)
+        if (w < wave) prefix += s_tmp[w];
+    __syncthreads();
+    return prefix + v - x;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_radix_pass(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
+    const uint32_t* __restrict__ hist_pass, uint64_t* status, uint32_t* ticket, uint32_t n_max,
+    uint32_t shift, uint32_t epoch) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int WAVES = kSortThreads / kWave;
+    constexpr uint32_t MASK = RADIX - 1;
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_wave_hist[WAVES][RADIX];
+    __shared__ uint32_t s_bin_local[RADIX];
+    __shared__ uint32_t s_bin_global[RADIX];
+    __shared__ uint32_t s_tmp[WAVES];
+    __shared__ uint32_t s_part;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    if (tid == 0) s_part = atomicAdd(ticket, 1u);
+    for (int k = tid; k < WAVES * RADIX; k += kSortThreads) (&s_wave_hist[0][0])[k] = 0u;
+    __syncthreads();
+    const uint32_t part = s_part;
+    const uint32_t n = *n_dev;
+    const size_t tile_base = (size_t)part * kSortTile;
+    // uniform over the block: past the keys, or more keys than the buffers hold
+    // (the host re-runs the frame with larger buffers in that case)
+    if (tile_base >= n || n > n_max) return;
+
+    const size_t wave_base = tile_base + (size_t)wave * (kWave * kSortItems);
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems];
+    uint32_t rank[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const size_t idx = wave_base + (size_t)it * kWave + lane;
+        k[it] = idx < n ? kin[idx] : ~0ull;
+    }
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const size_t idx = wave_base + (size_t)it * kWave + lane;
+        v[it] = idx < n ? vin[idx] : 0u;
+    }
+
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const size_t idx = wave_base + (size_t)it * kWave + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < BITS; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t r = 0;
+        if (valid) {
+            const uint64_t below = peers & lt_mask;
+            const uint32_t prior = s_wave_hist[wave][d];
+            r = prior + (uint32_t)__popcll(below);
+            if (below == 0ull) s_wave_hist[wave][d] = prior + (uint32_t)__popcll(peers);
+        }
+        rank[it] = r;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // Per digit: counts across waves -> wave offsets, tile count.
+    uint32_t tile_count = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) {
+            const uint32_t c = s_wave_hist[w][tid];
+            s_wave_hist[w][tid] = tile_count;
+            tile_count += c;
+        }
+        store_status(&status[(size_t)part * RADIX + tid],
+                     status_word(epoch, part == 0 ? kFlagPrefix : kFlagAggregate, tile_count));
+    }
+    const uint32_t local_start = block_exclusive_scan_256(tid < RADIX ? tile_count : 0u, s_tmp);
+    const uint32_t global_start = block_exclusive_scan_256(tid < RADIX ? hist_pass[tid] : 0u, s_tmp);
+    if (tid < RADIX) {
+        s_bin_local[tid] = local_start;
+        uint32_t excl = 0;
+        for (int64_t p = (int64_t)part - 1; p >= 0; p--) {
+            uint64_t w;
+            uint32_t flag;
+            do {
+                w = load_status(&status[(size_t)p * RADIX + tid]);
+                flag = ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
+            } while (flag == 0);
+            excl += (uint32_t)w;
+            if (flag == kFlagPrefix) break;
+        }
+        if (part != 0)
+            store_status(&status[(size_t)part * RADIX + tid],
+                         status_word(epoch, kFlagPrefix, excl + tile_count));
+        s_bin_global[tid] = global_start + excl;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const size_t idx = wave_base + (size_t)it * kWave + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
+            const uint32_t pos = s_bin_local[d] + s_wave_hist[wave][d] + rank[it];
+            s_keys[pos] = k[it];
+            s_vals[pos] = v[it];
+        }
+    }
+    __syncthreads();
+    const uint32_t tile_n = (uint32_t)min((size_t)kSortTile, (size_t)n - tile_base);
+    for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
+        const uint64_t key = s_keys[j];
+        const uint32_t d = (uint32_t)(key >> shift) & MASK;
+        const uint32_t g = s_bin_global[d] + (j - s_bin_local[d]);
+        kout[g] = key;
+        vout[g] = s_vals[j];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                             uint32_t* vout, const uint32_t* n_dev, uint32_t n_max,
+                             const uint32_t* hist_pass, uint64_t* status, uint32_t* ticket,
+                             uint32_t shift, uint32_t digit_bits, uint32_t epoch, hipStream_t s) {
+    if (n_max == 0) return hipSuccess;
+    const uint32_t grid = (n_max + kSortTile - 1) / kSortTile;
+    if (digit_bits == 8) {
+        hipLaunchKernelGGL(k_radix_pass<8>, dim3(grid), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
+                           n_dev, hist_pass, status, ticket, n_max, shift, epoch);
+    } else if (digit_bits == 4) {
+        hipLaunchKernelGGL(k_radix_pass<4>, dim3(grid), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
+                           n_dev, hist_pass, status, ticket, n_max, shift, epoch);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace gs3d

@@ -1,0 +1,853 @@
+// runtime.cpp — context, named device buffers, stage profiler and the C ABI.
+//
+// A gs3d_ctx is the native counterpart of the reference renderer's resource
+// services: BufferManager's named buffers (src/BufferManager.ts:32-154; here a
+// name -> device allocation map that only grows, freeing the old block), the
+// Profiler's per-pass timestamps (src/Profiler.ts:135-228; here hipEvent pairs
+// per stage on the context's stream) and GaussianSplatRenderer.rebin
+// (src/renderers/GaussianSplatRenderer.ts:1104-1266), which queues the whole
+// pipeline on one HIP stream with no host round trip inside the frame.
+//
+// Key count K lives on the device (ref_counter); the sort, identify and
+// raster launches read it there and size their grids from the key-buffer
+// capacity. At the end of a frame K is copied to pinned host memory; when it
+// exceeded the capacity (the emit kernel counts but does not store past it),
+// the buffers grow and the frame is re-run before any result is handed out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gs3d_internal.h"
+
+using namespace gs3d;
+
+namespace {
+
+constexpr int kCtrRefCounter = 32;
+constexpr int kCtrHist = 64;  // [kMaxPasses][256] digit histograms follow the counters
+constexpr size_t kCtrTotalWords = kCtrHist + kMaxPasses * 256;
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+const char* kStageLabels[GS3D_STAGE__COUNT] = {
+    "preprocess-splats", "prefix-scan", "emit-tile-refs", "radix-sort",
+    "identify-tile-ranges", "rasterize-tiles", "rebin-total"};
+
+}  // namespace
+
+struct gs3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::unordered_map<std::string, DevBuf> bufs;
+    uint32_t limits[6] = {0, 0, 0, 0, 0, 0};
+
+    // scene
+    uint32_t n = 0;
+    int sh_degree = 0;
+
+    // renderer state
+    uint32_t tile_x = 16, tile_y = 16;
+    uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
+    uint32_t digit_bits = 8;
+    uint64_t key_capacity = 0;
+    uint32_t n_passes = 0;
+    uint32_t shifts[kMaxPasses] = {};
+    bool result_in_b = false;
+    bool keep_unsorted = false;
+    uint32_t epoch = 0;
+
+    // last frame (for re-runs after a capacity miss)
+    bool have_frame = false;
+    gs3d_camera last_cam{};
+    uint32_t last_w = 0, last_h = 0, last_flags = 0;
+    uint32_t* pinned = nullptr;  // [0] = K of the last frame
+    hipEvent_t k_ready = nullptr;
+    bool k_pending = false;
+    uint64_t K = 0;
+
+    // profiler: a ring of event pairs, one slot per profiled frame; a slot is
+    // folded into the running sums before reuse and when averages are read.
+    static constexpr int kRing = 64;
+    hipEvent_t ring[kRing][GS3D_STAGE__COUNT][2] = {};
+    bool ring_unread[kRing] = {};
+    uint64_t frame_counter = 0;
+    int cur_slot = 0;
+    bool profiled = false;
+    bool last_profiled = false;
+    double stage_sum_us[GS3D_STAGE__COUNT] = {};
+    uint64_t stage_frames = 0;
+};
+
+namespace {
+
+int fail(gs3d_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? GS3D_E_OOM : GS3D_E_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+#define GS3D_TRY(expr)          \
+    do {                        \
+        int r_ = (expr);        \
+        if (r_ != GS3D_OK) return r_; \
+    } while (0)
+
+// Lookback epochs are process-wide, so a status block handed from one
+// context to another can never hold a word of a live epoch.
+std::atomic<uint32_t> g_epoch{0};
+
+uint32_t next_epoch(gs3d_ctx* c) {
+    uint32_t e;
+    do {
+        e = (g_epoch.fetch_add(1u) + 1u) & 0x3FFFFFFFu;
+    } while (e == 0);
+    c->epoch = e;
+    return e;
+}
+
+// BufferManager.resize semantics with reuse: grow-only, old block freed.
+int ensure(gs3d_ctx* c, const char* name, size_t bytes, void** out) {
+    DevBuf& b = c->bufs[name];
+    if (bytes == 0) bytes = 4;
+    if (b.cap < bytes) {
+        if (b.ptr) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            HIP_TRY(c, hipFree(b.ptr));
+            b.ptr = nullptr;
+            b.cap = 0;
+        }
+        hipError_t e = hipMalloc(&b.ptr, bytes);
+        if (e != hipSuccess)
+            return fail(c, GS3D_E_OOM, "allocating %zu bytes for '%s': %s", bytes, name,
+                        hipGetErrorString(e));
+        b.cap = bytes;
+    }
+    *out = b.ptr;
+    return GS3D_OK;
+}
+
+template <class T>
+int ensure_t(gs3d_ctx* c, const char* name, size_t count, T** out) {
+    void* p = nullptr;
+    GS3D_TRY(ensure(c, name, count * sizeof(T), &p));
+    *out = static_cast<T*>(p);
+    return GS3D_OK;
+}
+
+// Lookback status words: zeroed whenever the block is (re)allocated, so
+// nothing left in recycled device memory can read as a published status.
+int ensure_status(gs3d_ctx* c, const char* name, size_t count, uint64_t** out) {
+    const size_t before = c->bufs[name].cap;
+    GS3D_TRY(ensure_t(c, name, count, out));
+    if (c->bufs[name].cap != before) {
+        HIP_TRY(c, hipMemsetAsync(*out, 0, c->bufs[name].cap, c->stream));
+    }
+    return GS3D_OK;
+}
+
+template <class T>
+T* buf(gs3d_ctx* c, const char* name) {
+    auto it = c->bufs.find(name);
+    return it == c->bufs.end() ? nullptr : static_cast<T*>(it->second.ptr);
+}
+
+uint32_t bit_length(uint32_t v) {
+    uint32_t b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+// Radix passes for keys whose tile id is < tile_count: the 32 depth bits
+// always, the tile bits up to bit_length(tile_count - 1). Passes over digits
+// that are zero for every key are result-identical no-ops and are not run.
+void plan_passes(gs3d_ctx* c, uint32_t tile_count, uint32_t bits) {
+    const uint32_t tile_bits = tile_count > 1 ? bit_length(tile_count - 1) : 0;
+    c->n_passes = 0;
+    for (uint32_t s = 0; s < 32u + tile_bits; s += bits) c->shifts[c->n_passes++] = s;
+}
+
+void record(gs3d_ctx* c, int stage, int which) {
+    if (c->profiled) (void)hipEventRecord(c->ring[c->cur_slot][stage][which], c->stream);
+}
+
+// Add a completed profiled frame's stage times to the running sums.
+int fold_slot(gs3d_ctx* c, int slot) {
+    if (!c->ring_unread[slot]) return GS3D_OK;
+    HIP_TRY(c, hipEventSynchronize(c->ring[slot][GS3D_STAGE_TOTAL][1]));
+    for (int s = 0; s < GS3D_STAGE__COUNT; s++) {
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[slot][s][0], c->ring[slot][s][1]));
+        c->stage_sum_us[s] += (double)ms * 1000.0;
+    }
+    c->stage_frames++;
+    c->ring_unread[slot] = false;
+    return GS3D_OK;
+}
+
+int begin_profiled_frame(gs3d_ctx* c) {
+    if (!c->profiled) return GS3D_OK;
+    c->cur_slot = (int)(c->frame_counter++ % gs3d_ctx::kRing);
+    GS3D_TRY(fold_slot(c, c->cur_slot));  // 64 frames old: long finished
+    return GS3D_OK;
+}
+
+int set_device(gs3d_ctx* c) {
+    HIP_TRY(c, hipSetDevice(c->device));
+    return GS3D_OK;
+}
+
+// Sort n = *n_dev keys (grid sized for n_max) through the planned passes.
+// Sets *in_b when the sorted result ends in the b buffers.
+int run_sort(gs3d_ctx* c, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb,
+             const uint32_t* n_dev, uint64_t n_max, uint32_t* counters, bool* in_b) {
+    uint64_t* radix_status = nullptr;
+    const uint64_t tiles = (n_max + kSortTile - 1) / kSortTile + 1;
+    GS3D_TRY(ensure_status(c, "radix_status", tiles * (1u << c->digit_bits), &radix_status));
+    const uint32_t radix = 1u << c->digit_bits;
+    for (uint32_t p = 0; p < c->n_passes; p++) {
+        const bool even = (p % 2) == 0;
+        HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va,
+                                     n_dev, (uint32_t)n_max, counters + kCtrHist + p * radix,
+                                     radix_status, counters + kCtrSortTicket + p, c->shifts[p],
+                                     c->digit_bits, next_epoch(c), c->stream));
+    }
+    *in_b = (c->n_passes % 2) == 1;
+    return GS3D_OK;
+}
+
+int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, uint32_t flags) {
+    const uint32_t n = c->n;
+    c->width = W;
+    c->height = H;
+    c->tiles_x = (W + c->tile_x - 1) / c->tile_x;
+    c->tiles_y = (H + c->tile_y - 1) / c->tile_y;
+    const uint32_t T = c->tiles_x * c->tiles_y;
+    c->profiled = (flags & GS3D_REBIN_PROFILE) != 0;
+    c->keep_unsorted = (flags & GS3D_REBIN_KEEP_UNSORTED) != 0;
+    GS3D_TRY(begin_profiled_frame(c));
+    if (c->key_capacity == 0) c->key_capacity = std::max<uint64_t>(8ull * n, 1u << 16);
+    const uint64_t cap = c->key_capacity;
+
+    gs3d_projected_splat* projected;
+    uint32_t *counts, *offsets, *counters, *tile_offsets, *va, *vb;
+    uint64_t *scan_status, *ka, *kb;
+    uint16_t* image;
+    GS3D_TRY(ensure_t(c, "projected_splats", n, &projected));
+    GS3D_TRY(ensure_t(c, "splat_ref_counts", n, &counts));
+    GS3D_TRY(ensure_t(c, "splat_ref_offsets", (size_t)n + 1, &offsets));
+    GS3D_TRY(ensure_t(c, "counters", kCtrTotalWords, &counters));
+    GS3D_TRY(ensure_status(c, "scan_status", (size_t)n / kScanTile + 2, &scan_status));
+    GS3D_TRY(ensure_t(c, "sort_keys_a", cap, &ka));
+    GS3D_TRY(ensure_t(c, "sort_keys_b", cap, &kb));
+    GS3D_TRY(ensure_t(c, "sort_values_a", cap, &va));
+    GS3D_TRY(ensure_t(c, "sort_values_b", cap, &vb));
+    GS3D_TRY(ensure_t(c, "tile_offsets", (size_t)T + 1, &tile_offsets));
+    GS3D_TRY(ensure_t(c, "splat_target", (size_t)W * H * 4, &image));
+    plan_passes(c, T, c->digit_bits);
+    uint32_t* ref_counter = counters + kCtrRefCounter;
+
+    record(c, GS3D_STAGE_TOTAL, 0);
+    record(c, GS3D_STAGE_PREPROCESS, 0);
+    HIP_TRY(c, launch_fill_u32(counters, 0u, kCtrTotalWords, c->stream));
+    PreprocessArgs pa{};
+    pa.cam = *cam;
+    pa.pos = buf<float4>(c, "splat_positions");
+    pa.scale = buf<float4>(c, "splat_scales");
+    pa.rot = buf<float4>(c, "splat_rotations");
+    pa.color = buf<float4>(c, "splat_colors");
+    pa.sh = c->sh_degree > 0 ? buf<float>(c, "splat_sh") : nullptr;
+    pa.out = projected;
+    pa.counts = counts;
+    pa.n = n;
+    pa.sh_degree = c->sh_degree;
+    pa.tiles_x = c->tiles_x;
+    pa.tiles_y = c->tiles_y;
+    pa.inv_tile_x = (float)(1.0 / (double)c->tile_x);
+    pa.inv_tile_y = (float)(1.0 / (double)c->tile_y);
+    HIP_TRY(c, launch_preprocess(pa, c->stream));
+    record(c, GS3D_STAGE_PREPROCESS, 1);
+
+    record(c, GS3D_STAGE_SCAN, 0);
+    HIP_TRY(c, launch_scan(counts, offsets, ref_counter, scan_status, counters + kCtrScanTicket, n,
+                           next_epoch(c), c->stream));
+    record(c, GS3D_STAGE_SCAN, 1);
+
+    record(c, GS3D_STAGE_EMIT, 0);
+    EmitArgs ea{};
+    ea.projected = projected;
+    ea.offsets = offsets;
+    ea.keys = ka;
+    ea.values = va;
+    ea.hist = counters + kCtrHist;
+    ea.n = n;
+    ea.tiles_x = c->tiles_x;
+    ea.capacity = cap;
+    ea.n_passes = c->n_passes;
+    ea.digit_bits = c->digit_bits;
+    for (uint32_t p = 0; p < c->n_passes; p++) ea.shifts[p] = c->shifts[p];
+    HIP_TRY(c, launch_emit(ea, c->stream));
+    record(c, GS3D_STAGE_EMIT, 1);
+    if (c->keep_unsorted) {
+        uint64_t* uk;
+        uint32_t* uv;
+        GS3D_TRY(ensure_t(c, "unsorted_keys", cap, &uk));
+        GS3D_TRY(ensure_t(c, "unsorted_values", cap, &uv));
+        HIP_TRY(c, hipMemcpyAsync(uk, ka, cap * 8, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(uv, va, cap * 4, hipMemcpyDeviceToDevice, c->stream));
+    }
+
+    record(c, GS3D_STAGE_SORT, 0);
+    GS3D_TRY(run_sort(c, ka, va, kb, vb, ref_counter, cap, counters, &c->result_in_b));
+    record(c, GS3D_STAGE_SORT, 1);
+    const uint64_t* sorted_keys = c->result_in_b ? kb : ka;
+    const uint32_t* sorted_vals = c->result_in_b ? vb : va;
+
+    record(c, GS3D_STAGE_IDENTIFY, 0);
+    HIP_TRY(c, launch_identify_tiles(sorted_keys, ref_counter, (uint32_t)cap, tile_offsets, T, c->stream));
+    record(c, GS3D_STAGE_IDENTIFY, 1);
+
+    record(c, GS3D_STAGE_RASTER, 0);
+    HIP_TRY(c, launch_rasterize(projected, sorted_vals, tile_offsets, c->tile_x, c->tile_y,
+                                c->tiles_x, c->tiles_y, W, H, image, c->stream));
+    record(c, GS3D_STAGE_RASTER, 1);
+    record(c, GS3D_STAGE_TOTAL, 1);
+
+    HIP_TRY(c, hipMemcpyAsync(c->pinned, ref_counter, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipEventRecord(c->k_ready, c->stream));
+    c->k_pending = true;
+    if (c->profiled) c->ring_unread[c->cur_slot] = true;
+    c->last_profiled = c->profiled;
+    return GS3D_OK;
+}
+
+// Consume the last frame's K. With wait, block for it and, after a capacity
+// miss, grow the key buffers and re-run that frame so its results are whole.
+int settle(gs3d_ctx* c, bool wait) {
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (!c->k_pending) return GS3D_OK;
+        if (wait) {
+            HIP_TRY(c, hipEventSynchronize(c->k_ready));
+        } else {
+            hipError_t q = hipEventQuery(c->k_ready);
+            if (q == hipErrorNotReady) return GS3D_OK;
+            if (q != hipSuccess) return fail(c, GS3D_E_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        }
+        c->k_pending = false;
+        c->K = c->pinned[0];
+        if (c->K <= c->key_capacity) return GS3D_OK;
+        // capacity miss: grow (the next ensure() frees and reallocates)
+        c->key_capacity = c->K + c->K / 4 + kSortTile;
+        if (!wait) return GS3D_OK;  // the frame about to be queued uses the new capacity
+        GS3D_TRY(enqueue_rebin(c, &c->last_cam, c->last_w, c->last_h, c->last_flags));
+    }
+    return fail(c, GS3D_E_STATE, "key buffer did not settle after re-runs");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs3d_abi_version(void) { return GS3D_ABI_VERSION; }
+
+const char* gs3d_last_error(const gs3d_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+const char* gs3d_stage_label(int stage) {
+    return (stage >= 0 && stage < GS3D_STAGE__COUNT) ? kStageLabels[stage] : "";
+}
+
+int gs3d_create(int device, gs3d_ctx** out) {
+    if (!out) return GS3D_E_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return GS3D_E_NO_DEVICE;
+    if (device < 0 || device >= count) return GS3D_E_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GS3D_E_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return GS3D_E_NO_DEVICE;
+    gs3d_ctx* c = new gs3d_ctx();
+    c->device = device;
+    c->limits[0] = (uint32_t)prop.maxThreadsPerBlock;
+    c->limits[1] = (uint32_t)prop.maxThreadsDim[0];
+    c->limits[2] = (uint32_t)prop.maxThreadsDim[1];
+    c->limits[3] = (uint32_t)prop.maxThreadsDim[2];
+    c->limits[4] = (uint32_t)prop.maxGridSize[0];
+    c->limits[5] = (uint32_t)prop.sharedMemPerBlock;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void**)&c->pinned, 64, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->k_ready, hipEventDisableTiming) != hipSuccess) {
+        gs3d_destroy(c);
+        return GS3D_E_HIP;
+    }
+    for (int r = 0; r < gs3d_ctx::kRing; r++)
+        for (int s = 0; s < GS3D_STAGE__COUNT; s++)
+            for (int w = 0; w < 2; w++)
+                if (hipEventCreate(&c->ring[r][s][w]) != hipSuccess) {
+                    gs3d_destroy(c);
+                    return GS3D_E_HIP;
+                }
+    *out = c;
+    return GS3D_OK;
+}
+
+int gs3d_destroy(gs3d_ctx* c) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->bufs)
+        if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (int r = 0; r < gs3d_ctx::kRing; r++)
+        for (int s = 0; s < GS3D_STAGE__COUNT; s++)
+            for (int w = 0; w < 2; w++)
+                if (c->ring[r][s][w]) (void)hipEventDestroy(c->ring[r][s][w]);
+    if (c->k_ready) (void)hipEventDestroy(c->k_ready);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GS3D_OK;
+}
+
+int gs3d_sync(gs3d_ctx* c) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_device_limits(gs3d_ctx* c, uint32_t out[6]) {
+    if (!c || !out) return GS3D_E_INVALID_ARG;
+    std::memcpy(out, c->limits, sizeof c->limits);
+    return GS3D_OK;
+}
+
+int gs3d_set_splats(gs3d_ctx* c, const float* pos4, const float* scale4, const float* rot4,
+                    const float* color4, const float* f_dc, const float* sh_rest, int sh_degree,
+                    uint32_t n) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (sh_degree < 0 || sh_degree > 3) return fail(c, GS3D_E_INVALID_ARG, "sh_degree %d not in 0..3", sh_degree);
+    if (n > 0 && (!pos4 || !scale4 || !rot4 || !color4))
+        return fail(c, GS3D_E_INVALID_ARG, "null splat array");
+    if (n > 0 && sh_degree > 0 && (!f_dc || !sh_rest))
+        return fail(c, GS3D_E_INVALID_ARG, "sh_degree %d needs f_dc and sh_rest", sh_degree);
+    if ((uint64_t)n * 16 > (1ull << 40)) return fail(c, GS3D_E_INVALID_ARG, "too many splats");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    const char* names[4] = {"splat_positions", "splat_scales", "splat_rotations", "splat_colors"};
+    const float* src[4] = {pos4, scale4, rot4, color4};
+    for (int k = 0; k < 4; k++) {
+        float4* d;
+        GS3D_TRY(ensure_t(c, names[k], n, &d));
+        if (n) HIP_TRY(c, hipMemcpyAsync(d, src[k], (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    }
+    if (sh_degree > 0 && n > 0) {
+        const int ncoef = (sh_degree + 1) * (sh_degree + 1);
+        const size_t rest = (size_t)3 * (ncoef - 1);
+        float *dc_dev, *rest_dev, *sh;
+        GS3D_TRY(ensure_t(c, "staging_f_dc", (size_t)n * 3, &dc_dev));
+        GS3D_TRY(ensure_t(c, "staging_sh_rest", (size_t)n * rest, &rest_dev));
+        GS3D_TRY(ensure_t(c, "splat_sh", (size_t)n * 3 * ncoef, &sh));
+        HIP_TRY(c, hipMemcpyAsync(dc_dev, f_dc, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(rest_dev, sh_rest, (size_t)n * rest * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, launch_sh_relayout(dc_dev, rest_dev, sh, n, sh_degree, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->n = n;
+    c->sh_degree = sh_degree;
+    c->have_frame = false;
+    c->key_capacity = 0;
+    return GS3D_OK;
+}
+
+uint32_t gs3d_splat_count(const gs3d_ctx* c) { return c ? c->n : 0; }
+
+int gs3d_set_tile_size(gs3d_ctx* c, uint32_t x, uint32_t y, uint32_t out_xy[2]) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    // clampTileSize (GaussianSplatRenderer.ts:162-167) against the device limits
+    uint32_t tx = std::max<uint32_t>(1, std::min({x ? x : 1u, c->limits[1], c->limits[0]}));
+    uint32_t ty = std::max<uint32_t>(1, std::min({y ? y : 1u, c->limits[2], c->limits[0] / tx}));
+    if (!((tx == 8 && ty == 8) || (tx == 16 && ty == 16) || (tx == 32 && ty == 32)))
+        return fail(c, GS3D_E_UNKNOWN_NAME, "no rasterize-tiles variant for a %ux%u tile (8, 16, 32 square)",
+                    tx, ty);
+    GS3D_TRY(settle(c, true));
+    c->tile_x = tx;
+    c->tile_y = ty;
+    if (out_xy) {
+        out_xy[0] = tx;
+        out_xy[1] = ty;
+    }
+    return GS3D_OK;
+}
+
+int gs3d_get_tile_size(const gs3d_ctx* c, uint32_t out_xy[2]) {
+    if (!c || !out_xy) return GS3D_E_INVALID_ARG;
+    out_xy[0] = c->tile_x;
+    out_xy[1] = c->tile_y;
+    return GS3D_OK;
+}
+
+int gs3d_get_tile_count(const gs3d_ctx* c, uint32_t out_xy[2]) {
+    if (!c || !out_xy) return GS3D_E_INVALID_ARG;
+    out_xy[0] = c->tiles_x;
+    out_xy[1] = c->tiles_y;
+    return GS3D_OK;
+}
+
+int gs3d_rebin(gs3d_ctx* c, const gs3d_camera* camera, uint32_t width, uint32_t height,
+               uint32_t flags) {
+    if (!c || !camera) return GS3D_E_INVALID_ARG;
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(c, GS3D_E_INVALID_ARG, "viewport %ux%u out of range", width, height);
+    if (!c->bufs.count("splat_positions")) return fail(c, GS3D_E_STATE, "rebin before set_splats");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, false));
+    c->last_cam = *camera;
+    c->last_w = width;
+    c->last_h = height;
+    c->last_flags = flags;
+    c->have_frame = true;
+    return enqueue_rebin(c, camera, width, height, flags);
+}
+
+int gs3d_ref_count(gs3d_ctx* c, uint64_t* out) {
+    if (!c || !out) return GS3D_E_INVALID_ARG;
+    if (!c->have_frame) return fail(c, GS3D_E_STATE, "no rebin yet");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    *out = c->K;
+    return GS3D_OK;
+}
+
+int gs3d_sort_passes(gs3d_ctx* c, uint32_t* n_passes, uint32_t bit_offsets[16]) {
+    if (!c || !n_passes) return GS3D_E_INVALID_ARG;
+    *n_passes = c->n_passes;
+    if (bit_offsets)
+        for (uint32_t p = 0; p < c->n_passes; p++) bit_offsets[p] = c->shifts[p];
+    return GS3D_OK;
+}
+
+int gs3d_read(gs3d_ctx* c, int id, void* dst, size_t bytes, size_t* written) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (!c->have_frame) return fail(c, GS3D_E_STATE, "no rebin yet");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    const uint64_t K = c->K;
+    const size_t px = (size_t)c->width * c->height;
+    const void* src = nullptr;
+    size_t size = 0;
+    bool widen = false;
+    switch (id) {
+        case GS3D_BUF_PROJECTED_SPLATS: src = buf<void>(c, "projected_splats"); size = (size_t)c->n * 64; break;
+        case GS3D_BUF_SPLAT_REF_COUNTS: src = buf<void>(c, "splat_ref_counts"); size = (size_t)c->n * 4; break;
+        case GS3D_BUF_SPLAT_REF_OFFSETS: src = buf<void>(c, "splat_ref_offsets"); size = ((size_t)c->n + 1) * 4; break;
+        case GS3D_BUF_REF_COUNTER: src = buf<uint32_t>(c, "counters") + kCtrRefCounter; size = 4; break;
+        case GS3D_BUF_SORT_KEYS: src = buf<void>(c, c->result_in_b ? "sort_keys_b" : "sort_keys_a"); size = K * 8; break;
+        case GS3D_BUF_SORT_VALUES: src = buf<void>(c, c->result_in_b ? "sort_values_b" : "sort_values_a"); size = K * 4; break;
+        case GS3D_BUF_TILE_OFFSETS: src = buf<void>(c, "tile_offsets"); size = ((size_t)c->tiles_x * c->tiles_y + 1) * 4; break;
+        case GS3D_BUF_IMAGE_RGBA16F: src = buf<void>(c, "splat_target"); size = px * 8; break;
+        case GS3D_BUF_IMAGE_RGBA32F: src = buf<void>(c, "splat_target"); size = px * 16; widen = true; break;
+        case GS3D_BUF_UNSORTED_KEYS:
+            if (!c->keep_unsorted) return fail(c, GS3D_E_STATE, "rebin without GS3D_REBIN_KEEP_UNSORTED");
+            src = buf<void>(c, "unsorted_keys"); size = K * 8; break;
+        case GS3D_BUF_UNSORTED_VALUES:
+            if (!c->keep_unsorted) return fail(c, GS3D_E_STATE, "rebin without GS3D_REBIN_KEEP_UNSORTED");
+            src = buf<void>(c, "unsorted_values"); size = K * 4; break;
+        default: return fail(c, GS3D_E_UNKNOWN_NAME, "unknown buffer id %d", id);
+    }
+    if (written) *written = size;
+    if (!dst) return GS3D_OK;  // size query
+    if (bytes < size) return fail(c, GS3D_E_TOO_SMALL, "buffer %d needs %zu bytes, got %zu", id, size, bytes);
+    if (size == 0) return GS3D_OK;
+    if (widen) {
+        float* tmp;
+        GS3D_TRY(ensure_t(c, "readback_f32", px * 4, &tmp));
+        HIP_TRY(c, launch_f16_to_f32(static_cast<const uint16_t*>(src), tmp, px * 4, c->stream));
+        src = tmp;
+    }
+    HIP_TRY(c, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_read_tile_counts(gs3d_ctx* c, uint32_t* counts, size_t n, size_t* written) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    const size_t T = (size_t)c->tiles_x * c->tiles_y;
+    if (written) *written = T;
+    if (!c->have_frame) return fail(c, GS3D_E_STATE, "no rebin yet");
+    if (!counts) return GS3D_OK;
+    if (n < T) return fail(c, GS3D_E_TOO_SMALL, "need %zu counts, got %zu", T, n);
+    std::vector<uint32_t> off(T + 1);
+    GS3D_TRY(gs3d_read(c, GS3D_BUF_TILE_OFFSETS, off.data(), off.size() * 4, nullptr));
+    for (size_t t = 0; t < T; t++) counts[t] = off[t + 1] >= off[t] ? off[t + 1] - off[t] : 0u;
+    return GS3D_OK;
+}
+
+int gs3d_stage_times_us(gs3d_ctx* c, float* out, int n) {
+    if (!c || !out) return GS3D_E_INVALID_ARG;
+    if (!c->have_frame || !c->last_profiled) return fail(c, GS3D_E_STATE, "no profiled rebin yet");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++) {
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[c->cur_slot][s][0], c->ring[c->cur_slot][s][1]));
+        out[s] = ms * 1000.0f;
+    }
+    return GS3D_OK;
+}
+
+int gs3d_profiler_reset(gs3d_ctx* c) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < gs3d_ctx::kRing; r++) c->ring_unread[r] = false;
+    for (int s = 0; s < GS3D_STAGE__COUNT; s++) c->stage_sum_us[s] = 0.0;
+    c->stage_frames = 0;
+    return GS3D_OK;
+}
+
+int gs3d_stage_times_avg_us(gs3d_ctx* c, float* out, int n, uint64_t* frames) {
+    if (!c || !out) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    for (int r = 0; r < gs3d_ctx::kRing; r++) GS3D_TRY(fold_slot(c, r));
+    if (frames) *frames = c->stage_frames;
+    for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++)
+        out[s] = c->stage_frames ? (float)(c->stage_sum_us[s] / (double)c->stage_frames) : 0.0f;
+    return GS3D_OK;
+}
+
+// ---- stage entry points --------------------------------------------------
+
+int gs3d_stage_preprocess(gs3d_ctx* c, const gs3d_camera* cam, uint32_t tiles_x, uint32_t tiles_y,
+                          float inv_tile_x, float inv_tile_y, gs3d_projected_splat* projected,
+                          uint32_t* counts) {
+    if (!c || !cam || (!projected && c->n) || (!counts && c->n)) return GS3D_E_INVALID_ARG;
+    if (tiles_x == 0 || tiles_y == 0) return fail(c, GS3D_E_INVALID_ARG, "zero tile count");
+    if (!c->bufs.count("splat_positions")) return fail(c, GS3D_E_STATE, "preprocess before set_splats");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    const uint32_t n = c->n;
+    gs3d_projected_splat* dp;
+    uint32_t* dc;
+    GS3D_TRY(ensure_t(c, "stage_projected", n, &dp));
+    GS3D_TRY(ensure_t(c, "stage_counts", n, &dc));
+    PreprocessArgs pa{};
+    pa.cam = *cam;
+    pa.pos = buf<float4>(c, "splat_positions");
+    pa.scale = buf<float4>(c, "splat_scales");
+    pa.rot = buf<float4>(c, "splat_rotations");
+    pa.color = buf<float4>(c, "splat_colors");
+    pa.sh = c->sh_degree > 0 ? buf<float>(c, "splat_sh") : nullptr;
+    pa.out = dp;
+    pa.counts = dc;
+    pa.n = n;
+    pa.sh_degree = c->sh_degree;
+    pa.tiles_x = tiles_x;
+    pa.tiles_y = tiles_y;
+    pa.inv_tile_x = inv_tile_x;
+    pa.inv_tile_y = inv_tile_y;
+    HIP_TRY(c, launch_preprocess(pa, c->stream));
+    if (n) {
+        HIP_TRY(c, hipMemcpyAsync(projected, dp, (size_t)n * 64, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(counts, dc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_stage_scan(gs3d_ctx* c, const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                    uint32_t* total) {
+    if (!c || !offsets || (n && !counts)) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    uint32_t *din, *dout, *ctr;
+    uint64_t* st;
+    GS3D_TRY(ensure_t(c, "stage_counts", n, &din));
+    GS3D_TRY(ensure_t(c, "stage_offsets", (size_t)n + 1, &dout));
+    GS3D_TRY(ensure_t(c, "stage_counters", kCtrTotalWords, &ctr));
+    GS3D_TRY(ensure_status(c, "stage_scan_status", (size_t)n / kScanTile + 2, &st));
+    if (n) HIP_TRY(c, hipMemcpyAsync(din, counts, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_fill_u32(ctr, 0u, kCtrTotalWords, c->stream));
+    HIP_TRY(c, launch_scan(din, dout, ctr + kCtrRefCounter, st, ctr + kCtrScanTicket, n, next_epoch(c),
+                           c->stream));
+    HIP_TRY(c, hipMemcpyAsync(offsets, dout, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    uint32_t tot = 0;
+    HIP_TRY(c, hipMemcpyAsync(&tot, ctr + kCtrRefCounter, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (total) *total = tot;
+    return GS3D_OK;
+}
+
+int gs3d_stage_emit(gs3d_ctx* c, const gs3d_projected_splat* projected, const uint32_t* offsets,
+                    uint32_t n, uint32_t tiles_x, uint64_t* keys, uint32_t* values) {
+    if (!c || !offsets || (n && !projected)) return GS3D_E_INVALID_ARG;
+    if (tiles_x == 0) return fail(c, GS3D_E_INVALID_ARG, "zero tiles_x");
+    const uint32_t K = offsets[n];
+    if (K && (!keys || !values)) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    gs3d_projected_splat* dp;
+    uint32_t *doff, *dv;
+    uint64_t* dk;
+    GS3D_TRY(ensure_t(c, "stage_projected", n, &dp));
+    GS3D_TRY(ensure_t(c, "stage_offsets", (size_t)n + 1, &doff));
+    GS3D_TRY(ensure_t(c, "stage_keys_a", K, &dk));
+    GS3D_TRY(ensure_t(c, "stage_values_a", K, &dv));
+    if (n) HIP_TRY(c, hipMemcpyAsync(dp, projected, (size_t)n * 64, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(doff, offsets, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    EmitArgs ea{};
+    ea.projected = dp;
+    ea.offsets = doff;
+    ea.keys = dk;
+    ea.values = dv;
+    ea.hist = nullptr;
+    ea.n = n;
+    ea.tiles_x = tiles_x;
+    ea.capacity = K;
+    ea.n_passes = 0;
+    ea.digit_bits = 8;
+    HIP_TRY(c, launch_emit(ea, c->stream));
+    if (K) {
+        HIP_TRY(c, hipMemcpyAsync(keys, dk, (size_t)K * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(values, dv, (size_t)K * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_stage_sort(gs3d_ctx* c, const uint64_t* keys_in, const uint32_t* values_in, uint32_t n,
+                    uint32_t digit_bits, uint64_t* keys_out, uint32_t* values_out) {
+    if (!c || (n && (!keys_in || !values_in || !keys_out || !values_out))) return GS3D_E_INVALID_ARG;
+    if (digit_bits != 4 && digit_bits != 8)
+        return fail(c, GS3D_E_INVALID_ARG, "digit width %u not 4 or 8", digit_bits);
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    if (n == 0) return GS3D_OK;
+    uint64_t *ka, *kb;
+    uint32_t *va, *vb, *ctr, *shifts_dev;
+    GS3D_TRY(ensure_t(c, "stage_keys_a", n, &ka));
+    GS3D_TRY(ensure_t(c, "stage_keys_b", n, &kb));
+    GS3D_TRY(ensure_t(c, "stage_values_a", n, &va));
+    GS3D_TRY(ensure_t(c, "stage_values_b", n, &vb));
+    GS3D_TRY(ensure_t(c, "stage_counters", kCtrTotalWords, &ctr));
+    GS3D_TRY(ensure_t(c, "stage_shifts", kMaxPasses, &shifts_dev));
+    const uint32_t saved_bits = c->digit_bits, saved_passes = c->n_passes;
+    uint32_t saved_shifts[kMaxPasses];
+    std::memcpy(saved_shifts, c->shifts, sizeof saved_shifts);
+    c->digit_bits = digit_bits;
+    c->n_passes = 0;
+    for (uint32_t s = 0; s < 64; s += digit_bits) c->shifts[c->n_passes++] = s;
+    uint32_t host_ctr[kCtrWords] = {};
+    host_ctr[kCtrRefCounter] = n;
+    HIP_TRY(c, hipMemcpyAsync(ka, keys_in, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(va, values_in, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_fill_u32(ctr, 0u, kCtrTotalWords, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(ctr, host_ctr, sizeof host_ctr, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(shifts_dev, c->shifts, sizeof c->shifts, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_digit_histogram(ka, ctr + kCtrRefCounter, n, ctr + kCtrHist, digit_bits,
+                                      c->n_passes, shifts_dev, c->stream));
+    bool in_b = false;
+    int rc = run_sort(c, ka, va, kb, vb, ctr + kCtrRefCounter, n, ctr, &in_b);
+    c->digit_bits = saved_bits;
+    c->n_passes = saved_passes;
+    std::memcpy(c->shifts, saved_shifts, sizeof saved_shifts);
+    if (rc != GS3D_OK) return rc;
+    HIP_TRY(c, hipMemcpyAsync(keys_out, in_b ? kb : ka, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(values_out, in_b ? vb : va, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_stage_identify(gs3d_ctx* c, const uint64_t* sorted_keys, uint32_t n, uint32_t tile_count,
+                        uint32_t* tile_offsets) {
+    if (!c || !tile_offsets || (n && !sorted_keys)) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    uint64_t* dk;
+    uint32_t *dt, *ctr;
+    GS3D_TRY(ensure_t(c, "stage_keys_a", n, &dk));
+    GS3D_TRY(ensure_t(c, "stage_tile_offsets", (size_t)tile_count + 1, &dt));
+    GS3D_TRY(ensure_t(c, "stage_counters", kCtrTotalWords, &ctr));
+    uint32_t host_ctr[kCtrWords] = {};
+    host_ctr[kCtrRefCounter] = n;
+    if (n) HIP_TRY(c, hipMemcpyAsync(dk, sorted_keys, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(ctr, host_ctr, sizeof host_ctr, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_identify_tiles(dk, ctr + kCtrRefCounter, n, dt, tile_count, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(tile_offsets, dt, ((size_t)tile_count + 1) * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uint32_t n_splats,
+                         const uint32_t* sorted_values, uint32_t n_refs,
+                         const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
+                         uint32_t width, uint32_t height, float* image) {
+    if (!c || !tile_offsets || !image || (n_splats && !projected) || (n_refs && !sorted_values))
+        return GS3D_E_INVALID_ARG;
+    if (width == 0 || height == 0) return fail(c, GS3D_E_INVALID_ARG, "zero viewport");
+    if (!((tile_x == 8 && tile_y == 8) || (tile_x == 16 && tile_y == 16) || (tile_x == 32 && tile_y == 32)))
+        return fail(c, GS3D_E_UNKNOWN_NAME, "no rasterize-tiles variant for %ux%u", tile_x, tile_y);
+    const uint32_t tiles_x = (width + tile_x - 1) / tile_x, tiles_y = (height + tile_y - 1) / tile_y;
+    const size_t T = (size_t)tiles_x * tiles_y;
+    // every range a tile will read must lie inside the caller's arrays
+    for (size_t t = 0; t < T; t++) {
+        const uint32_t s = tile_offsets[t], e = tile_offsets[t + 1];
+        if (e > s && e > n_refs) return fail(c, GS3D_E_INVALID_ARG, "tile %zu range past n_refs", t);
+    }
+    for (uint32_t i = 0; i < n_refs; i++)
+        if (sorted_values[i] >= n_splats) return fail(c, GS3D_E_INVALID_ARG, "splat id %u out of range", sorted_values[i]);
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    gs3d_projected_splat* dp;
+    uint32_t *dv, *dt;
+    uint16_t* dimg;
+    float* dimg32;
+    GS3D_TRY(ensure_t(c, "stage_projected", n_splats, &dp));
+    GS3D_TRY(ensure_t(c, "stage_values_a", n_refs, &dv));
+    GS3D_TRY(ensure_t(c, "stage_tile_offsets", T + 1, &dt));
+    GS3D_TRY(ensure_t(c, "stage_image", (size_t)width * height * 4, &dimg));
+    GS3D_TRY(ensure_t(c, "stage_image_f32", (size_t)width * height * 4, &dimg32));
+    if (n_splats)
+        HIP_TRY(c, hipMemcpyAsync(dp, projected, (size_t)n_splats * 64, hipMemcpyHostToDevice, c->stream));
+    if (n_refs)
+        HIP_TRY(c, hipMemcpyAsync(dv, sorted_values, (size_t)n_refs * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dt, tile_offsets, (T + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_rasterize(dp, dv, dt, tile_x, tile_y, tiles_x, tiles_y, width, height, dimg,
+                                c->stream));
+    HIP_TRY(c, launch_f16_to_f32(dimg, dimg32, (size_t)width * height * 4, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(image, dimg32, (size_t)width * height * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+"""ctypes binding of libgs3d.so (the C ABI declared in include/gs3d.h).
+
+Loading fails loudly when the built library is missing: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgs3d.so"
+
+GS3D_OK = 0
+ERRORS = {
+    -1: "GS3D_E_INVALID_ARG", -2: "GS3D_E_NO_DEVICE", -3: "GS3D_E_HIP", -4: "GS3D_E_OOM",
+    -5: "GS3D_E_STATE", -6: "GS3D_E_UNKNOWN_NAME", -7: "GS3D_E_TOO_SMALL",
+}
+
+BUF_PROJECTED_SPLATS = 0
+BUF_SPLAT_REF_COUNTS = 1
+BUF_SPLAT_REF_OFFSETS = 2
+BUF_REF_COUNTER = 3
+BUF_SORT_KEYS = 4
+BUF_SORT_VALUES = 5
+BUF_TILE_OFFSETS = 6
+BUF_IMAGE_RGBA16F = 7
+BUF_IMAGE_RGBA32F = 8
+BUF_UNSORTED_KEYS = 9
+BUF_UNSORTED_VALUES = 10
+
+REBIN_PROFILE = 1
+REBIN_KEEP_UNSORTED = 2
+
+STAGES = ("preprocess-splats", "prefix-scan", "emit-tile-refs", "radix-sort",
+          "identify-tile-ranges", "rasterize-tiles", "rebin-total")
+
+P = C.c_void_p
+U32 = C.c_uint32
+F32 = C.c_float
+I32 = C.c_int
+SZ = C.c_size_t
+
+_PROTOS = {
+    "gs3d_abi_version": (I32, []),
+    "gs3d_create": (I32, [I32, C.POINTER(P)]),
+    "gs3d_destroy": (I32, [P]),
+    "gs3d_last_error": (C.c_char_p, [P]),
+    "gs3d_sync": (I32, [P]),
+    "gs3d_device_limits": (I32, [P, P]),
+    "gs3d_set_splats": (I32, [P, P, P, P, P, P, P, I32, U32]),
+    "gs3d_splat_count": (U32, [P]),
+    "gs3d_set_tile_size": (I32, [P, U32, U32, P]),
+    "gs3d_get_tile_size": (I32, [P, P]),
+    "gs3d_get_tile_count": (I32, [P, P]),
+    "gs3d_rebin": (I32, [P, P, U32, U32, U32]),
+    "gs3d_ref_count": (I32, [P, C.POINTER(C.c_uint64)]),
+    "gs3d_read": (I32, [P, I32, P, SZ, C.POINTER(SZ)]),
+    "gs3d_read_tile_counts": (I32, [P, P, SZ, C.POINTER(SZ)]),
+    "gs3d_stage_times_us": (I32, [P, P, I32]),
+    "gs3d_stage_label": (C.c_char_p, [I32]),
+    "gs3d_profiler_reset": (I32, [P]),
+    "gs3d_stage_times_avg_us": (I32, [P, P, I32, C.POINTER(C.c_uint64)]),
+    "gs3d_sort_passes": (I32, [P, C.POINTER(U32), P]),
+    "gs3d_stage_preprocess": (I32, [P, P, U32, U32, F32, F32, P, P]),
+    "gs3d_stage_scan": (I32, [P, P, U32, P, C.POINTER(U32)]),
+    "gs3d_stage_emit": (I32, [P, P, P, U32, U32, P, P]),
+    "gs3d_stage_sort": (I32, [P, P, P, U32, U32, P, P]),
+    "gs3d_stage_identify": (I32, [P, P, U32, U32, P]),
+    "gs3d_stage_rasterize": (I32, [P, P, U32, P, U32, P, U32, U32, U32, U32, P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libgs3d.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        l = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_PROTOS)
+
+
+class Gs3dError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def ptr(a) -> int | None:
+    """Address of a C-contiguous numpy array (None for None)."""
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data

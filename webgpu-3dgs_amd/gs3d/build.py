@@ -1,0 +1,93 @@
+"""Build libgs3d.so (HIP, gfx950), the Node N-API addon and the CPU oracle.
+
+Everything is compiled in-tree so the built files travel with the source tree:
+  webgpu-3dgs_amd/gs3d/lib/libgs3d.so        the product library (C ABI)
+  webgpu-3dgs_amd/addon/gs3d_napi.node        Node binding over the C ABI
+  oracle/liboracle_gs3d.so                    the CPU checker (tests only)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent          # webgpu-3dgs_amd/gs3d
+ROOT_PKG = PKG.parent                          # webgpu-3dgs_amd
+REPO = ROOT_PKG.parent
+CSRC = ROOT_PKG / "csrc"
+OBJ = ROOT_PKG / "_build"
+LIB = PKG / "lib" / "libgs3d.so"
+ADDON_SRC = ROOT_PKG / "addon" / "gs3d_napi.c"
+ADDON = ROOT_PKG / "addon" / "gs3d_napi.node"
+ORACLE_SRC = REPO / "oracle" / "gs3d_oracle.c"
+ORACLE_LIB = REPO / "oracle" / "liboracle_gs3d.so"
+INCLUDE = REPO / "include"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no v_pk_*_f32 generation (see the build notes in DESIGN.md).
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", f"-I{INCLUDE}", f"-I{CSRC}",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_library(jobs: int = 8) -> Path:
+    OBJ.mkdir(exist_ok=True)
+    LIB.parent.mkdir(exist_ok=True)
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+
+    def compile_one(src: Path) -> Path:
+        obj = OBJ / (src.name + ".o")
+        lang = ["-x", "hip"] if src.suffix == ".hip" else ["-x", "hip"]
+        _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(src), "-o", str(obj)])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, sources))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
+          "-std=c11", "-Wall", f"-I{INCLUDE}", "-o", str(tmp), str(ORACLE_SRC), "-lm"])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build_addon() -> Path | None:
+    """N-API addon (C, no node-gyp: N-API symbols resolve from the node binary)."""
+    if not ADDON_SRC.exists() or not Path("/usr/include/node/node_api.h").exists():
+        return None
+    tmp = ADDON.with_suffix(".node.tmp")
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-Wall", "-I/usr/include/node",
+          f"-I{INCLUDE}", "-DNAPI_VERSION=8", "-o", str(tmp), str(ADDON_SRC),
+          f"-L{LIB.parent}", "-lgs3d", "-Wl,-rpath,$ORIGIN/../gs3d/lib"])
+    os.replace(tmp, ADDON)
+    return ADDON
+
+
+def build_all() -> None:
+    build_oracle()
+    build_library()
+    build_addon()
+
+
+def clean() -> None:
+    shutil.rmtree(OBJ, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build_all()

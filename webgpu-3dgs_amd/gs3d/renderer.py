@@ -1,0 +1,200 @@
+"""Python mirror of the renderer API over the C ABI (tests and the bench use it).
+
+Method names follow ``src/renderers/GaussianSplatRenderer.ts`` (setTileSize
+:301-317, getTileCount/getTileSize :1268-1274, readTileCounts :1278-1310,
+rebin :1104-1266) and ``src/SceneSyncer.ts`` (setSplatData :25-43).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .scene import Splats
+
+
+class GaussianSplatRenderer:
+    """Owns one gs3d context (device memory + one HIP stream) on `device`."""
+
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        rc = self._lib.gs3d_create(device, C.byref(h))
+        if rc != L.GS3D_OK:
+            raise L.Gs3dError(rc, f"gs3d_create(device={device}) failed")
+        self._h = h
+        self.splat_count = 0
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        if rc != L.GS3D_OK:
+            raise L.Gs3dError(rc, self._lib.gs3d_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.gs3d_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        self._check(self._lib.gs3d_sync(self._h))
+
+    def device_limits(self) -> dict:
+        out = np.zeros(6, np.uint32)
+        self._check(self._lib.gs3d_device_limits(self._h, L.ptr(out)))
+        keys = ("maxTotalThreads", "maxSizeX", "maxSizeY", "maxSizeZ", "maxDispatch",
+                "maxSharedMemory")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    # -- scene --------------------------------------------------------------
+    def set_splat_data(self, s: Splats) -> None:
+        c = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        pos, sc, rot, col = c(s.positions), c(s.scales), c(s.rotations), c(s.colors)
+        dc, rest = c(s.f_dc), c(s.sh_rest)
+        self._keep = (pos, sc, rot, col, dc, rest)
+        self._check(self._lib.gs3d_set_splats(
+            self._h, L.ptr(pos), L.ptr(sc), L.ptr(rot), L.ptr(col),
+            L.ptr(dc) if s.sh_degree else None, L.ptr(rest) if s.sh_degree else None,
+            s.sh_degree, s.n))
+        self._keep = None
+        self.splat_count = s.n
+
+    # -- renderer API ---------------------------------------------------------
+    def set_tile_size(self, x: int, y: int) -> tuple[int, int]:
+        out = np.zeros(2, np.uint32)
+        self._check(self._lib.gs3d_set_tile_size(self._h, x, y, L.ptr(out)))
+        return int(out[0]), int(out[1])
+
+    def get_tile_size(self) -> tuple[int, int]:
+        out = np.zeros(2, np.uint32)
+        self._check(self._lib.gs3d_get_tile_size(self._h, L.ptr(out)))
+        return int(out[0]), int(out[1])
+
+    def get_tile_count(self) -> tuple[int, int]:
+        out = np.zeros(2, np.uint32)
+        self._check(self._lib.gs3d_get_tile_count(self._h, L.ptr(out)))
+        return int(out[0]), int(out[1])
+
+    def rebin(self, camera_uniform: np.ndarray, width: int, height: int, flags: int = 0) -> None:
+        cam = np.ascontiguousarray(camera_uniform, np.float32)
+        if cam.size != 72:
+            raise ValueError("camera uniform must hold 72 floats")
+        self._check(self._lib.gs3d_rebin(self._h, L.ptr(cam), width, height, flags))
+
+    def ref_count(self) -> int:
+        k = C.c_uint64()
+        self._check(self._lib.gs3d_ref_count(self._h, C.byref(k)))
+        return int(k.value)
+
+    def read(self, buffer_id: int, dtype, shape=None) -> np.ndarray:
+        size = C.c_size_t()
+        self._check(self._lib.gs3d_read(self._h, buffer_id, None, 0, C.byref(size)))
+        out = np.empty(size.value // np.dtype(dtype).itemsize, dtype)
+        self._check(self._lib.gs3d_read(self._h, buffer_id, L.ptr(out), out.nbytes, C.byref(size)))
+        return out.reshape(shape) if shape is not None else out
+
+    def read_projected(self) -> np.ndarray:
+        return self.read(L.BUF_PROJECTED_SPLATS, np.float32).reshape(-1, 16)
+
+    def read_image(self, width: int, height: int) -> np.ndarray:
+        """H x W x 4 float32 (the rgba16float target widened)."""
+        return self.read(L.BUF_IMAGE_RGBA32F, np.float32).reshape(height, width, 4)
+
+    def read_tile_counts(self) -> np.ndarray | None:
+        """Per-tile counts with the underflow guard; None on failure (readTileCounts)."""
+        n = C.c_size_t()
+        if self._lib.gs3d_read_tile_counts(self._h, None, 0, C.byref(n)) != L.GS3D_OK:
+            return None
+        out = np.empty(n.value, np.uint32)
+        if self._lib.gs3d_read_tile_counts(self._h, L.ptr(out), out.size, C.byref(n)) != L.GS3D_OK:
+            return None
+        return out
+
+    def stage_times_us(self) -> dict:
+        out = np.zeros(len(L.STAGES), np.float32)
+        self._check(self._lib.gs3d_stage_times_us(self._h, L.ptr(out), out.size))
+        return {k: float(v) for k, v in zip(L.STAGES, out)}
+
+    def profiler_reset(self) -> None:
+        self._check(self._lib.gs3d_profiler_reset(self._h))
+
+    def stage_times_avg_us(self) -> tuple[dict, int]:
+        out = np.zeros(len(L.STAGES), np.float32)
+        frames = C.c_uint64()
+        self._check(self._lib.gs3d_stage_times_avg_us(self._h, L.ptr(out), out.size,
+                                                      C.byref(frames)))
+        return {k: float(v) for k, v in zip(L.STAGES, out)}, int(frames.value)
+
+    def sort_passes(self) -> list[int]:
+        n = C.c_uint32()
+        out = np.zeros(16, np.uint32)
+        self._check(self._lib.gs3d_sort_passes(self._h, C.byref(n), L.ptr(out)))
+        return [int(v) for v in out[: n.value]]
+
+    # -- stage entry points (each reference pass fed caller data) -------------
+    def stage_preprocess(self, camera_uniform, tiles_x, tiles_y, inv_tile_x, inv_tile_y):
+        n = self.splat_count
+        proj = np.zeros((n, 16), np.float32)
+        counts = np.zeros(n, np.uint32)
+        cam = np.ascontiguousarray(camera_uniform, np.float32)
+        self._check(self._lib.gs3d_stage_preprocess(self._h, L.ptr(cam), tiles_x, tiles_y,
+                                                    inv_tile_x, inv_tile_y, L.ptr(proj),
+                                                    L.ptr(counts)))
+        return proj, counts
+
+    def stage_scan(self, counts: np.ndarray):
+        counts = np.ascontiguousarray(counts, np.uint32)
+        out = np.zeros(counts.size + 1, np.uint32)
+        total = C.c_uint32()
+        self._check(self._lib.gs3d_stage_scan(self._h, L.ptr(counts), counts.size, L.ptr(out),
+                                              C.byref(total)))
+        return out, int(total.value)
+
+    def stage_emit(self, projected: np.ndarray, offsets: np.ndarray, tiles_x: int):
+        projected = np.ascontiguousarray(projected, np.float32)
+        offsets = np.ascontiguousarray(offsets, np.uint32)
+        n = offsets.size - 1
+        k = int(offsets[n])
+        keys = np.zeros(k, np.uint64)
+        vals = np.zeros(k, np.uint32)
+        self._check(self._lib.gs3d_stage_emit(self._h, L.ptr(projected), L.ptr(offsets), n, tiles_x,
+                                              L.ptr(keys), L.ptr(vals)))
+        return keys, vals
+
+    def stage_sort(self, keys: np.ndarray, values: np.ndarray, digit_bits: int = 8):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        values = np.ascontiguousarray(values, np.uint32)
+        ko = np.zeros_like(keys)
+        vo = np.zeros_like(values)
+        self._check(self._lib.gs3d_stage_sort(self._h, L.ptr(keys), L.ptr(values), keys.size,
+                                              digit_bits, L.ptr(ko), L.ptr(vo)))
+        return ko, vo
+
+    def stage_identify(self, sorted_keys: np.ndarray, tile_count: int):
+        sorted_keys = np.ascontiguousarray(sorted_keys, np.uint64)
+        out = np.zeros(tile_count + 1, np.uint32)
+        self._check(self._lib.gs3d_stage_identify(self._h, L.ptr(sorted_keys), sorted_keys.size,
+                                                  tile_count, L.ptr(out)))
+        return out
+
+    def stage_rasterize(self, projected, sorted_values, tile_offsets, tile, width, height):
+        projected = np.ascontiguousarray(projected, np.float32)
+        sorted_values = np.ascontiguousarray(sorted_values, np.uint32)
+        tile_offsets = np.ascontiguousarray(tile_offsets, np.uint32)
+        img = np.zeros((height, width, 4), np.float32)
+        self._check(self._lib.gs3d_stage_rasterize(
+            self._h, L.ptr(projected), projected.shape[0], L.ptr(sorted_values),
+            sorted_values.size, L.ptr(tile_offsets), tile, tile, width, height, L.ptr(img)))
+        return img
