@@ -145,6 +145,17 @@ uint32_t gs3d_splat_count(const gs3d_ctx* ctx);
  * 1024 threads); other requests are rejected with GS3D_E_UNKNOWN_NAME. */
 int gs3d_set_tile_size(gs3d_ctx* ctx, uint32_t x, uint32_t y, uint32_t out_xy[2]);
 int gs3d_get_tile_size(const gs3d_ctx* ctx, uint32_t out_xy[2]);
+
+/* Compiled kernel variants (PipelineManager.create/update with codeConstants,
+ * src/PipelineManager.ts:34-105; the WGSL is specialised by text substitution,
+ * here a precompiled variant is selected):
+ *   "rasterize-tiles" "PIXELS_PER_LANE"  1, 2, 4 (workgroup tile^2/ppl >= 64)
+ *   "radix-sort"      "RADIX_BITS"       4, 8
+ * get_kernel_variant also reads "rasterize-tiles" "TILE_X". Anything else:
+ * GS3D_E_UNKNOWN_NAME. A tile-size change resets an impossible PIXELS_PER_LANE
+ * to 1. */
+int gs3d_set_kernel_variant(gs3d_ctx* ctx, const char* kernel, const char* param, int value);
+int gs3d_get_kernel_variant(gs3d_ctx* ctx, const char* kernel, const char* param, int* value);
 int gs3d_get_tile_count(const gs3d_ctx* ctx, uint32_t out_xy[2]);
 
 /* Stages 1-6A on the context's stream: preprocess, scan, emit, sort, identify,
@@ -179,6 +190,27 @@ int gs3d_stage_times_avg_us(gs3d_ctx* ctx, float* out, int n, uint64_t* frames);
 /* Number of radix passes the last rebin's sort ran and the bit offset of each
  * (passes whose digit is constant over all keys are skipped). */
 int gs3d_sort_passes(gs3d_ctx* ctx, uint32_t* n_passes, uint32_t bit_offsets[16]);
+
+/* ---- named device buffers (BufferManager, src/BufferManager.ts:32-154) ----
+ * The renderer's own buffers ("projected_splats", "splat_ref_counts",
+ * "splat_ref_offsets", "sort_keys_a", "sort_values_a", "tile_offsets",
+ * "splat_target", ...) live in the same registry. */
+/* initBuffers/createBuffer: allocate (at least 4 bytes); an existing name is
+ * resized. Contents are undefined after a (re)allocation. */
+int gs3d_buffer_create(gs3d_ctx* ctx, const char* name, size_t bytes);
+/* resize: GS3D_E_UNKNOWN_NAME for a name never created. */
+int gs3d_buffer_resize(gs3d_ctx* ctx, const char* name, size_t bytes);
+int gs3d_buffer_size(gs3d_ctx* ctx, const char* name, size_t* bytes);
+/* write: clipped to the buffer (BufferManager.ts:92-125); when nothing fits it
+ * is skipped with GS3D_E_TOO_SMALL. `written` gets the bytes copied. */
+int gs3d_buffer_write(gs3d_ctx* ctx, const char* name, size_t offset, const void* src,
+                      size_t bytes, size_t* written);
+int gs3d_buffer_read(gs3d_ctx* ctx, const char* name, size_t offset, void* dst, size_t bytes);
+/* destroy: frees the buffer; an unknown name is a no-op (BufferManager.ts:164-170). */
+int gs3d_buffer_destroy(gs3d_ctx* ctx, const char* name);
+/* All buffers, largest first: names[i*64 .. i*64+63], sizes[i], for i < n_max;
+ * `count` gets the total number (Profiler.getBuffersSortedBySize). */
+int gs3d_buffer_list(gs3d_ctx* ctx, char* names, uint64_t* sizes, size_t n_max, size_t* count);
 
 /* ---- stage entry points (each WGSL pass fed caller data) ---------------- */
 

@@ -267,3 +267,41 @@ def test_errors_through_abi(renderer):
     fresh.close()
     lim = renderer.device_limits()
     assert lim["maxTotalThreads"] >= 1024
+
+
+@pytest.mark.parametrize("tile,ppl", [(16, 2), (16, 4), (32, 2), (32, 4)])
+def test_pipeline_raster_variants(renderer, tile, ppl):
+    say(f"pipeline tile={tile} pixels_per_lane={ppl}")
+    s = sc.activate(sc.clustered(44, 40_000, 16, sh_degree=3))
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(tile, tile)
+    renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", ppl)
+    try:
+        _, _, err = rebin_and_compare(renderer, s, sc.orbit_eye(7.0, 200.0, 35.0), 800, 600, tile)
+    finally:
+        renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 1)
+    assert err <= TOL_IMAGE
+
+
+def test_pipeline_radix_4bit(renderer):
+    say("pipeline radix 4-bit digits")
+    s = sc.activate(sc.clustered(45, 30_000, 16, sh_degree=3))
+    renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 4)
+    try:
+        _, _, err = rebin_and_compare(renderer, s, sc.orbit_eye(6.0, 90.0, -20.0), 640, 480)
+        assert renderer.sort_passes() == list(range(0, 32 + 11, 4))  # 1200 tiles -> 11 tile bits
+    finally:
+        renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)
+    assert err <= TOL_IMAGE
+
+
+def test_kernel_variant_errors(renderer):
+    renderer.set_tile_size(8, 8)
+    with pytest.raises(L.Gs3dError):
+        renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 2)  # 8x8 is one wavefront
+    with pytest.raises(L.Gs3dError):
+        renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 6)
+    with pytest.raises(L.Gs3dError):
+        renderer.set_kernel_variant("nope", "X", 1)
+    renderer.set_tile_size(16, 16)
+    assert renderer.get_kernel_variant("rasterize-tiles", "TILE_X") == 16

@@ -1,0 +1,492 @@
+/*
+ * gs3d_napi.c — Node N-API binding of libgs3d.so (include/gs3d.h).
+ *
+ * A thin C layer: every function converts JS arguments (numbers, typed
+ * arrays, strings), calls one C-ABI entry point and turns a non-zero return
+ * into a thrown JS Error carrying gs3d_last_error(). The context is a JS
+ * external whose finalizer destroys it. Built against N-API 8 headers; the
+ * napi_* symbols resolve from the node binary at load time.
+ */
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "gs3d.h"
+
+#define NAPI_CALL(env, call)                                                    \
+    do {                                                                        \
+        if ((call) != napi_ok) {                                                \
+            napi_throw_error((env), NULL, "gs3d addon: N-API call failed: " #call); \
+            return NULL;                                                        \
+        }                                                                       \
+    } while (0)
+
+#define ARGS(n)                                                                 \
+    size_t argc = (n);                                                          \
+    napi_value argv[(n) > 0 ? (n) : 1];                                         \
+    NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL))
+
+static napi_value throw_rc(napi_env env, gs3d_ctx* c, int rc, const char* what) {
+    char msg[768];
+    snprintf(msg, sizeof msg, "%s failed (%d): %s", what, rc, c ? gs3d_last_error(c) : "");
+    char code[16];
+    snprintf(code, sizeof code, "%d", rc);
+    napi_throw_error(env, code, msg);
+    return NULL;
+}
+
+#define CHECK(env, c, expr, what)                       \
+    do {                                                \
+        int rc_ = (expr);                               \
+        if (rc_ != GS3D_OK) return throw_rc(env, c, rc_, what); \
+    } while (0)
+
+static void finalize_ctx(napi_env env, void* data, void* hint) {
+    (void)env;
+    (void)hint;
+    gs3d_ctx** slot = (gs3d_ctx**)data;
+    if (*slot) gs3d_destroy(*slot);
+    free(slot);
+}
+
+static gs3d_ctx* get_ctx(napi_env env, napi_value v) {
+    void* data = NULL;
+    if (napi_get_value_external(env, v, &data) != napi_ok || !data || !*(gs3d_ctx**)data) {
+        napi_throw_type_error(env, NULL, "gs3d: invalid or destroyed context handle");
+        return NULL;
+    }
+    return *(gs3d_ctx**)data;
+}
+
+static int get_u32(napi_env env, napi_value v, uint32_t* out) {
+    return napi_get_value_uint32(env, v, out) == napi_ok;
+}
+
+/* Data pointer + byte length of a TypedArray / ArrayBuffer / DataView;
+ * null and undefined give NULL. */
+static int get_bytes(napi_env env, napi_value v, void** data, size_t* bytes) {
+    napi_valuetype t;
+    if (napi_typeof(env, v, &t) != napi_ok) return 0;
+    if (t == napi_null || t == napi_undefined) {
+        *data = NULL;
+        *bytes = 0;
+        return 1;
+    }
+    bool is;
+    if (napi_is_typedarray(env, v, &is) == napi_ok && is) {
+        napi_typedarray_type tt;
+        size_t len, off;
+        napi_value ab;
+        if (napi_get_typedarray_info(env, v, &tt, &len, data, &ab, &off) != napi_ok) return 0;
+        size_t el = 1;
+        switch (tt) {
+            case napi_int16_array: case napi_uint16_array: el = 2; break;
+            case napi_int32_array: case napi_uint32_array: case napi_float32_array: el = 4; break;
+            case napi_float64_array: case napi_bigint64_array: case napi_biguint64_array: el = 8; break;
+            default: el = 1;
+        }
+        *bytes = len * el;
+        return 1;
+    }
+    if (napi_is_arraybuffer(env, v, &is) == napi_ok && is)
+        return napi_get_arraybuffer_info(env, v, data, bytes) == napi_ok;
+    return 0;
+}
+
+static napi_value make_u32(napi_env env, uint32_t v) {
+    napi_value r;
+    napi_create_uint32(env, v, &r);
+    return r;
+}
+
+static napi_value make_typed(napi_env env, napi_typedarray_type t, const void* src, size_t count,
+                             size_t el) {
+    void* data;
+    napi_value ab, out;
+    if (napi_create_arraybuffer(env, count * el, &data, &ab) != napi_ok) return NULL;
+    if (count) memcpy(data, src, count * el);
+    if (napi_create_typedarray(env, t, count, ab, 0, &out) != napi_ok) return NULL;
+    return out;
+}
+
+static napi_value pair(napi_env env, uint32_t a, uint32_t b) {
+    napi_value arr;
+    napi_create_array_with_length(env, 2, &arr);
+    napi_set_element(env, arr, 0, make_u32(env, a));
+    napi_set_element(env, arr, 1, make_u32(env, b));
+    return arr;
+}
+
+/* create(device) -> external */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int32_t dev = 0;
+    if (argc > 0) napi_get_value_int32(env, argv[0], &dev);
+    gs3d_ctx* c = NULL;
+    CHECK(env, NULL, gs3d_create(dev, &c), "gs3d_create");
+    gs3d_ctx** slot = (gs3d_ctx**)malloc(sizeof *slot);
+    *slot = c;
+    napi_value ext;
+    NAPI_CALL(env, napi_create_external(env, slot, finalize_ctx, NULL, &ext));
+    return ext;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    void* data = NULL;
+    NAPI_CALL(env, napi_get_value_external(env, argv[0], &data));
+    gs3d_ctx** slot = (gs3d_ctx**)data;
+    if (slot && *slot) {
+        gs3d_destroy(*slot);
+        *slot = NULL;
+    }
+    return NULL;
+}
+
+static napi_value js_last_error(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    napi_value s;
+    NAPI_CALL(env, napi_create_string_utf8(env, gs3d_last_error(c), NAPI_AUTO_LENGTH, &s));
+    return s;
+}
+
+static napi_value js_sync(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    CHECK(env, c, gs3d_sync(c), "gs3d_sync");
+    return NULL;
+}
+
+static napi_value js_device_limits(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t lim[6];
+    CHECK(env, c, gs3d_device_limits(c, lim), "gs3d_device_limits");
+    return make_typed(env, napi_uint32_array, lim, 6, 4);
+}
+
+/* setSplats(ctx, pos, scale, rot, color, fDc|null, shRest|null, shDegree, n) */
+static napi_value js_set_splats(napi_env env, napi_callback_info info) {
+    ARGS(9);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    void* p[6];
+    size_t b[6];
+    for (int k = 0; k < 6; k++)
+        if (!get_bytes(env, argv[1 + k], &p[k], &b[k])) {
+            napi_throw_type_error(env, NULL, "setSplats: expected typed arrays");
+            return NULL;
+        }
+    int32_t deg = 0;
+    uint32_t n = 0;
+    napi_get_value_int32(env, argv[7], &deg);
+    get_u32(env, argv[8], &n);
+    for (int k = 0; k < 4; k++)
+        if (b[k] < (size_t)n * 16) {
+            napi_throw_range_error(env, NULL, "setSplats: vec4 array shorter than 16*n bytes");
+            return NULL;
+        }
+    if (deg > 0) {
+        const size_t rest = (size_t)3 * ((deg + 1) * (deg + 1) - 1);
+        if (b[4] < (size_t)n * 12 || b[5] < (size_t)n * rest * 4) {
+            napi_throw_range_error(env, NULL, "setSplats: f_dc / sh_rest shorter than n rows");
+            return NULL;
+        }
+    }
+    CHECK(env, c, gs3d_set_splats(c, p[0], p[1], p[2], p[3], p[4], p[5], deg, n), "gs3d_set_splats");
+    return NULL;
+}
+
+static napi_value js_set_tile_size(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t x = 0, y = 0, out[2];
+    get_u32(env, argv[1], &x);
+    get_u32(env, argv[2], &y);
+    CHECK(env, c, gs3d_set_tile_size(c, x, y, out), "gs3d_set_tile_size");
+    return pair(env, out[0], out[1]);
+}
+
+static napi_value js_get_tile_size(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t out[2];
+    CHECK(env, c, gs3d_get_tile_size(c, out), "gs3d_get_tile_size");
+    return pair(env, out[0], out[1]);
+}
+
+static napi_value js_get_tile_count(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t out[2];
+    CHECK(env, c, gs3d_get_tile_count(c, out), "gs3d_get_tile_count");
+    return pair(env, out[0], out[1]);
+}
+
+/* rebin(ctx, camera Float32Array(72), width, height, flags) */
+static napi_value js_rebin(napi_env env, napi_callback_info info) {
+    ARGS(5);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    void* cam;
+    size_t cb;
+    if (!get_bytes(env, argv[1], &cam, &cb) || cb < sizeof(gs3d_camera)) {
+        napi_throw_type_error(env, NULL, "rebin: camera must be a Float32Array of 72 floats");
+        return NULL;
+    }
+    uint32_t w = 0, h = 0, flags = 0;
+    get_u32(env, argv[2], &w);
+    get_u32(env, argv[3], &h);
+    if (argc > 4) get_u32(env, argv[4], &flags);
+    CHECK(env, c, gs3d_rebin(c, (const gs3d_camera*)cam, w, h, flags), "gs3d_rebin");
+    return NULL;
+}
+
+static napi_value js_ref_count(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint64_t k = 0;
+    CHECK(env, c, gs3d_ref_count(c, &k), "gs3d_ref_count");
+    napi_value r;
+    napi_create_double(env, (double)k, &r);
+    return r;
+}
+
+/* read(ctx, bufferId) -> ArrayBuffer */
+static napi_value js_read(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    int32_t id = 0;
+    napi_get_value_int32(env, argv[1], &id);
+    size_t size = 0;
+    CHECK(env, c, gs3d_read(c, id, NULL, 0, &size), "gs3d_read");
+    void* data;
+    napi_value ab;
+    NAPI_CALL(env, napi_create_arraybuffer(env, size, &data, &ab));
+    CHECK(env, c, gs3d_read(c, id, data, size, &size), "gs3d_read");
+    return ab;
+}
+
+/* readTileCounts(ctx) -> Uint32Array | null (null on any failure) */
+static napi_value js_read_tile_counts(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    napi_value nul;
+    napi_get_null(env, &nul);
+    if (!c) return NULL;
+    size_t n = 0;
+    if (gs3d_read_tile_counts(c, NULL, 0, &n) != GS3D_OK) return nul;
+    void* data;
+    napi_value ab, out;
+    NAPI_CALL(env, napi_create_arraybuffer(env, n * 4, &data, &ab));
+    if (gs3d_read_tile_counts(c, (uint32_t*)data, n, &n) != GS3D_OK) return nul;
+    NAPI_CALL(env, napi_create_typedarray(env, napi_uint32_array, n, ab, 0, &out));
+    return out;
+}
+
+static napi_value js_stage_times(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    float t[GS3D_STAGE__COUNT];
+    CHECK(env, c, gs3d_stage_times_us(c, t, GS3D_STAGE__COUNT), "gs3d_stage_times_us");
+    return make_typed(env, napi_float32_array, t, GS3D_STAGE__COUNT, 4);
+}
+
+static napi_value js_stage_times_avg(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    float t[GS3D_STAGE__COUNT];
+    uint64_t frames = 0;
+    CHECK(env, c, gs3d_stage_times_avg_us(c, t, GS3D_STAGE__COUNT, &frames), "gs3d_stage_times_avg_us");
+    return make_typed(env, napi_float32_array, t, GS3D_STAGE__COUNT, 4);
+}
+
+static napi_value js_profiler_reset(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    CHECK(env, c, gs3d_profiler_reset(c), "gs3d_profiler_reset");
+    return NULL;
+}
+
+static napi_value js_stage_label(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    int32_t i = 0;
+    napi_get_value_int32(env, argv[0], &i);
+    napi_value s;
+    NAPI_CALL(env, napi_create_string_utf8(env, gs3d_stage_label(i), NAPI_AUTO_LENGTH, &s));
+    return s;
+}
+
+static int get_name(napi_env env, napi_value v, char* buf, size_t cap) {
+    size_t len = 0;
+    return napi_get_value_string_utf8(env, v, buf, cap, &len) == napi_ok && len > 0 && len < cap - 1;
+}
+
+static napi_value js_buffer_create(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char name[64];
+    double bytes = 0;
+    if (!get_name(env, argv[1], name, sizeof name)) {
+        napi_throw_type_error(env, NULL, "buffer name must be a non-empty string");
+        return NULL;
+    }
+    napi_get_value_double(env, argv[2], &bytes);
+    CHECK(env, c, gs3d_buffer_create(c, name, (size_t)bytes), "gs3d_buffer_create");
+    return NULL;
+}
+
+static napi_value js_buffer_resize(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char name[64];
+    double bytes = 0;
+    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    napi_get_value_double(env, argv[2], &bytes);
+    CHECK(env, c, gs3d_buffer_resize(c, name, (size_t)bytes), "gs3d_buffer_resize");
+    return NULL;
+}
+
+/* bufferSize(ctx, name) -> number | undefined for an unknown name */
+static napi_value js_buffer_size(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char name[64];
+    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    size_t bytes = 0;
+    if (gs3d_buffer_size(c, name, &bytes) != GS3D_OK) return NULL;
+    napi_value r;
+    napi_create_double(env, (double)bytes, &r);
+    return r;
+}
+
+/* bufferWrite(ctx, name, offset, data) -> bytes written (0 = skipped) */
+static napi_value js_buffer_write(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char name[64];
+    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    double off = 0;
+    napi_get_value_double(env, argv[2], &off);
+    void* data;
+    size_t bytes;
+    if (!get_bytes(env, argv[3], &data, &bytes)) {
+        napi_throw_type_error(env, NULL, "bufferWrite: data must be a typed array or ArrayBuffer");
+        return NULL;
+    }
+    size_t written = 0;
+    int rc = gs3d_buffer_write(c, name, (size_t)off, data, bytes, &written);
+    if (rc != GS3D_OK && rc != GS3D_E_TOO_SMALL) return throw_rc(env, c, rc, "gs3d_buffer_write");
+    napi_value r;
+    napi_create_double(env, (double)written, &r);
+    return r;
+}
+
+static napi_value js_buffer_read(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char name[64];
+    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    double off = 0, bytes = 0;
+    napi_get_value_double(env, argv[2], &off);
+    napi_get_value_double(env, argv[3], &bytes);
+    void* data;
+    napi_value ab;
+    NAPI_CALL(env, napi_create_arraybuffer(env, (size_t)bytes, &data, &ab));
+    CHECK(env, c, gs3d_buffer_read(c, name, (size_t)off, data, (size_t)bytes), "gs3d_buffer_read");
+    return ab;
+}
+
+static napi_value js_buffer_destroy(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char name[64];
+    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    CHECK(env, c, gs3d_buffer_destroy(c, name), "gs3d_buffer_destroy");
+    return NULL;
+}
+
+/* bufferList(ctx) -> [{name, size}], largest first */
+static napi_value js_buffer_list(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    size_t count = 0;
+    CHECK(env, c, gs3d_buffer_list(c, NULL, NULL, 0, &count), "gs3d_buffer_list");
+    char* names = (char*)calloc(count ? count : 1, 64);
+    uint64_t* sizes = (uint64_t*)calloc(count ? count : 1, 8);
+    int rc = gs3d_buffer_list(c, names, sizes, count, &count);
+    napi_value arr;
+    napi_create_array_with_length(env, count, &arr);
+    for (size_t i = 0; rc == GS3D_OK && i < count; i++) {
+        napi_value o, nm, sz;
+        napi_create_object(env, &o);
+        napi_create_string_utf8(env, names + 64 * i, NAPI_AUTO_LENGTH, &nm);
+        napi_create_double(env, (double)sizes[i], &sz);
+        napi_set_named_property(env, o, "name", nm);
+        napi_set_named_property(env, o, "size", sz);
+        napi_set_element(env, arr, (uint32_t)i, o);
+    }
+    free(names);
+    free(sizes);
+    if (rc != GS3D_OK) return throw_rc(env, c, rc, "gs3d_buffer_list");
+    return arr;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+    napi_property_descriptor d[] = {
+#define FN(js, c) {js, NULL, c, NULL, NULL, NULL, napi_enumerable, NULL}
+        FN("create", js_create),
+        FN("destroy", js_destroy),
+        FN("lastError", js_last_error),
+        FN("sync", js_sync),
+        FN("deviceLimits", js_device_limits),
+        FN("setSplats", js_set_splats),
+        FN("setTileSize", js_set_tile_size),
+        FN("getTileSize", js_get_tile_size),
+        FN("getTileCount", js_get_tile_count),
+        FN("rebin", js_rebin),
+        FN("refCount", js_ref_count),
+        FN("read", js_read),
+        FN("readTileCounts", js_read_tile_counts),
+        FN("stageTimes", js_stage_times),
+        FN("stageTimesAvg", js_stage_times_avg),
+        FN("profilerReset", js_profiler_reset),
+        FN("stageLabel", js_stage_label),
+        FN("bufferCreate", js_buffer_create),
+        FN("bufferResize", js_buffer_resize),
+        FN("bufferSize", js_buffer_size),
+        FN("bufferWrite", js_buffer_write),
+        FN("bufferRead", js_buffer_read),
+        FN("bufferDestroy", js_buffer_destroy),
+        FN("bufferList", js_buffer_list),
+#undef FN
+    };
+    NAPI_CALL(env, napi_define_properties(env, exports, sizeof d / sizeof d[0], d));
+    napi_value v;
+    napi_create_int32(env, gs3d_abi_version(), &v);
+    napi_set_named_property(env, exports, "abiVersion", v);
+    return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

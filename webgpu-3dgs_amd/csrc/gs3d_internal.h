@@ -66,14 +66,19 @@ struct EmitArgs {
     const uint32_t* offsets;  // n + 1 entries
     uint64_t* keys;
     uint32_t* values;
-    uint32_t* hist;           // [passes][radix] digit histograms, or null
+    uint32_t* hist;           // [passes][radix] digit histograms (accumulated), or null
+    uint32_t* hist_partial;   // [emit_grid(n)][passes][radix] scratch rows, or null
     uint32_t n;
     uint32_t tiles_x;
     uint64_t capacity;        // keys/values entries allocated
-    uint32_t n_passes;
+    uint32_t n_passes;        // histogram of pass p is over the digit at bit p * digit_bits
     uint32_t digit_bits;
-    uint32_t shifts[kMaxPasses];
 };
+
+// Emit runs at most this many workgroups (grid-stride over 256-splat chunks),
+// which bounds the per-workgroup histogram rows to fold.
+constexpr uint32_t kEmitMaxBlocks = 1024;
+uint32_t emit_grid(uint32_t n);
 
 // ---- launchers (each enqueues on `stream`, returns the HIP status) --------
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
@@ -86,6 +91,8 @@ hipError_t launch_emit(const EmitArgs& a, hipStream_t s);
 hipError_t launch_digit_histogram(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
                                   uint32_t* hist, uint32_t digit_bits, uint32_t n_passes,
                                   const uint32_t* shifts, hipStream_t s);
+// Status words one radix pass needs: radix_status_tiles(n_max) x RADIX.
+uint32_t radix_status_tiles(uint32_t n_max);
 // One stable LSD pass over n = *n_dev keys (grid covers n_max; n > n_max: no-op).
 hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                              uint32_t* vout, const uint32_t* n_dev, uint32_t n_max,
@@ -94,10 +101,13 @@ hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t*
 // tile_offsets[0..tile_count] from n = *n_dev sorted keys (n > n_max: all 0).
 hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
                                  uint32_t* tile_offsets, uint32_t tile_count, hipStream_t s);
+// Square tiles of 8, 16 or 32 px; pixels_per_lane 1, 2 or 4 (a workgroup of
+// tile*tile/ppl threads, at least one wavefront).
+bool raster_variant_exists(uint32_t tile, uint32_t pixels_per_lane);
 hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_t* values,
                             const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                             uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
-                            uint16_t* image_rgba16f, hipStream_t s);
+                            uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
 
 }  // namespace gs3d

@@ -6,13 +6,12 @@
 // thread looping over one splat's tiles (divergent, uncoalesced), a workgroup
 // owns 256 consecutive splats and therefore one contiguous output range
 // [offsets[i0], offsets[i0+256]); thread j of the range finds its splat by a
-// binary search over the 256 local starts in LDS, so every store is coalesced
-// and a 6000-tile splat costs the same as 6000 small ones.
+// direct lookup in an LDS owner table (one byte store per key to build), so
+// every store of keys and ids is coalesced.
 //
-// Fused here: the digit histograms of every radix pass that the sort will run
-// (LDS atomics per workgroup, one global atomic per non-zero bin), so the sort
-// does not re-read the keys to count them. Keys past `capacity` are counted
-// but not stored; the host detects that from ref_counter and re-emits.
+// Fused here: the digit histograms of every radix pass that the sort will run,
+// so the sort does not re-read the keys to count them. Keys past `capacity`
+// are counted but not stored; the host sees that in ref_counter and re-runs.
 #include "gs3d_internal.h"
 
 namespace gs3d {
@@ -21,29 +20,28 @@ namespace {
 
 constexpr int kEmitThreads = 256;
 
-__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
-    __shared__ uint32_t s_start[kEmitThreads];
-    __shared__ uint32_t s_tmin_x[kEmitThreads];
-    __shared__ uint32_t s_tmin_y[kEmitThreads];
-    __shared__ uint32_t s_width[kEmitThreads];
-    __shared__ uint32_t s_keylo[kEmitThreads];
-    __shared__ uint32_t s_hist[kMaxPasses * 256];
+// One 256-splat chunk: stage the chunk's splats in LDS, then emit its
+// contiguous key range with every store coalesced. The range is walked in
+// windows of kOwnerWindow keys: each splat first marks the window slots it owns
+// (owner[x] = local splat index), then thread j reads its owner directly.
+constexpr uint32_t kOwnerWindow = 4096;
 
+__device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint32_t* s_start,
+                                           uint32_t* s_tmin_x, uint32_t* s_tmin_y,
+                                           uint32_t* s_width, uint32_t* s_keylo, uint8_t* s_owner,
+                                           uint32_t* s_hist) {
     const int tid = threadIdx.x;
-    const uint32_t i0 = blockIdx.x * kEmitThreads;
     const uint32_t i = i0 + tid;
     const uint32_t last = min(i0 + kEmitThreads, a.n);
     const uint32_t radix = 1u << a.digit_bits;
-    const uint32_t hist_words = a.hist ? a.n_passes * radix : 0u;
-    for (uint32_t k = tid; k < hist_words; k += kEmitThreads) s_hist[k] = 0u;
-
     const uint32_t block_start = a.offsets[i0];
     const uint32_t block_end = a.offsets[last];
+    uint32_t my_start = 0, my_count = 0;
     if (i < a.n) {
         const uint32_t off = a.offsets[i];
-        const uint32_t cnt = a.offsets[i + 1] - off;  // > 0 exactly for valid splats
-        s_start[tid] = off - block_start;
-        if (cnt) {
+        my_count = a.offsets[i + 1] - off;  // > 0 exactly for valid splats
+        my_start = off - block_start;
+        if (my_count) {
             const float4* ps = reinterpret_cast<const float4*>(a.projected + i);
             const float4 f0 = ps[0];
             const float4 f3 = ps[3];
@@ -53,49 +51,98 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
             s_tmin_y[tid] = t0y;
             s_width[tid] = t1x - t0x + 1u;
             s_keylo[tid] = __float_as_uint(-f0.z);
-        } else {
-            s_width[tid] = 1u;
         }
-    } else {
-        s_start[tid] = block_end - block_start;  // past-the-end sentinel for the search
-        s_width[tid] = 1u;
     }
-    __syncthreads();
+    s_start[tid] = my_start;
 
     const uint32_t total = block_end - block_start;
-    const uint32_t nloc = last - i0;
-    for (uint32_t j = tid; j < total; j += kEmitThreads) {
-        // largest s with s_start[s] <= j (upper_bound - 1) over [0, nloc)
-        uint32_t lo = 0, hi = nloc;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_start[mid] <= j) lo = mid; else hi = mid;
-        }
-        const uint32_t r = j - s_start[lo];
-        const uint32_t w = s_width[lo];
-        const uint32_t ry = r / w;
-        const uint32_t rx = r - ry * w;
-        const uint32_t tile = (s_tmin_y[lo] + ry) * a.tiles_x + s_tmin_x[lo] + rx;
-        const uint64_t key = ((uint64_t)tile << 32) | (uint64_t)s_keylo[lo];
-        const uint64_t pos = (uint64_t)block_start + j;
-        if (pos < a.capacity) {
-            a.keys[pos] = key;
-            a.values[pos] = i0 + lo;
-        }
-        if (a.hist) {
-            for (uint32_t p = 0; p < a.n_passes; p++) {
-                const uint32_t d = (uint32_t)(key >> a.shifts[p]) & (radix - 1u);
-                atomicAdd(&s_hist[p * radix + d], 1u);
+    const int lane = tid & (kWave - 1);
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (uint32_t w0 = 0; w0 < total; w0 += kOwnerWindow) {
+        const uint32_t wn = min(kOwnerWindow, total - w0);
+        __syncthreads();  // previous window's owners consumed (and the staging above visible)
+        const uint32_t lo = max(my_start, w0), hi = min(my_start + my_count, w0 + wn);
+        for (uint32_t x = lo; x < hi; x++) s_owner[x - w0] = (uint8_t)tid;
+        __syncthreads();
+        // every lane of the wavefront runs the same number of rounds (ballots below)
+        const uint32_t rounds = (wn + kEmitThreads - 1) / kEmitThreads;
+        for (uint32_t round = 0; round < rounds; round++) {
+            const uint32_t jw = round * kEmitThreads + tid;
+            const bool live = jw < wn;
+            uint64_t key = 0;
+            if (live) {
+                const uint32_t o = s_owner[jw];
+                const uint32_t r = w0 + jw - s_start[o];
+                const uint32_t w = s_width[o];
+                const uint32_t ry = r / w;
+                const uint32_t rx = r - ry * w;
+                const uint32_t tile = (s_tmin_y[o] + ry) * a.tiles_x + s_tmin_x[o] + rx;
+                key = ((uint64_t)tile << 32) | (uint64_t)s_keylo[o];
+                const uint64_t pos = (uint64_t)block_start + w0 + jw;
+                if (pos < a.capacity) {
+                    a.keys[pos] = key;
+                    a.values[pos] = i0 + o;
+                }
+            }
+            if (a.hist_partial) {
+                // Lanes with equal digits are grouped by ballots (a wave-level
+                // multisplit); only each group's lowest lane touches LDS, so the
+                // digits that repeat across a wavefront (depth exponent, tile
+                // row) do not serialize on one LDS address. Pass p's digit sits
+                // at bit p * digit_bits.
+                for (uint32_t p = 0; p < a.n_passes; p++) {
+                    const uint32_t d = (uint32_t)(key >> (p * a.digit_bits)) & (radix - 1u);
+                    uint64_t peers = __ballot(live);
+                    for (uint32_t b = 0; b < a.digit_bits; b++) {
+                        const bool bit = (d >> b) & 1u;
+                        const uint64_t bal = __ballot(bit);
+                        peers &= bit ? bal : ~bal;
+                    }
+                    if (live && (peers & lt_mask) == 0ull)
+                        atomicAdd(&s_hist[p * radix + d], (uint32_t)__popcll(peers));
+                }
             }
         }
     }
-    if (a.hist) {
+    __syncthreads();  // LDS staging is reused by the next chunk
+}
+
+// Grid-stride over 256-splat chunks; the digit histograms accumulate in LDS
+// across a workgroup's chunks and leave as one plain row per workgroup
+// (hist_partial[block][pass][digit]); k_hist_reduce folds the rows.
+__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
+    __shared__ uint32_t s_start[kEmitThreads];
+    __shared__ uint32_t s_tmin_x[kEmitThreads];
+    __shared__ uint32_t s_tmin_y[kEmitThreads];
+    __shared__ uint32_t s_width[kEmitThreads];
+    __shared__ uint32_t s_keylo[kEmitThreads];
+    __shared__ uint8_t s_owner[kOwnerWindow];
+    __shared__ uint32_t s_hist[kMaxPasses * 256];
+    const uint32_t radix = 1u << a.digit_bits;
+    const uint32_t hist_words = a.hist_partial ? a.n_passes * radix : 0u;
+    for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) s_hist[k] = 0u;
+    const uint32_t chunks = (a.n + kEmitThreads - 1) / kEmitThreads;
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x)
+        emit_chunk(a, c * kEmitThreads, s_start, s_tmin_x, s_tmin_y, s_width, s_keylo, s_owner, s_hist);
+    if (a.hist_partial) {
         __syncthreads();
-        for (uint32_t k = tid; k < hist_words; k += kEmitThreads) {
-            const uint32_t c = s_hist[k];
-            if (c) atomicAdd(&a.hist[k], c);
-        }
+        uint32_t* row = a.hist_partial + (size_t)blockIdx.x * hist_words;
+        for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) row[k] = s_hist[k];
     }
+}
+
+// hist[w] += sum over rows r of partial[r][w], rows split over gridDim.y
+// slices (one atomic per word per slice).
+__global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* __restrict__ partial,
+                                                     uint32_t rows, uint32_t words,
+                                                     uint32_t* __restrict__ hist) {
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    if (w >= words) return;
+    const uint32_t per = (rows + gridDim.y - 1) / gridDim.y;
+    const uint32_t r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+    uint32_t sum = 0;
+    for (uint32_t r = r0; r < r1; r++) sum += partial[(size_t)r * words + w];
+    if (sum) atomicAdd(&hist[w], sum);
 }
 
 // Digit histograms of n = *n_dev keys for every pass (standalone sort path).
@@ -123,11 +170,22 @@ __global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* keys, c
 
 }  // namespace
 
+uint32_t emit_grid(uint32_t n) {
+    const uint32_t chunks = (n + kEmitThreads - 1) / kEmitThreads;
+    return chunks < kEmitMaxBlocks ? (chunks ? chunks : 1u) : kEmitMaxBlocks;
+}
+
 hipError_t launch_emit(const EmitArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     if (a.n_passes > kMaxPasses || (a.digit_bits != 4 && a.digit_bits != 8)) return hipErrorInvalidValue;
-    const uint32_t grid = (a.n + kEmitThreads - 1) / kEmitThreads;
+    const uint32_t grid = emit_grid(a.n);
     hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kEmitThreads), 0, s, a);
+    if (a.hist_partial && a.hist) {
+        const uint32_t words = a.n_passes * (1u << a.digit_bits);
+        const uint32_t slices = grid < 32u ? grid : 32u;
+        hipLaunchKernelGGL(k_hist_reduce, dim3((words + 255u) / 256u, slices), dim3(256), 0, s,
+                           a.hist_partial, grid, words, a.hist);
+    }
     return hipGetLastError();
 }
 

@@ -13,11 +13,12 @@
 //      digit; popcount below the lane = stable rank, the lowest lane bumps the
 //      wave's private LDS histogram;
 //   3. per digit: the tile's count is published as an aggregate, the workgroup
-//      looks back over predecessor tiles' status words (decoupled lookback)
-//      for its exclusive prefix, then publishes the inclusive prefix; the
-//      digit's global start comes from the digit histograms fused into emit;
-//   4. keys/values are placed digit-sorted in LDS, then written out in LDS
-//      order so runs of equal digits leave as contiguous stores.
+//      looks back over predecessor tiles' status words (decoupled lookback,
+//      up to 8 predecessors per round trip) for its exclusive prefix, then
+//      publishes the inclusive prefix; the digit's global start comes from
+//      the digit histograms fused into emit;
+//   4. keys, then values, are placed digit-sorted in LDS and written out in
+//      LDS order, so runs of equal digits leave as contiguous stores.
 // HBM traffic per pass: 12 B read + 12 B written per key. Tiles beyond
 // n = *n_dev exit at once, so a graph-captured launch can size the grid to the
 // buffer capacity while the key count lives on the device.
@@ -54,6 +55,12 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t x, uint32_
     return prefix + v - x;
 }
 
+// Lookback reads the status words of up to 8 predecessor tiles with
+// independent loads (one latency per 8 tiles) and spins only on a word that is
+// not published yet. Status words are tile-major, status[tile * RADIX + d], so
+// a tile publishes all its digits with one coalesced store.
+constexpr int kLookbackGroup = 8;
+
 template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
@@ -63,8 +70,8 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     constexpr int RADIX = 1 << BITS;
     constexpr int WAVES = kSortThreads / kWave;
     constexpr uint32_t MASK = RADIX - 1;
-    __shared__ uint64_t s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    // keys are staged first, then values, through the same 32 KB
+    __shared__ uint64_t s_stage[kSortTile];
     __shared__ uint32_t s_wave_hist[WAVES][RADIX];
     __shared__ uint32_t s_bin_local[RADIX];
     __shared__ uint32_t s_bin_global[RADIX];
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     }
     __syncthreads();
 
-    // Per digit: counts across waves -> wave offsets, tile count.
+    // Per digit: counts across waves -> wave offsets, tile count; publish.
     uint32_t tile_count = 0;
     if (tid < RADIX) {
 #pragma unroll
@@ -140,16 +147,29 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     const uint32_t global_start = block_exclusive_scan_256(tid < RADIX ? hist_pass[tid] : 0u, s_tmp);
     if (tid < RADIX) {
         s_bin_local[tid] = local_start;
+        // decoupled lookback, up to 8 predecessors per round trip
         uint32_t excl = 0;
-        for (int64_t p = (int64_t)part - 1; p >= 0; p--) {
-            uint64_t w;
-            uint32_t flag;
-            do {
-                w = load_status(&status[(size_t)p * RADIX + tid]);
-                flag = ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
-            } while (flag == 0);
-            excl += (uint32_t)w;
-            if (flag == kFlagPrefix) break;
+        int64_t q = (int64_t)part - 1;  // nearest predecessor not yet folded in
+        while (q >= 0) {
+            uint64_t w[kLookbackGroup];
+#pragma unroll
+            for (int i = 0; i < kLookbackGroup; i++)
+                w[i] = (q - i >= 0) ? load_status(&status[(size_t)(q - i) * RADIX + tid]) : 0ull;
+            bool stop = false;
+#pragma unroll
+            for (int i = 0; i < kLookbackGroup; i++) {
+                if (stop || q - i < 0) continue;
+                uint64_t x = w[i];
+                uint32_t flag = ((uint32_t)(x >> 34) == epoch) ? (uint32_t)((x >> 32) & 3u) : 0u;
+                while (flag == 0) {
+                    x = load_status(&status[(size_t)(q - i) * RADIX + tid]);
+                    flag = ((uint32_t)(x >> 34) == epoch) ? (uint32_t)((x >> 32) & 3u) : 0u;
+                }
+                excl += (uint32_t)x;
+                stop = flag == kFlagPrefix;
+            }
+            if (stop) break;
+            q -= kLookbackGroup;
         }
         if (part != 0)
             store_status(&status[(size_t)part * RADIX + tid],
@@ -158,28 +178,49 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     }
     __syncthreads();
 
+    // keys: digit-sorted into LDS, then out in LDS order
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
         if (idx < n) {
             const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
-            const uint32_t pos = s_bin_local[d] + s_wave_hist[wave][d] + rank[it];
-            s_keys[pos] = k[it];
-            s_vals[pos] = v[it];
+            rank[it] += s_bin_local[d] + s_wave_hist[wave][d];  // position in the tile
+            s_stage[rank[it]] = k[it];
         }
     }
     __syncthreads();
     const uint32_t tile_n = (uint32_t)min((size_t)kSortTile, (size_t)n - tile_base);
-    for (uint32_t j = tid; j < tile_n; j += kSortThreads) {
-        const uint64_t key = s_keys[j];
-        const uint32_t d = (uint32_t)(key >> shift) & MASK;
-        const uint32_t g = s_bin_global[d] + (j - s_bin_local[d]);
-        kout[g] = key;
-        vout[g] = s_vals[j];
+    uint32_t dst[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const uint32_t j = (uint32_t)(it * kSortThreads + tid);
+        dst[it] = 0;
+        if (j < tile_n) {
+            const uint64_t key = s_stage[j];
+            const uint32_t d = (uint32_t)(key >> shift) & MASK;
+            dst[it] = s_bin_global[d] + (j - s_bin_local[d]);
+            kout[dst[it]] = key;
+        }
+    }
+    __syncthreads();
+    // values: same permutation, reusing the staging memory
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const size_t idx = wave_base + (size_t)it * kWave + lane;
+        if (idx < n) s_vals[rank[it]] = v[it];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const uint32_t j = (uint32_t)(it * kSortThreads + tid);
+        if (j < tile_n) vout[dst[it]] = s_vals[j];
     }
 }
 
 }  // namespace
+
+uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / kSortTile + 1; }
 
 hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
                              uint32_t* vout, const uint32_t* n_dev, uint32_t n_max,

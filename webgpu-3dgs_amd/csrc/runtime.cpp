@@ -36,7 +36,8 @@ constexpr size_t kCtrTotalWords = kCtrHist + kMaxPasses * 256;
 
 struct DevBuf {
     void* ptr = nullptr;
-    size_t cap = 0;
+    size_t cap = 0;   // bytes allocated
+    size_t size = 0;  // bytes in use (the BufferManager-visible size)
 };
 
 const char* kStageLabels[GS3D_STAGE__COUNT] = {
@@ -60,6 +61,7 @@ struct gs3d_ctx {
     uint32_t tile_x = 16, tile_y = 16;
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
+    uint32_t raster_ppl = 1;
     uint64_t key_capacity = 0;
     uint32_t n_passes = 0;
     uint32_t shifts[kMaxPasses] = {};
@@ -147,6 +149,7 @@ int ensure(gs3d_ctx* c, const char* name, size_t bytes, void** out) {
                         hipGetErrorString(e));
         b.cap = bytes;
     }
+    b.size = bytes;
     *out = b.ptr;
     return GS3D_OK;
 }
@@ -229,8 +232,9 @@ int set_device(gs3d_ctx* c) {
 int run_sort(gs3d_ctx* c, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb,
              const uint32_t* n_dev, uint64_t n_max, uint32_t* counters, bool* in_b) {
     uint64_t* radix_status = nullptr;
-    const uint64_t tiles = (n_max + kSortTile - 1) / kSortTile + 1;
-    GS3D_TRY(ensure_status(c, "radix_status", tiles * (1u << c->digit_bits), &radix_status));
+    GS3D_TRY(ensure_status(c, "radix_status",
+                           (size_t)radix_status_tiles((uint32_t)n_max) * (1u << c->digit_bits),
+                           &radix_status));
     const uint32_t radix = 1u << c->digit_bits;
     for (uint32_t p = 0; p < c->n_passes; p++) {
         const bool even = (p % 2) == 0;
@@ -307,12 +311,15 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     ea.keys = ka;
     ea.values = va;
     ea.hist = counters + kCtrHist;
+    uint32_t* hist_partial;
+    GS3D_TRY(ensure_t(c, "radix_hist_partial",
+                      (size_t)emit_grid(n) * c->n_passes * (1u << c->digit_bits), &hist_partial));
+    ea.hist_partial = hist_partial;
     ea.n = n;
     ea.tiles_x = c->tiles_x;
     ea.capacity = cap;
     ea.n_passes = c->n_passes;
     ea.digit_bits = c->digit_bits;
-    for (uint32_t p = 0; p < c->n_passes; p++) ea.shifts[p] = c->shifts[p];
     HIP_TRY(c, launch_emit(ea, c->stream));
     record(c, GS3D_STAGE_EMIT, 1);
     if (c->keep_unsorted) {
@@ -336,7 +343,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
 
     record(c, GS3D_STAGE_RASTER, 0);
     HIP_TRY(c, launch_rasterize(projected, sorted_vals, tile_offsets, c->tile_x, c->tile_y,
-                                c->tiles_x, c->tiles_y, W, H, image, c->stream));
+                                c->tiles_x, c->tiles_y, W, H, c->raster_ppl, image, c->stream));
     record(c, GS3D_STAGE_RASTER, 1);
     record(c, GS3D_STAGE_TOTAL, 1);
 
@@ -500,11 +507,48 @@ int gs3d_set_tile_size(gs3d_ctx* c, uint32_t x, uint32_t y, uint32_t out_xy[2]) 
     GS3D_TRY(settle(c, true));
     c->tile_x = tx;
     c->tile_y = ty;
+    if (!raster_variant_exists(tx, c->raster_ppl)) c->raster_ppl = 1;  // 8x8 has one wavefront
     if (out_xy) {
         out_xy[0] = tx;
         out_xy[1] = ty;
     }
     return GS3D_OK;
+}
+
+int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, int value) {
+    if (!c || !kernel || !param) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(settle(c, true));
+    if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "PIXELS_PER_LANE")) {
+        if (value < 0 || !raster_variant_exists(c->tile_x, (uint32_t)value))
+            return fail(c, GS3D_E_UNKNOWN_NAME, "no rasterize-tiles variant with %d pixels per lane for %u px tiles",
+                        value, c->tile_x);
+        c->raster_ppl = (uint32_t)value;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "RADIX_BITS")) {
+        if (value != 4 && value != 8)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "no radix-sort variant with %d-bit digits", value);
+        c->digit_bits = (uint32_t)value;
+        return GS3D_OK;
+    }
+    return fail(c, GS3D_E_UNKNOWN_NAME, "no kernel variant %s.%s", kernel, param);
+}
+
+int gs3d_get_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, int* value) {
+    if (!c || !kernel || !param || !value) return GS3D_E_INVALID_ARG;
+    if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "PIXELS_PER_LANE")) {
+        *value = (int)c->raster_ppl;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "TILE_X")) {
+        *value = (int)c->tile_x;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "RADIX_BITS")) {
+        *value = (int)c->digit_bits;
+        return GS3D_OK;
+    }
+    return fail(c, GS3D_E_UNKNOWN_NAME, "no kernel variant %s.%s", kernel, param);
 }
 
 int gs3d_get_tile_size(const gs3d_ctx* c, uint32_t out_xy[2]) {
@@ -645,6 +689,92 @@ int gs3d_stage_times_avg_us(gs3d_ctx* c, float* out, int n, uint64_t* frames) {
     return GS3D_OK;
 }
 
+// ---- named buffers (BufferManager, src/BufferManager.ts:32-154) -----------
+
+int gs3d_buffer_create(gs3d_ctx* c, const char* name, size_t bytes) {
+    if (!c || !name || !*name) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(set_device(c));
+    void* p;
+    GS3D_TRY(ensure(c, name, bytes < 4 ? 4 : bytes, &p));  // initBuffers: at least 4 bytes
+    return GS3D_OK;
+}
+
+int gs3d_buffer_resize(gs3d_ctx* c, const char* name, size_t bytes) {
+    if (!c || !name) return GS3D_E_INVALID_ARG;
+    if (!c->bufs.count(name)) return fail(c, GS3D_E_UNKNOWN_NAME, "no buffer '%s'", name);
+    return gs3d_buffer_create(c, name, bytes);
+}
+
+int gs3d_buffer_size(gs3d_ctx* c, const char* name, size_t* bytes) {
+    if (!c || !name || !bytes) return GS3D_E_INVALID_ARG;
+    auto it = c->bufs.find(name);
+    if (it == c->bufs.end()) return fail(c, GS3D_E_UNKNOWN_NAME, "no buffer '%s'", name);
+    *bytes = it->second.size;
+    return GS3D_OK;
+}
+
+int gs3d_buffer_write(gs3d_ctx* c, const char* name, size_t offset, const void* src, size_t bytes,
+                      size_t* written) {
+    if (!c || !name || (bytes && !src)) return GS3D_E_INVALID_ARG;
+    auto it = c->bufs.find(name);
+    if (it == c->bufs.end()) return fail(c, GS3D_E_UNKNOWN_NAME, "no buffer '%s'", name);
+    const size_t size = it->second.size;
+    // an over-long write is clipped to the buffer; nothing left -> skipped
+    const size_t n = offset < size ? std::min(bytes, size - offset) : 0;
+    if (written) *written = n;
+    if (n == 0) return fail(c, GS3D_E_TOO_SMALL, "write to '%s' skipped: size exceeds buffer limit", name);
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    HIP_TRY(c, hipMemcpyAsync(static_cast<char*>(it->second.ptr) + offset, src, n,
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_buffer_read(gs3d_ctx* c, const char* name, size_t offset, void* dst, size_t bytes) {
+    if (!c || !name || (bytes && !dst)) return GS3D_E_INVALID_ARG;
+    auto it = c->bufs.find(name);
+    if (it == c->bufs.end()) return fail(c, GS3D_E_UNKNOWN_NAME, "no buffer '%s'", name);
+    if (offset > it->second.size || bytes > it->second.size - offset)
+        return fail(c, GS3D_E_TOO_SMALL, "read of %zu bytes at %zu past '%s' (%zu bytes)", bytes, offset,
+                    name, it->second.size);
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c, true));
+    if (bytes)
+        HIP_TRY(c, hipMemcpyAsync(dst, static_cast<const char*>(it->second.ptr) + offset, bytes,
+                                  hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_buffer_destroy(gs3d_ctx* c, const char* name) {
+    if (!c || !name) return GS3D_E_INVALID_ARG;
+    auto it = c->bufs.find(name);
+    if (it == c->bufs.end()) return GS3D_OK;  // destroying an unknown name is a no-op
+    GS3D_TRY(set_device(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (it->second.ptr) HIP_TRY(c, hipFree(it->second.ptr));
+    c->bufs.erase(it);
+    return GS3D_OK;
+}
+
+// Names and sizes of every buffer, largest first (Profiler.getBuffersSortedBySize,
+// src/Profiler.ts:111-123). names: n_max * 64 chars; sizes: n_max entries.
+int gs3d_buffer_list(gs3d_ctx* c, char* names, uint64_t* sizes, size_t n_max, size_t* count) {
+    if (!c || !count) return GS3D_E_INVALID_ARG;
+    std::vector<std::pair<std::string, size_t>> all;
+    for (auto& kv : c->bufs) all.emplace_back(kv.first, kv.second.size);
+    std::sort(all.begin(), all.end(), [](const auto& a, const auto& b) {
+        return a.second != b.second ? a.second > b.second : a.first < b.first;
+    });
+    *count = all.size();
+    for (size_t i = 0; i < all.size() && i < n_max; i++) {
+        if (names) std::snprintf(names + 64 * i, 64, "%s", all[i].first.c_str());
+        if (sizes) sizes[i] = all[i].second;
+    }
+    return GS3D_OK;
+}
+
 // ---- stage entry points --------------------------------------------------
 
 int gs3d_stage_preprocess(gs3d_ctx* c, const gs3d_camera* cam, uint32_t tiles_x, uint32_t tiles_y,
@@ -730,6 +860,7 @@ int gs3d_stage_emit(gs3d_ctx* c, const gs3d_projected_splat* projected, const ui
     ea.keys = dk;
     ea.values = dv;
     ea.hist = nullptr;
+    ea.hist_partial = nullptr;
     ea.n = n;
     ea.tiles_x = tiles_x;
     ea.capacity = K;
@@ -842,7 +973,8 @@ int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uin
     if (n_refs)
         HIP_TRY(c, hipMemcpyAsync(dv, sorted_values, (size_t)n_refs * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(dt, tile_offsets, (T + 1) * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, launch_rasterize(dp, dv, dt, tile_x, tile_y, tiles_x, tiles_y, width, height, dimg,
+    const uint32_t ppl = raster_variant_exists(tile_x, c->raster_ppl) ? c->raster_ppl : 1u;
+    HIP_TRY(c, launch_rasterize(dp, dv, dt, tile_x, tile_y, tiles_x, tiles_y, width, height, ppl, dimg,
                                 c->stream));
     HIP_TRY(c, launch_f16_to_f32(dimg, dimg32, (size_t)width * height * 4, c->stream));
     HIP_TRY(c, hipMemcpyAsync(image, dimg32, (size_t)width * height * 16, hipMemcpyDeviceToHost, c->stream));

@@ -52,6 +52,8 @@ _PROTOS = {
     "gs3d_set_tile_size": (I32, [P, U32, U32, P]),
     "gs3d_get_tile_size": (I32, [P, P]),
     "gs3d_get_tile_count": (I32, [P, P]),
+    "gs3d_set_kernel_variant": (I32, [P, C.c_char_p, C.c_char_p, I32]),
+    "gs3d_get_kernel_variant": (I32, [P, C.c_char_p, C.c_char_p, C.POINTER(I32)]),
     "gs3d_rebin": (I32, [P, P, U32, U32, U32]),
     "gs3d_ref_count": (I32, [P, C.POINTER(C.c_uint64)]),
     "gs3d_read": (I32, [P, I32, P, SZ, C.POINTER(SZ)]),
@@ -61,6 +63,13 @@ _PROTOS = {
     "gs3d_profiler_reset": (I32, [P]),
     "gs3d_stage_times_avg_us": (I32, [P, P, I32, C.POINTER(C.c_uint64)]),
     "gs3d_sort_passes": (I32, [P, C.POINTER(U32), P]),
+    "gs3d_buffer_create": (I32, [P, C.c_char_p, SZ]),
+    "gs3d_buffer_resize": (I32, [P, C.c_char_p, SZ]),
+    "gs3d_buffer_size": (I32, [P, C.c_char_p, C.POINTER(SZ)]),
+    "gs3d_buffer_write": (I32, [P, C.c_char_p, SZ, P, SZ, C.POINTER(SZ)]),
+    "gs3d_buffer_read": (I32, [P, C.c_char_p, SZ, P, SZ]),
+    "gs3d_buffer_destroy": (I32, [P, C.c_char_p]),
+    "gs3d_buffer_list": (I32, [P, P, P, SZ, C.POINTER(SZ)]),
     "gs3d_stage_preprocess": (I32, [P, P, U32, U32, F32, F32, P, P]),
     "gs3d_stage_scan": (I32, [P, P, U32, P, C.POINTER(U32)]),
     "gs3d_stage_emit": (I32, [P, P, P, U32, U32, P, P]),

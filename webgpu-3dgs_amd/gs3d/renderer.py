@@ -82,6 +82,15 @@ class GaussianSplatRenderer:
         self._check(self._lib.gs3d_get_tile_size(self._h, L.ptr(out)))
         return int(out[0]), int(out[1])
 
+    def set_kernel_variant(self, kernel: str, param: str, value: int) -> None:
+        self._check(self._lib.gs3d_set_kernel_variant(self._h, kernel.encode(), param.encode(), value))
+
+    def get_kernel_variant(self, kernel: str, param: str) -> int:
+        v = C.c_int()
+        self._check(self._lib.gs3d_get_kernel_variant(self._h, kernel.encode(), param.encode(),
+                                                      C.byref(v)))
+        return int(v.value)
+
     def get_tile_count(self) -> tuple[int, int]:
         out = np.zeros(2, np.uint32)
         self._check(self._lib.gs3d_get_tile_count(self._h, L.ptr(out)))

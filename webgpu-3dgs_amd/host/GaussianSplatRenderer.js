@@ -1,0 +1,186 @@
+'use strict';
+// GaussianSplatRenderer with the reference's API (src/renderers/GaussianSplatRenderer.ts):
+// init (:331-471), render with the rebin gate (:1033-1098), rebin (:1104-1266),
+// setTileSize / clampTileSize (:301-317, :162-167), setDebugMode (:321-323),
+// setAlwaysRebin (:327-329), getTileCount / getTileSize (:1268-1274) and
+// readTileCounts (:1278-1310). The passes run in libgs3d.so on one HIP stream;
+// this class keeps the host-side state machine and the resource registries.
+
+const { WorkgroupManager, linear1D, tiled1D, tile2D, BufferManager, PipelineManager } = require('./managers');
+
+const REBIN_PROFILE = 1;
+const BUF = {
+  PROJECTED_SPLATS: 0, SPLAT_REF_COUNTS: 1, SPLAT_REF_OFFSETS: 2, REF_COUNTER: 3, SORT_KEYS: 4,
+  SORT_VALUES: 5, TILE_OFFSETS: 6, IMAGE_RGBA16F: 7, IMAGE_RGBA32F: 8,
+};
+const MAX_TILES_PER_SPLAT = 16;  // Config.MAX_TILES_PER_SPLAT: the reference's key budget per splat
+
+function loadAddon() {
+  try {
+    return require('../addon/gs3d_napi.node');
+  } catch (e) {
+    throw new Error(`gs3d N-API addon not built (webgpu-3dgs_amd/addon/gs3d_napi.node): ${e.message}`);
+  }
+}
+
+class GaussianSplatRenderer {
+  constructor(scene, { device = 0, addon = loadAddon(), profiler = null } = {}) {
+    this.addon = addon;
+    this.ctx = addon.create(device);
+    this.scene = scene;
+    this.bufferManager = new BufferManager(addon, this.ctx);
+    this.pipelineManager = new PipelineManager();
+    this.workgroups = WorkgroupManager.fromDevice(addon, this.ctx);
+    this.profiler = profiler;
+    this.tileSizeX = 16;
+    this.tileSizeY = 16;
+    this.tileCountX = 1;
+    this.tileCountY = 1;
+    this.lastViewport = [-1, -1];
+    this.lastCameraVersion = -1;
+    this.alwaysRebin = false;
+    this.debugMode = 0;
+    this.debugRef = 256;
+    this.lastSplatCount = -1;
+    this.uploadedSplats = null;
+    this.rebinCount = 0;
+  }
+
+  // Launch-geometry layouts of the passes (registerBinningLayouts, :127-158).
+  registerBinningLayouts(splatCount) {
+    const maxRefs = Math.max(1, Math.max(1, splatCount) * MAX_TILES_PER_SPLAT);
+    this.workgroups.register({ name: 'splat-1d', problemSize: [splatCount, 1, 1], strategyFn: linear1D, strategyArgs: [256] });
+    this.workgroups.register({ name: 'radix', problemSize: [maxRefs, 1, 1], strategyFn: linear1D, strategyArgs: [256, 16] });
+    this.workgroups.register({ name: 'scan', problemSize: [splatCount, 1, 1], strategyFn: tiled1D, strategyArgs: [8, 4, 256] });
+    [this.tileSizeX, this.tileSizeY] = this.clampTileSize(this.scene.tiles[0], this.scene.tiles[1]);
+    this.workgroups.register({ name: 'rasterize', problemSize: [1, 1, 1], strategyFn: tile2D, strategyArgs: [this.tileSizeX, this.tileSizeY] });
+  }
+
+  clampTileSize(x, y) {
+    const lim = this.workgroups.limits;
+    const tx = Math.max(1, Math.min(Math.floor(x) || 1, lim.maxSizeX, lim.maxTotalThreads));
+    const ty = Math.max(1, Math.min(Math.floor(y) || 1, lim.maxSizeY, Math.floor(lim.maxTotalThreads / tx)));
+    return [tx, ty];
+  }
+
+  init(format = 'rgba16float') {
+    this.format = format;
+    const splatCount = Math.max(1, (this.scene.splats && this.scene.splats.splatCount) || 1);
+    this.registerBinningLayouts(splatCount);
+    this.lastSplatCount = -1;
+    for (const name of ['preprocess-splats', 'prefix-scan', 'emit-tile-refs', 'identify-tile-ranges']) {
+      this.pipelineManager.create({ name, type: 'compute' });
+    }
+    this.pipelineManager.create({ name: 'radix-sort', type: 'compute', codeConstants: { RADIX_BITS: 8 } });
+    this.pipelineManager.create({
+      name: 'rasterize-tiles', type: 'compute',
+      codeConstants: { TILE_X: this.tileSizeX, TILE_Y: this.tileSizeY, TILE_AREA: this.tileSizeX * this.tileSizeY },
+    });
+    this.addon.setTileSize(this.ctx, this.tileSizeX, this.tileSizeY);
+  }
+
+  // SceneSyncer.setSplatData (src/SceneSyncer.ts:25-43): upload the loader's arrays.
+  setSplatData(splats) {
+    const deg = splats.sphericalHarmonicsDegree || 0;
+    this.addon.setSplats(this.ctx, splats.positions, splats.scales, splats.rotations, splats.colors,
+      deg > 0 ? splats.fDc : null, deg > 0 ? splats.sphericalHarmonics : null, deg, splats.splatCount);
+    this.uploadedSplats = splats;
+    this.lastSplatCount = -1;
+  }
+
+  setTileSize(x, y) {
+    const [tx, ty] = this.clampTileSize(x, y);
+    if (tx === this.tileSizeX && ty === this.tileSizeY) return [tx, ty];
+    this.pipelineManager.update('rasterize-tiles', { codeConstants: { TILE_X: tx, TILE_Y: ty, TILE_AREA: tx * ty } });
+    this.addon.setTileSize(this.ctx, tx, ty);
+    this.tileSizeX = tx;
+    this.tileSizeY = ty;
+    this.scene.tiles = [tx, ty];
+    this.workgroups.update('rasterize', { strategyArgs: [tx, ty] });
+    this.lastViewport = [-1, -1];  // force the tile grid + target to be recomputed
+    return [tx, ty];
+  }
+
+  setDebugMode(mode) {
+    this.debugMode = mode | 0;
+  }
+
+  setAlwaysRebin(on) {
+    this.alwaysRebin = !!on;
+  }
+
+  // frame: {width, height, camera (Camera), cameraVersion?}
+  render(frame) {
+    const splats = this.scene.splats;
+    if (!splats) return false;
+    if (this.uploadedSplats !== splats) this.setSplatData(splats);
+    const splatCount = splats.splatCount;
+    const width = Math.max(1, frame.width | 0);
+    const height = Math.max(1, frame.height | 0);
+    const tilesX = Math.ceil(width / this.tileSizeX);
+    const tilesY = Math.ceil(height / this.tileSizeY);
+    this.tileCountX = tilesX;
+    this.tileCountY = tilesY;
+    const splatCountChanged = splatCount !== this.lastSplatCount;
+    if (splatCountChanged) {
+      this.workgroups.update('splat-1d', { problemSize: [splatCount, 1, 1] });
+      this.lastSplatCount = splatCount;
+    }
+    const viewportChanged = width !== this.lastViewport[0] || height !== this.lastViewport[1];
+    if (viewportChanged) this.lastViewport = [width, height];
+    const version = frame.cameraVersion !== undefined ? frame.cameraVersion : frame.camera.version;
+    const cameraMoved = version !== this.lastCameraVersion;
+    this.lastCameraVersion = version;
+    const needsRebin = this.alwaysRebin || cameraMoved || splatCountChanged || viewportChanged;
+    if (needsRebin) this.rebin(frame.camera, splatCount, tilesX, tilesY, width, height);
+    return needsRebin;
+  }
+
+  rebin(camera, splatCount, tilesX, tilesY, width, height) {
+    const flags = this.profiler ? REBIN_PROFILE : 0;
+    this.addon.rebin(this.ctx, camera.getUniformData(width, height), width, height, flags);
+    this.rebinCount++;
+  }
+
+  getTileCount() {
+    return [this.tileCountX, this.tileCountY];
+  }
+
+  getTileSize() {
+    return [this.tileSizeX, this.tileSizeY];
+  }
+
+  // Per-tile splat counts with the underflow guard; null on failure.
+  async readTileCounts() {
+    try {
+      return this.addon.readTileCounts(this.ctx);
+    } catch (e) {
+      return null;
+    }
+  }
+
+  readBuffer(id) {
+    return this.addon.read(this.ctx, id);
+  }
+
+  // Premultiplied (C, 1 - T) target, rgba16float raw halves or widened to f32.
+  readImage(asFloat32 = true) {
+    const ab = this.addon.read(this.ctx, asFloat32 ? BUF.IMAGE_RGBA32F : BUF.IMAGE_RGBA16F);
+    return asFloat32 ? new Float32Array(ab) : new Uint16Array(ab);
+  }
+
+  refCount() {
+    return this.addon.refCount(this.ctx);
+  }
+
+  sync() {
+    this.addon.sync(this.ctx);
+  }
+
+  destroy() {
+    if (this.ctx) this.addon.destroy(this.ctx);
+    this.ctx = null;
+  }
+}
+
+module.exports = { GaussianSplatRenderer, loadAddon, BUF, REBIN_PROFILE };
