@@ -32,17 +32,40 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(n, k, tiles, pixels, passes, deg):
-    """Bytes each stage must move at minimum, on the canonical layouts."""
+def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
+    """Bytes each stage must move at minimum, per launch group of one frame.
+
+    preprocess: vec4 inputs + SH coefficients in, 64-B projected splat + count
+    + depth key out; depth sort: per pass a u32 key + u32 id read and written
+    (the first pass reads keys only), plus one read of the keys to histogram;
+    scan: order + gathered count in, offset out; emit: offsets, order and 32 B
+    of the projected splat in, 12-B (key, id) out; key sort: 24 B per key per
+    pass; identify: one (T+1)-entry write plus the sorted tile words it
+    searches; raster: the 48 B used of each listed splat, tile ranges, 8-B
+    pixel out.
+    """
     pre_in = 64 + (12 * (deg + 1) ** 2 if deg else 0)
     return {
-        "preprocess-splats": n * (pre_in + 68),
-        "prefix-scan": 8 * n + 4,
-        "emit-tile-refs": 68 * n + 12 * k,
-        "radix-sort": passes * 24 * k,
+        "preprocess-splats": n * (pre_in + 68 + 4),
+        "radix-sort-splats": n * (4 + 12 + 16 * (depth_passes - 1)) if depth_passes else 0,
+        "prefix-scan": 12 * n + 4,
+        "emit-tile-refs": 44 * n + 12 * k,
+        "radix-sort": tile_passes * 24 * k,
         "identify-tile-ranges": 8 * k + 4 * (tiles + 1),
-        "rasterize-tiles": 68 * k + 8 * (tiles + 1) + 8 * pixels,
+        "rasterize-tiles": 48 * k + 8 * (tiles + 1) + 8 * pixels,
     }
+
+
+def load_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE (calibrated against a plain copy), from profiles/traffic.json."""
+    f = REPO / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get(workload, {}).get(kernel)
+    except (ValueError, AttributeError):
+        return None
 
 
 def cpu_baseline(cam, splats, W, H, tile, seconds):
@@ -81,6 +104,7 @@ def main() -> int:
         return 2
 
     from gs3d import camera, scene
+    from gs3d._lib import STAGES
     from gs3d.renderer import GaussianSplatRenderer
 
     t = time.perf_counter()
@@ -100,24 +124,39 @@ def main() -> int:
     k = r.ref_count()
     log(f"[bench] warmup done: K={k} keys, passes at bits {r.sort_passes()}")
 
-    # timed region: K back-to-back rebins, hipEvent pair per stage on the
-    # context's stream (the stream every kernel is launched on)
+    # Stage breakdown: a short run with a hipEvent pair around every stage.
+    # Each event pair adds a dispatch gap, so these frames are not the timed ones.
+    r.set_profile_mask(STAGES)
+    r.profiler_reset()
+    for _ in range(20):
+        r.rebin(cu, W, H, 1)  # GS3D_REBIN_PROFILE
+    stages, _ = r.stage_times_avg_us()
+    dominant = max(STAGES[:-1], key=lambda n: stages[n])
+
+    # Timed region: K back-to-back rebins; every 8th one brackets the dominant
+    # kernel with a hipEvent pair on the context's stream (the stream every
+    # kernel is launched on), the rest queue no events at all.
+    r.set_profile_mask([dominant])
     r.profiler_reset()
     r.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r.rebin(cu, W, H, 1)  # GS3D_REBIN_PROFILE
+    for i in range(args.steps):
+        r.rebin(cu, W, H, 1 if i % 8 == 0 else 0)
     r.sync()
     el = time.perf_counter() - t0
-    stages, frames = r.stage_times_avg_us()
+    timed, frames = r.stage_times_avg_us()
+    dominant_us = timed[dominant]
     ms = el * 1000.0 / args.steps
-    log(f"[bench] {args.steps} steps in {el:.3f} s: {ms:.3f} ms/frame; stages (us) "
+    log(f"[bench] {args.steps} steps in {el:.3f} s: {ms:.3f} ms/frame; {dominant} {dominant_us:.1f} us "
+        f"over {frames} sampled frames; profiled breakdown (us) "
         + ", ".join(f"{n}={v:.1f}" for n, v in stages.items()))
 
     tiles_x, tiles_y = r.get_tile_count()
     T = tiles_x * tiles_y
-    passes = len(r.sort_passes())
-    algo = algorithmic_bytes(splats.n, k, T, W * H, passes, splats.sh_degree)
+    shifts = r.sort_passes()
+    depth_passes = sum(1 for b in shifts if b < 32)
+    passes = len(shifts) - depth_passes
+    algo = algorithmic_bytes(splats.n, k, T, W * H, depth_passes, passes, splats.sh_degree)
 
     cpu = None
     visited = None
@@ -133,14 +172,27 @@ def main() -> int:
         per_stage[name] = {"us": round(us, 2), "algo_bytes": int(b), "GB/s": round(gbs, 1),
                            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     if visited:
-        us = stages["rasterize-tiles"]
+        us = dominant_us if dominant == "rasterize-tiles" else stages["rasterize-tiles"]
         tf = visited * RASTER_FLOP_PER_PAIR / (us * 1e-6) / 1e12
         per_stage["rasterize-tiles"].update({"pairs": visited, "TFLOP/s": round(tf, 2),
                                              "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
-    dominant = max(algo, key=lambda n: stages.get(n, 0.0))
     d = per_stage[dominant]
-    roofline = {"kernel": dominant, "bound": "hbm", "achieved": d["GB/s"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": d["hbm_frac"], "traffic": None}
+    d["us_timed_region"] = round(dominant_us, 2)
+    d["GB/s"] = round(algo[dominant] / (dominant_us * 1e-6) / 1e9, 1)
+    d["hbm_frac"] = round(d["GB/s"] / HBM_PEAK_GBS, 4)
+    traffic = load_traffic(wl.name, dominant)
+    if dominant == "rasterize-tiles" and visited:
+        # The rasterizer is bound by fp32 vector issue, not HBM: algorithmic
+        # flops = oracle pixel-splat pairs x 23 over the kernel time, against
+        # MI355X's fp32 peak.
+        tf = visited * RASTER_FLOP_PER_PAIR / (dominant_us * 1e-6) / 1e12
+        roofline = {"kernel": dominant, "bound": "valu", "achieved": round(tf, 2),
+                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                    "hbm_frac": d["hbm_frac"]}
+    else:
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": d["GB/s"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": d["hbm_frac"], "traffic": traffic}
     fps = 1000.0 / ms
     out = {
         "metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1,
@@ -149,10 +201,13 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl.name, "splats": splats.n, "sh_degree": splats.sh_degree,
                    "width": W, "height": H, "tile": args.tile, "keys": k, "tiles": T,
-                   "radix_passes": passes, "digit_bits": 8},
-        "mkeys_per_s": round(k / (stages["radix-sort"] * 1e-6) / 1e6, 1) if stages["radix-sort"] else None,
+                   "radix_passes": {"splats": depth_passes, "keys": passes}, "digit_bits": 8},
+        "mkeys_per_s": round(k / ((stages["radix-sort"] + stages["radix-sort-splats"]) * 1e-6) / 1e6, 1)
+        if stages["radix-sort"] else None,
         "stages_us": {n: round(v, 2) for n, v in stages.items()},
-        "profiled_frames": frames,
+        "stages_note": "stages_us from 20 frames with an event pair per stage (each pair adds a "
+                       "dispatch gap); ms_per_step from frames without them",
+        "dominant_sampled_frames": frames,
         "per_stage": per_stage,
         "roofline": roofline,
         "cpu_baseline": cpu,

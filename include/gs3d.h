@@ -93,17 +93,18 @@ enum gs3d_buffer_id {
 #define GS3D_REBIN_PROFILE     1u   /* record a hipEvent pair per stage           */
 #define GS3D_REBIN_KEEP_UNSORTED 2u /* keep a copy of the emit-order keys/values  */
 
-/* Stage labels of gs3d_stage_times_us (reference Profiler labels where a
- * reference pass exists, GaussianSplatRenderer.ts:1143-1265). */
+/* Stage labels of gs3d_stage_times_us, in execution order (reference Profiler
+ * labels where a reference pass exists, GaussianSplatRenderer.ts:1143-1265). */
 enum gs3d_stage {
-    GS3D_STAGE_PREPROCESS = 0,  /* "preprocess-splats"                       */
-    GS3D_STAGE_SCAN       = 1,  /* "prefix-scan"  (local+blocks+add fused)   */
-    GS3D_STAGE_EMIT       = 2,  /* "emit-tile-refs" (+ digit histograms)     */
-    GS3D_STAGE_SORT       = 3,  /* "radix-sort"  (all onesweep passes)       */
-    GS3D_STAGE_IDENTIFY   = 4,  /* "identify-tile-ranges"                    */
-    GS3D_STAGE_RASTER     = 5,  /* "rasterize-tiles"                         */
-    GS3D_STAGE_TOTAL      = 6,  /* whole rebin, first launch to last         */
-    GS3D_STAGE__COUNT     = 7
+    GS3D_STAGE_PREPROCESS = 0,  /* "preprocess-splats"                            */
+    GS3D_STAGE_DEPTH_SORT = 1,  /* "radix-sort-splats": depth digits, on splats   */
+    GS3D_STAGE_SCAN       = 2,  /* "prefix-scan"  (local+blocks+add fused)        */
+    GS3D_STAGE_EMIT       = 3,  /* "emit-tile-refs" (+ tile-digit histograms)     */
+    GS3D_STAGE_SORT       = 4,  /* "radix-sort": tile digits, on the keys         */
+    GS3D_STAGE_IDENTIFY   = 5,  /* "identify-tile-ranges"                         */
+    GS3D_STAGE_RASTER     = 6,  /* "rasterize-tiles"                              */
+    GS3D_STAGE_TOTAL      = 7,  /* whole rebin, first launch to last              */
+    GS3D_STAGE__COUNT     = 8
 };
 
 typedef struct gs3d_ctx gs3d_ctx;
@@ -185,6 +186,10 @@ const char* gs3d_stage_label(int stage);
  * (the Profiler's averaging window, Profiler.ts:231-251); `frames` gets the
  * number of frames averaged. */
 int gs3d_profiler_reset(gs3d_ctx* ctx);
+/* Which stages a GS3D_REBIN_PROFILE rebin brackets with events (bit i = enum
+ * gs3d_stage i; default all). Each event pair adds a dispatch gap, so a caller
+ * timing whole frames profiles only what it needs. Unrecorded stages read 0. */
+int gs3d_set_profile_mask(gs3d_ctx* ctx, uint32_t stage_mask);
 int gs3d_stage_times_avg_us(gs3d_ctx* ctx, float* out, int n, uint64_t* frames);
 
 /* Number of radix passes the last rebin's sort ran and the bit offset of each

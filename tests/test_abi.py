@@ -28,7 +28,7 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_abi_version_and_labels():
     lib = L.lib()
     assert lib.gs3d_abi_version() == 1
-    assert [lib.gs3d_stage_label(i).decode() for i in range(7)] == list(L.STAGES)
+    assert [lib.gs3d_stage_label(i).decode() for i in range(8)] == list(L.STAGES)
     assert lib.gs3d_stage_label(99) == b""
 
 

@@ -55,6 +55,10 @@ struct PreprocessArgs {
     const float* sh;  // SoA, see above; null when sh_degree == 0
     gs3d_projected_splat* out;
     uint32_t* counts;
+    uint32_t* depth_keys;  // bits(-depth) of visible splats, 0xFFFFFFFF for culled; or null
+    uint32_t* counters;    // if set: counters[0..counter_words) = 0 except [count_slot] = n
+    uint32_t counter_words;
+    uint32_t count_slot;
     uint32_t n;
     int sh_degree;
     uint32_t tiles_x, tiles_y;
@@ -63,7 +67,8 @@ struct PreprocessArgs {
 
 struct EmitArgs {
     const gs3d_projected_splat* projected;
-    const uint32_t* offsets;  // n + 1 entries
+    const uint32_t* order;    // position -> splat id (null: identity)
+    const uint32_t* offsets;  // n + 1 entries, per position
     uint64_t* keys;
     uint32_t* values;
     uint32_t* hist;           // [passes][radix] digit histograms (accumulated), or null
@@ -71,8 +76,9 @@ struct EmitArgs {
     uint32_t n;
     uint32_t tiles_x;
     uint64_t capacity;        // keys/values entries allocated
-    uint32_t n_passes;        // histogram of pass p is over the digit at bit p * digit_bits
+    uint32_t n_passes;        // histogram of pass p: digit at bit hist_shift0 + p * digit_bits
     uint32_t digit_bits;
+    uint32_t hist_shift0;
 };
 
 // Emit runs at most this many workgroups (grid-stride over 256-splat chunks),
@@ -80,27 +86,59 @@ struct EmitArgs {
 constexpr uint32_t kEmitMaxBlocks = 1024;
 uint32_t emit_grid(uint32_t n);
 
+// Lanes of the wavefront whose BITS-bit digit equals this lane's, among lanes
+// with `valid` set (wave-level match by one ballot per digit bit). Written on
+// 32-bit halves so each bit costs one bfe + one XNOR-AND per half.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+    const uint64_t vm = __ballot(valid);
+    uint32_t lo = (uint32_t)vm, hi = (uint32_t)(vm >> 32);
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        const uint64_t bal = __ballot((d >> b) & 1u);
+        const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // all ones if bit b set
+        lo &= ~((uint32_t)bal ^ m);
+        hi &= ~((uint32_t)(bal >> 32) ^ m);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t match_digit_rt(uint32_t d, bool valid, uint32_t bits) {
+    return bits == 8 ? match_digit<8>(d, valid) : match_digit<4>(d, valid);
+}
+
 // ---- launchers (each enqueues on `stream`, returns the HIP status) --------
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_sh_relayout(const float* f_dc, const float* rest, float* sh, uint32_t n, int deg,
                               hipStream_t s);
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s);
-hipError_t launch_scan(const uint32_t* counts, uint32_t* offsets, uint32_t* ref_counter,
-                       uint64_t* status, uint32_t* ticket, uint32_t n, uint32_t epoch, hipStream_t s);
+// Exclusive scan of counts[order[i]] (order null: counts[i]).
+hipError_t launch_scan(const uint32_t* counts, const uint32_t* order, uint32_t* offsets,
+                       uint32_t* ref_counter, uint64_t* status, uint32_t* ticket, uint32_t n,
+                       uint32_t epoch, hipStream_t s);
 hipError_t launch_emit(const EmitArgs& a, hipStream_t s);
-hipError_t launch_digit_histogram(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
-                                  uint32_t* hist, uint32_t digit_bits, uint32_t n_passes,
-                                  const uint32_t* shifts, hipStream_t s);
+// hist[p][d] += #keys (of n = *n_dev, n <= n_max) with digit d at bit
+// shift0 + p*digit_bits. hist_partial: key_hist_rows(n_max) * n_passes * radix words.
+uint32_t key_hist_rows(uint32_t n_max);
+hipError_t launch_key_histogram(const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
+                                uint32_t n_max, uint32_t* hist, uint32_t* hist_partial,
+                                uint32_t digit_bits, uint32_t n_passes, uint32_t shift0,
+                                hipStream_t s);
 // Status words one radix pass needs: radix_status_tiles(n_max) x RADIX.
 uint32_t radix_status_tiles(uint32_t n_max);
-// One stable LSD pass over n = *n_dev keys (grid covers n_max; n > n_max: no-op).
-hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
-                             uint32_t* vout, const uint32_t* n_dev, uint32_t n_max,
-                             const uint32_t* hist_pass, uint64_t* status, uint32_t* ticket,
-                             uint32_t shift, uint32_t digit_bits, uint32_t epoch, hipStream_t s);
+// One stable LSD pass over n = *n_dev (key, value) pairs, keys of key_bytes
+// (4 or 8); with iota_values the input values are the positions 0..n-1 and vin
+// is not read. The grid covers n_max; n > n_max is a no-op.
+hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
+                             uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
+                             uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
+                             uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
+                             hipStream_t s);
 // tile_offsets[0..tile_count] from n = *n_dev sorted keys (n > n_max: all 0).
+// If k_host is set (host-mapped memory), *k_host = *n_dev.
 hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
-                                 uint32_t* tile_offsets, uint32_t tile_count, hipStream_t s);
+                                 uint32_t* tile_offsets, uint32_t tile_count, uint32_t* k_host,
+                                 hipStream_t s);
 // Square tiles of 8, 16 or 32 px; pixels_per_lane 1, 2 or 4 (a workgroup of
 // tile*tile/ppl threads, at least one wavefront).
 bool raster_variant_exists(uint32_t tile, uint32_t pixels_per_lane);

@@ -28,21 +28,22 @@ constexpr uint32_t kOwnerWindow = 4096;
 
 __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint32_t* s_start,
                                            uint32_t* s_tmin_x, uint32_t* s_tmin_y,
-                                           uint32_t* s_width, uint32_t* s_keylo, uint8_t* s_owner,
-                                           uint32_t* s_hist) {
+                                           uint32_t* s_width, uint32_t* s_keylo, uint32_t* s_sid,
+                                           uint8_t* s_owner, uint32_t* s_hist) {
     const int tid = threadIdx.x;
     const uint32_t i = i0 + tid;
     const uint32_t last = min(i0 + kEmitThreads, a.n);
     const uint32_t radix = 1u << a.digit_bits;
     const uint32_t block_start = a.offsets[i0];
     const uint32_t block_end = a.offsets[last];
-    uint32_t my_start = 0, my_count = 0;
+    uint32_t my_start = 0, my_count = 0, sid = i;
     if (i < a.n) {
         const uint32_t off = a.offsets[i];
         my_count = a.offsets[i + 1] - off;  // > 0 exactly for valid splats
         my_start = off - block_start;
+        if (a.order) sid = a.order[i];
         if (my_count) {
-            const float4* ps = reinterpret_cast<const float4*>(a.projected + i);
+            const float4* ps = reinterpret_cast<const float4*>(a.projected + sid);
             const float4 f0 = ps[0];
             const float4 f3 = ps[3];
             const uint32_t t0x = __float_as_uint(f3.x), t0y = __float_as_uint(f3.y);
@@ -54,6 +55,7 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
         }
     }
     s_start[tid] = my_start;
+    s_sid[tid] = sid;
 
     const uint32_t total = block_end - block_start;
     const int lane = tid & (kWave - 1);
@@ -81,23 +83,17 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
                 const uint64_t pos = (uint64_t)block_start + w0 + jw;
                 if (pos < a.capacity) {
                     a.keys[pos] = key;
-                    a.values[pos] = i0 + o;
+                    a.values[pos] = s_sid[o];
                 }
             }
             if (a.hist_partial) {
                 // Lanes with equal digits are grouped by ballots (a wave-level
                 // multisplit); only each group's lowest lane touches LDS, so the
                 // digits that repeat across a wavefront (depth exponent, tile
-                // row) do not serialize on one LDS address. Pass p's digit sits
-                // at bit p * digit_bits.
+                // row) do not serialize on one LDS address.
                 for (uint32_t p = 0; p < a.n_passes; p++) {
-                    const uint32_t d = (uint32_t)(key >> (p * a.digit_bits)) & (radix - 1u);
-                    uint64_t peers = __ballot(live);
-                    for (uint32_t b = 0; b < a.digit_bits; b++) {
-                        const bool bit = (d >> b) & 1u;
-                        const uint64_t bal = __ballot(bit);
-                        peers &= bit ? bal : ~bal;
-                    }
+                    const uint32_t d = (uint32_t)(key >> (a.hist_shift0 + p * a.digit_bits)) & (radix - 1u);
+                    const uint64_t peers = match_digit_rt(d, live, a.digit_bits);
                     if (live && (peers & lt_mask) == 0ull)
                         atomicAdd(&s_hist[p * radix + d], (uint32_t)__popcll(peers));
                 }
@@ -116,6 +112,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
     __shared__ uint32_t s_tmin_y[kEmitThreads];
     __shared__ uint32_t s_width[kEmitThreads];
     __shared__ uint32_t s_keylo[kEmitThreads];
+    __shared__ uint32_t s_sid[kEmitThreads];
     __shared__ uint8_t s_owner[kOwnerWindow];
     __shared__ uint32_t s_hist[kMaxPasses * 256];
     const uint32_t radix = 1u << a.digit_bits;
@@ -123,7 +120,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
     for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) s_hist[k] = 0u;
     const uint32_t chunks = (a.n + kEmitThreads - 1) / kEmitThreads;
     for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x)
-        emit_chunk(a, c * kEmitThreads, s_start, s_tmin_x, s_tmin_y, s_width, s_keylo, s_owner, s_hist);
+        emit_chunk(a, c * kEmitThreads, s_start, s_tmin_x, s_tmin_y, s_width, s_keylo, s_sid, s_owner,
+                   s_hist);
     if (a.hist_partial) {
         __syncthreads();
         uint32_t* row = a.hist_partial + (size_t)blockIdx.x * hist_words;
@@ -145,27 +143,40 @@ __global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* __restrict_
     if (sum) atomicAdd(&hist[w], sum);
 }
 
-// Digit histograms of n = *n_dev keys for every pass (standalone sort path).
-__global__ __launch_bounds__(256) void k_digit_histogram(const uint64_t* keys, const uint32_t* n_dev,
-                                                         uint32_t* hist, uint32_t digit_bits,
-                                                         uint32_t n_passes, const uint32_t* shifts_dev) {
+// Digit histograms of n = *n_dev keys for every pass, one plain row per
+// workgroup (grid-stride; equal digits within a wavefront aggregated by
+// ballots as in k_emit).
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ keys,
+                                                       const uint32_t* __restrict__ n_dev,
+                                                       uint32_t n_max, uint32_t* __restrict__ partial,
+                                                       uint32_t digit_bits, uint32_t n_passes,
+                                                       uint32_t shift0) {
     __shared__ uint32_t s_hist[kMaxPasses * 256];
-    __shared__ uint32_t s_shift[kMaxPasses];
     const uint32_t radix = 1u << digit_bits;
     const uint32_t words = n_passes * radix;
     for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) s_hist[k] = 0u;
-    if (threadIdx.x < n_passes) s_shift[threadIdx.x] = shifts_dev[threadIdx.x];
     __syncthreads();
-    const uint32_t n = *n_dev;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t key = keys[i];
-        for (uint32_t p = 0; p < n_passes; p++)
-            atomicAdd(&s_hist[p * radix + ((uint32_t)(key >> s_shift[p]) & (radix - 1u))], 1u);
+    uint32_t n = *n_dev;
+    if (n > n_max) n = 0;  // keys did not fit: count nothing, the frame is re-run
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t rounds = (n + stride - 1) / stride;
+    for (size_t r = 0; r < rounds; r++) {
+        const size_t i = r * stride + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const bool live = i < n;
+        const KeyT key = live ? keys[i] : (KeyT)0;
+        for (uint32_t p = 0; p < n_passes; p++) {
+            const uint32_t d = (uint32_t)(key >> (shift0 + p * digit_bits)) & (radix - 1u);
+            const uint64_t peers = match_digit_rt(d, live, digit_bits);
+            if (live && (peers & lt_mask) == 0ull)
+                atomicAdd(&s_hist[p * radix + d], (uint32_t)__popcll(peers));
+        }
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x)
-        if (s_hist[k]) atomicAdd(&hist[k], s_hist[k]);
+    uint32_t* row = partial + (size_t)blockIdx.x * words;
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) row[k] = s_hist[k];
 }
 
 }  // namespace
@@ -189,15 +200,33 @@ hipError_t launch_emit(const EmitArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_digit_histogram(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
-                                  uint32_t* hist, uint32_t digit_bits, uint32_t n_passes,
-                                  const uint32_t* shifts, hipStream_t s) {
-    if (n_max == 0) return hipSuccess;
+uint32_t key_hist_rows(uint32_t n_max) {
+    const uint32_t blocks = (n_max + 255u) / 256u;
+    return blocks < 512u ? (blocks ? blocks : 1u) : 512u;
+}
+
+hipError_t launch_key_histogram(const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
+                                uint32_t n_max, uint32_t* hist, uint32_t* hist_partial,
+                                uint32_t digit_bits, uint32_t n_passes, uint32_t shift0,
+                                hipStream_t s) {
+    if (n_max == 0 || n_passes == 0) return hipSuccess;
     if (n_passes > kMaxPasses || (digit_bits != 4 && digit_bits != 8)) return hipErrorInvalidValue;
-    uint32_t grid = (n_max + 255u) / 256u;
-    if (grid > 2048u) grid = 2048u;
-    hipLaunchKernelGGL(k_digit_histogram, dim3(grid), dim3(256), 0, s, keys, n_dev, hist, digit_bits,
-                       n_passes, shifts);
+    const uint32_t grid = key_hist_rows(n_max);
+    if (key_bytes == 8) {
+        hipLaunchKernelGGL(k_key_histogram<uint64_t>, dim3(grid), dim3(256), 0, s,
+                           static_cast<const uint64_t*>(keys), n_dev, n_max, hist_partial, digit_bits,
+                           n_passes, shift0);
+    } else if (key_bytes == 4) {
+        hipLaunchKernelGGL(k_key_histogram<uint32_t>, dim3(grid), dim3(256), 0, s,
+                           static_cast<const uint32_t*>(keys), n_dev, n_max, hist_partial, digit_bits,
+                           n_passes, shift0);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    const uint32_t words = n_passes * (1u << digit_bits);
+    const uint32_t slices = grid < 32u ? grid : 32u;
+    hipLaunchKernelGGL(k_hist_reduce, dim3((words + 255u) / 256u, slices), dim3(256), 0, s,
+                       hist_partial, grid, words, hist);
     return hipGetLastError();
 }
 

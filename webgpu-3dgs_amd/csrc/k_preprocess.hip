@@ -88,6 +88,9 @@ __device__ __forceinline__ void sh_color(const PreprocessArgs& a, uint32_t i, fl
 template <int DEG>
 __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    // the frame's counters (tickets, histograms, K) start from zero; nothing
+    // reads them before this kernel ends
+    if (a.counters && i < a.counter_words) a.counters[i] = (i == a.count_slot) ? a.n : 0u;
     if (i >= a.n) return;
     float4* out = reinterpret_cast<float4*>(a.out + i);
 
@@ -203,6 +206,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     out[2] = o2;
     out[3] = o3;
     a.counts[i] = count;
+    if (a.depth_keys) a.depth_keys[i] = count ? __float_as_uint(-o0.z) : 0xFFFFFFFFu;
 }
 
 // f_dc (N x 3) + rest (N x 3R, channel-major per splat) -> coefficient-major SoA.
@@ -221,8 +225,9 @@ __global__ __launch_bounds__(256) void k_sh_relayout(const float* f_dc, const fl
 }  // namespace
 
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
-    const uint32_t grid = (a.n + 255u) / 256u;
+    const uint32_t threads = a.n > (a.counters ? a.counter_words : 0u) ? a.n : a.counter_words;
+    if (threads == 0) return hipSuccess;
+    const uint32_t grid = (threads + 255u) / 256u;
     switch (a.sh_degree) {
         case 0: hipLaunchKernelGGL(k_preprocess<0>, dim3(grid), dim3(256), 0, s, a); break;
         case 1: hipLaunchKernelGGL(k_preprocess<1>, dim3(grid), dim3(256), 0, s, a); break;

@@ -35,10 +35,12 @@ namespace {
 __global__ __launch_bounds__(256) void k_identify(const uint64_t* __restrict__ keys,
                                                   const uint32_t* __restrict__ n_dev,
                                                   uint32_t* __restrict__ tile_offsets,
-                                                  uint32_t tile_count, uint32_t n_max) {
+                                                  uint32_t tile_count, uint32_t n_max,
+                                                  uint32_t* __restrict__ k_host) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t > tile_count) return;
     uint32_t n = *n_dev;
+    if (t == 0 && k_host) *k_host = n;  // K for the host, no copy packet needed
     if (n > n_max) n = 0;  // keys did not fit: draw nothing, the host re-runs the frame
     const uint32_t* kw = reinterpret_cast<const uint32_t*>(keys);
     uint32_t out = 0;
@@ -222,10 +224,11 @@ __global__ void k_f16_to_f32(const uint16_t* in, float* out, size_t n) {
 }  // namespace
 
 hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
-                                 uint32_t* tile_offsets, uint32_t tile_count, hipStream_t s) {
+                                 uint32_t* tile_offsets, uint32_t tile_count, uint32_t* k_host,
+                                 hipStream_t s) {
     const uint32_t threads = tile_count + 1u;
     hipLaunchKernelGGL(k_identify, dim3((threads + 255u) / 256u), dim3(256), 0, s, keys, n_dev,
-                       tile_offsets, tile_count, n_max);
+                       tile_offsets, tile_count, n_max, k_host);
     return hipGetLastError();
 }
 

@@ -8,7 +8,8 @@
 // aggregate, looks back over its predecessors' status words a wavefront (64
 // predecessors) at a time, then writes final offsets. The result is the same
 // exclusive sum mod 2^32 with offsets[N] = ref_counter[0] = total.
-// HBM traffic: 4 B in + 4 B out per splat.
+// With `order` it scans counts[order[i]] (the counts in depth order).
+// HBM traffic: 4 B in + 4 B out per splat (+ 4 B of order).
 #include "gs3d_internal.h"
 
 namespace gs3d {
@@ -34,6 +35,7 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ order,
                                                        uint32_t* __restrict__ offsets,
                                                        uint32_t* __restrict__ ref_counter,
                                                        uint64_t* status, uint32_t* ticket,
@@ -50,7 +52,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restric
     const size_t base = (size_t)part * kScanTile + (size_t)tid * kScanItems;
 
     uint32_t v[kScanItems];
-    if (base + kScanItems <= n) {
+    if (order) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? counts[order[base + k]] : 0u;
+    } else if (base + kScanItems <= n) {
         const uint4* src = reinterpret_cast<const uint4*>(counts + base);
         uint4 x0 = src[0], x1 = src[1];
         v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
@@ -138,12 +143,12 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t value, size_t n) {
 
 }  // namespace
 
-hipError_t launch_scan(const uint32_t* counts, uint32_t* offsets, uint32_t* ref_counter,
-                       uint64_t* status, uint32_t* ticket, uint32_t n, uint32_t epoch,
-                       hipStream_t s) {
+hipError_t launch_scan(const uint32_t* counts, const uint32_t* order, uint32_t* offsets,
+                       uint32_t* ref_counter, uint64_t* status, uint32_t* ticket, uint32_t n,
+                       uint32_t epoch, hipStream_t s) {
     const uint32_t parts = n == 0 ? 1u : (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan, dim3(parts), dim3(kScanThreads), 0, s, counts, offsets, ref_counter,
-                       status, ticket, n, epoch);
+    hipLaunchKernelGGL(k_scan, dim3(parts), dim3(kScanThreads), 0, s, counts, order, offsets,
+                       ref_counter, status, ticket, n, epoch);
     return hipGetLastError();
 }
 

@@ -61,17 +61,19 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t x, uint32_
 // a tile publishes all its digits with one coalesced store.
 constexpr int kLookbackGroup = 8;
 
-template <int BITS>
+template <int BITS, typename KeyT, bool IOTA>
 __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
+    const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
     const uint32_t* __restrict__ hist_pass, uint64_t* status, uint32_t* ticket, uint32_t n_max,
     uint32_t shift, uint32_t epoch) {
     constexpr int RADIX = 1 << BITS;
     constexpr int WAVES = kSortThreads / kWave;
     constexpr uint32_t MASK = RADIX - 1;
-    // keys are staged first, then values, through the same 32 KB
+    // keys are staged first, then values, through the same 32 KB; the digit of
+    // every sorted position is kept so the value pass needs no extra registers
     __shared__ uint64_t s_stage[kSortTile];
+    __shared__ uint8_t s_digit[kSortTile];
     __shared__ uint32_t s_wave_hist[WAVES][RADIX];
     __shared__ uint32_t s_bin_local[RADIX];
     __shared__ uint32_t s_bin_global[RADIX];
@@ -92,33 +94,24 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     if (tile_base >= n || n > n_max) return;
 
     const size_t wave_base = tile_base + (size_t)wave * (kWave * kSortItems);
-    uint64_t k[kSortItems];
-    uint32_t v[kSortItems];
+    KeyT k[kSortItems];
     uint32_t rank[kSortItems];
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
-        k[it] = idx < n ? kin[idx] : ~0ull;
-    }
-#pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        const size_t idx = wave_base + (size_t)it * kWave + lane;
-        v[it] = idx < n ? vin[idx] : 0u;
+        k[it] = idx < n ? kin[idx] : (KeyT)~(KeyT)0;
     }
 
+    // Wave-level multisplit: per item, BITS ballots narrow each lane's group of
+    // equal digits; rank = prior count of the digit in this wavefront + lanes
+    // of the group below this one; the group's lowest lane bumps the count.
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
         const bool valid = idx < n;
         const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < BITS; b++) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
+        const uint64_t peers = match_digit<BITS>(d, valid);
         uint32_t r = 0;
         if (valid) {
             const uint64_t below = peers & lt_mask;
@@ -179,6 +172,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     __syncthreads();
 
     // keys: digit-sorted into LDS, then out in LDS order
+    KeyT* s_keys = reinterpret_cast<KeyT*>(s_stage);
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
@@ -186,21 +180,18 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
         if (idx < n) {
             const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
             rank[it] += s_bin_local[d] + s_wave_hist[wave][d];  // position in the tile
-            s_stage[rank[it]] = k[it];
+            s_keys[rank[it]] = k[it];
+            s_digit[rank[it]] = (uint8_t)d;
         }
     }
     __syncthreads();
     const uint32_t tile_n = (uint32_t)min((size_t)kSortTile, (size_t)n - tile_base);
-    uint32_t dst[kSortItems];
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const uint32_t j = (uint32_t)(it * kSortThreads + tid);
-        dst[it] = 0;
         if (j < tile_n) {
-            const uint64_t key = s_stage[j];
-            const uint32_t d = (uint32_t)(key >> shift) & MASK;
-            dst[it] = s_bin_global[d] + (j - s_bin_local[d]);
-            kout[dst[it]] = key;
+            const uint32_t d = s_digit[j];
+            kout[s_bin_global[d] + (j - s_bin_local[d])] = s_keys[j];
         }
     }
     __syncthreads();
@@ -208,13 +199,16 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
-        if (idx < n) s_vals[rank[it]] = v[it];
+        if (idx < n) s_vals[rank[it]] = IOTA ? (uint32_t)idx : vin[idx];
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const uint32_t j = (uint32_t)(it * kSortThreads + tid);
-        if (j < tile_n) vout[dst[it]] = s_vals[j];
+        if (j < tile_n) {
+            const uint32_t d = s_digit[j];
+            vout[s_bin_global[d] + (j - s_bin_local[d])] = s_vals[j];
+        }
     }
 }
 
@@ -222,21 +216,39 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
 
 uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / kSortTile + 1; }
 
-hipError_t launch_radix_pass(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
-                             uint32_t* vout, const uint32_t* n_dev, uint32_t n_max,
-                             const uint32_t* hist_pass, uint64_t* status, uint32_t* ticket,
-                             uint32_t shift, uint32_t digit_bits, uint32_t epoch, hipStream_t s) {
+template <int BITS, typename KeyT, bool IOTA>
+static void launch_pass(uint32_t grid, const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
+                        const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass,
+                        uint64_t* status, uint32_t* ticket, uint32_t shift, uint32_t epoch,
+                        hipStream_t s) {
+    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA>), dim3(grid), dim3(kSortThreads), 0, s,
+                       static_cast<const KeyT*>(kin), vin, static_cast<KeyT*>(kout), vout, n_dev,
+                       hist_pass, status, ticket, n_max, shift, epoch);
+}
+
+hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
+                             uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
+                             uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
+                             uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
+                             hipStream_t s) {
     if (n_max == 0) return hipSuccess;
     const uint32_t grid = (n_max + kSortTile - 1) / kSortTile;
-    if (digit_bits == 8) {
-        hipLaunchKernelGGL(k_radix_pass<8>, dim3(grid), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
-                           n_dev, hist_pass, status, ticket, n_max, shift, epoch);
-    } else if (digit_bits == 4) {
-        hipLaunchKernelGGL(k_radix_pass<4>, dim3(grid), dim3(kSortThreads), 0, s, kin, vin, kout, vout,
-                           n_dev, hist_pass, status, ticket, n_max, shift, epoch);
-    } else {
-        return hipErrorInvalidValue;
+    const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
+                    (iota_values ? 1 : 0);
+#define GS3D_PASS(B, K, I) \
+    launch_pass<B, K, I>(grid, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, s)
+    switch (key) {
+        case 110: GS3D_PASS(8, uint64_t, false); break;
+        case 111: GS3D_PASS(8, uint64_t, true); break;
+        case 120: GS3D_PASS(8, uint32_t, false); break;
+        case 121: GS3D_PASS(8, uint32_t, true); break;
+        case 210: GS3D_PASS(4, uint64_t, false); break;
+        case 211: GS3D_PASS(4, uint64_t, true); break;
+        case 220: GS3D_PASS(4, uint32_t, false); break;
+        case 221: GS3D_PASS(4, uint32_t, true); break;
+        default: return hipErrorInvalidValue;
     }
+#undef GS3D_PASS
     return hipGetLastError();
 }
 

@@ -10,9 +10,11 @@
 //
 // Key count K lives on the device (ref_counter); the sort, identify and
 // raster launches read it there and size their grids from the key-buffer
-// capacity. At the end of a frame K is copied to pinned host memory; when it
-// exceeded the capacity (the emit kernel counts but does not store past it),
-// the buffers grow and the frame is re-run before any result is handed out.
+// capacity. The identify kernel also stores K into host-mapped memory; when a
+// caller waits for results (sync, read) and K exceeded the capacity (the emit
+// kernel counts but does not store past it), the buffers grow and the last
+// frame is re-run before any result is handed out. A frame queues no copies
+// and no events unless it is profiled.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +33,9 @@ using namespace gs3d;
 namespace {
 
 constexpr int kCtrRefCounter = 32;
+constexpr int kCtrSplatCount = 33;  // N, for kernels that read their element count on the device
+constexpr int kCtrRefCounterId = 34;  // K of the id-order scan (reference-layout offsets)
+constexpr int kCtrIdScanTicket = 35;
 constexpr int kCtrHist = 64;  // [kMaxPasses][256] digit histograms follow the counters
 constexpr size_t kCtrTotalWords = kCtrHist + kMaxPasses * 256;
 
@@ -41,8 +46,8 @@ struct DevBuf {
 };
 
 const char* kStageLabels[GS3D_STAGE__COUNT] = {
-    "preprocess-splats", "prefix-scan", "emit-tile-refs", "radix-sort",
-    "identify-tile-ranges", "rasterize-tiles", "rebin-total"};
+    "preprocess-splats", "radix-sort-splats", "prefix-scan", "emit-tile-refs",
+    "radix-sort", "identify-tile-ranges", "rasterize-tiles", "rebin-total"};
 
 }  // namespace
 
@@ -73,8 +78,7 @@ struct gs3d_ctx {
     bool have_frame = false;
     gs3d_camera last_cam{};
     uint32_t last_w = 0, last_h = 0, last_flags = 0;
-    uint32_t* pinned = nullptr;  // [0] = K of the last frame
-    hipEvent_t k_ready = nullptr;
+    uint32_t* pinned = nullptr;  // host-mapped; [0] = K of the last frame, written by identify
     bool k_pending = false;
     uint64_t K = 0;
 
@@ -83,6 +87,9 @@ struct gs3d_ctx {
     static constexpr int kRing = 64;
     hipEvent_t ring[kRing][GS3D_STAGE__COUNT][2] = {};
     bool ring_unread[kRing] = {};
+    uint32_t ring_mask[kRing] = {};
+    uint32_t profile_mask = (1u << GS3D_STAGE__COUNT) - 1u;
+    uint64_t stage_count[GS3D_STAGE__COUNT] = {};
     uint64_t frame_counter = 0;
     int cur_slot = 0;
     bool profiled = false;
@@ -198,17 +205,20 @@ void plan_passes(gs3d_ctx* c, uint32_t tile_count, uint32_t bits) {
 }
 
 void record(gs3d_ctx* c, int stage, int which) {
-    if (c->profiled) (void)hipEventRecord(c->ring[c->cur_slot][stage][which], c->stream);
+    if (c->profiled && (c->profile_mask >> stage & 1u))
+        (void)hipEventRecord(c->ring[c->cur_slot][stage][which], c->stream);
 }
 
 // Add a completed profiled frame's stage times to the running sums.
 int fold_slot(gs3d_ctx* c, int slot) {
     if (!c->ring_unread[slot]) return GS3D_OK;
-    HIP_TRY(c, hipEventSynchronize(c->ring[slot][GS3D_STAGE_TOTAL][1]));
     for (int s = 0; s < GS3D_STAGE__COUNT; s++) {
+        if (!(c->ring_mask[slot] >> s & 1u)) continue;
+        HIP_TRY(c, hipEventSynchronize(c->ring[slot][s][1]));
         float ms = 0.f;
         HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[slot][s][0], c->ring[slot][s][1]));
         c->stage_sum_us[s] += (double)ms * 1000.0;
+        c->stage_count[s]++;
     }
     c->stage_frames++;
     c->ring_unread[slot] = false;
@@ -227,26 +237,58 @@ int set_device(gs3d_ctx* c) {
     return GS3D_OK;
 }
 
-// Sort n = *n_dev keys (grid sized for n_max) through the planned passes.
-// Sets *in_b when the sorted result ends in the b buffers.
-int run_sort(gs3d_ctx* c, uint64_t* ka, uint32_t* va, uint64_t* kb, uint32_t* vb,
-             const uint32_t* n_dev, uint64_t n_max, uint32_t* counters, bool* in_b) {
+// Run LSD passes shifts[0..passes) over n = *n_dev pairs (grid sized for
+// n_max), ping-ponging a <-> b. hist: [passes][radix] digit counts of the
+// input; ticket: one counter per pass; first_iota: pass 0 takes the
+// positions as values. Sets *in_b when the result ends in the b buffers.
+int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint32_t key_bytes,
+               bool first_iota, const uint32_t* n_dev, uint64_t n_max, const uint32_t* shifts,
+               uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b) {
     uint64_t* radix_status = nullptr;
-    GS3D_TRY(ensure_status(c, "radix_status",
-                           (size_t)radix_status_tiles((uint32_t)n_max) * (1u << c->digit_bits),
-                           &radix_status));
     const uint32_t radix = 1u << c->digit_bits;
-    for (uint32_t p = 0; p < c->n_passes; p++) {
+    GS3D_TRY(ensure_status(c, "radix_status",
+                           (size_t)radix_status_tiles((uint32_t)n_max) * radix, &radix_status));
+    for (uint32_t p = 0; p < passes; p++) {
         const bool even = (p % 2) == 0;
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va,
-                                     n_dev, (uint32_t)n_max, counters + kCtrHist + p * radix,
-                                     radix_status, counters + kCtrSortTicket + p, c->shifts[p],
+                                     key_bytes, first_iota && p == 0, n_dev, (uint32_t)n_max,
+                                     hist + p * radix, radix_status, tickets + p, shifts[p],
                                      c->digit_bits, next_epoch(c), c->stream));
     }
-    *in_b = (c->n_passes % 2) == 1;
+    *in_b = (passes % 2) == 1;
     return GS3D_OK;
 }
 
+// Reference-layout offsets: the exclusive scan of counts in splat-id order
+// (prefix_scan_*.wgsl). The frame itself scans in depth order; this one runs
+// on demand for readback and for GS3D_REBIN_KEEP_UNSORTED.
+int enqueue_id_order_scan(gs3d_ctx* c, uint32_t** out) {
+    uint32_t *counts = buf<uint32_t>(c, "splat_ref_counts"), *counters = buf<uint32_t>(c, "counters");
+    uint32_t* offs;
+    uint64_t* st;
+    GS3D_TRY(ensure_t(c, "splat_ref_offsets", (size_t)c->n + 1, &offs));
+    GS3D_TRY(ensure_status(c, "scan_status_id", (size_t)c->n / kScanTile + 2, &st));
+    HIP_TRY(c, launch_fill_u32(counters + kCtrIdScanTicket, 0u, 1, c->stream));
+    HIP_TRY(c, launch_scan(counts, nullptr, offs, counters + kCtrRefCounterId, st,
+                           counters + kCtrIdScanTicket, c->n, next_epoch(c), c->stream));
+    *out = offs;
+    return GS3D_OK;
+}
+
+// One frame (rebin, GaussianSplatRenderer.ts:1104-1266), all on the context's
+// stream, no host round trip:
+//   preprocess -> depth keys of the N splats
+//   LSD over the N depth keys (digits of bits 0..31), values = splat ids
+//   scan of the tile counts in depth order -> K, write offsets
+//   emit (tile, depth) keys in depth order (+ tile-digit histograms)
+//   LSD over the K keys, tile digits only (bits 32..)
+//   identify tile ranges, rasterize
+// Sorting the 32 depth bits on the splats instead of on the keys is
+// result-identical to the full-width LSD sort of the keys: every key of a
+// splat carries the splat's depth, and emitting splats in stable depth order,
+// each splat's tiles in row-major order, yields exactly the key stream a
+// stable sort of the id-order emit by those 32 bits would; the tile-digit
+// passes then see the same input.
 int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, uint32_t flags) {
     const uint32_t n = c->n;
     c->width = W;
@@ -259,29 +301,44 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     GS3D_TRY(begin_profiled_frame(c));
     if (c->key_capacity == 0) c->key_capacity = std::max<uint64_t>(8ull * n, 1u << 16);
     const uint64_t cap = c->key_capacity;
+    const uint32_t bits = c->digit_bits;
+    const uint32_t radix = 1u << bits;
 
     gs3d_projected_splat* projected;
-    uint32_t *counts, *offsets, *counters, *tile_offsets, *va, *vb;
+    uint32_t *counts, *offsets, *counters, *tile_offsets, *va, *vb, *dka, *dkb, *ida, *idb;
     uint64_t *scan_status, *ka, *kb;
     uint16_t* image;
     GS3D_TRY(ensure_t(c, "projected_splats", n, &projected));
     GS3D_TRY(ensure_t(c, "splat_ref_counts", n, &counts));
-    GS3D_TRY(ensure_t(c, "splat_ref_offsets", (size_t)n + 1, &offsets));
+    GS3D_TRY(ensure_t(c, "splat_ref_offsets_depth_order", (size_t)n + 1, &offsets));
     GS3D_TRY(ensure_t(c, "counters", kCtrTotalWords, &counters));
     GS3D_TRY(ensure_status(c, "scan_status", (size_t)n / kScanTile + 2, &scan_status));
+    GS3D_TRY(ensure_t(c, "depth_keys_a", n, &dka));
+    GS3D_TRY(ensure_t(c, "depth_keys_b", n, &dkb));
+    GS3D_TRY(ensure_t(c, "splat_order_a", n, &ida));
+    GS3D_TRY(ensure_t(c, "splat_order_b", n, &idb));
     GS3D_TRY(ensure_t(c, "sort_keys_a", cap, &ka));
     GS3D_TRY(ensure_t(c, "sort_keys_b", cap, &kb));
     GS3D_TRY(ensure_t(c, "sort_values_a", cap, &va));
     GS3D_TRY(ensure_t(c, "sort_values_b", cap, &vb));
     GS3D_TRY(ensure_t(c, "tile_offsets", (size_t)T + 1, &tile_offsets));
     GS3D_TRY(ensure_t(c, "splat_target", (size_t)W * H * 4, &image));
-    plan_passes(c, T, c->digit_bits);
+    plan_passes(c, T, bits);
+    const uint32_t depth_passes = 32u / bits;
+    const uint32_t tile_passes = c->n_passes - depth_passes;
     uint32_t* ref_counter = counters + kCtrRefCounter;
+    uint32_t* hist_depth = counters + kCtrHist;
+    uint32_t* hist_tile = counters + kCtrHist + depth_passes * radix;
+    uint32_t* hist_partial;
+    GS3D_TRY(ensure_t(c, "radix_hist_partial",
+                      (size_t)std::max(emit_grid(n), key_hist_rows(n)) * kMaxPasses * radix, &hist_partial));
 
     record(c, GS3D_STAGE_TOTAL, 0);
     record(c, GS3D_STAGE_PREPROCESS, 0);
-    HIP_TRY(c, launch_fill_u32(counters, 0u, kCtrTotalWords, c->stream));
     PreprocessArgs pa{};
+    pa.counters = counters;  // zeroed by the preprocess kernel itself, N stored
+    pa.counter_words = (uint32_t)kCtrTotalWords;
+    pa.count_slot = kCtrSplatCount;
     pa.cam = *cam;
     pa.pos = buf<float4>(c, "splat_positions");
     pa.scale = buf<float4>(c, "splat_scales");
@@ -290,6 +347,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     pa.sh = c->sh_degree > 0 ? buf<float>(c, "splat_sh") : nullptr;
     pa.out = projected;
     pa.counts = counts;
+    pa.depth_keys = dka;
     pa.n = n;
     pa.sh_degree = c->sh_degree;
     pa.tiles_x = c->tiles_x;
@@ -299,46 +357,51 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     HIP_TRY(c, launch_preprocess(pa, c->stream));
     record(c, GS3D_STAGE_PREPROCESS, 1);
 
+    // depth order of the splats (stable: ties keep splat-id order)
+    record(c, GS3D_STAGE_DEPTH_SORT, 0);
+    HIP_TRY(c, launch_key_histogram(dka, 4, counters + kCtrSplatCount, n, hist_depth, hist_partial,
+                                    bits, depth_passes, 0, c->stream));
+    bool order_in_b = false;
+    // pass 0 takes the positions as values (ida is not read); odd passes write ida
+    GS3D_TRY(run_passes(c, dka, ida, dkb, idb, 4, true, counters + kCtrSplatCount, n, c->shifts,
+                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b));
+    const uint32_t* order = order_in_b ? idb : ida;
+    record(c, GS3D_STAGE_DEPTH_SORT, 1);
+
     record(c, GS3D_STAGE_SCAN, 0);
-    HIP_TRY(c, launch_scan(counts, offsets, ref_counter, scan_status, counters + kCtrScanTicket, n,
-                           next_epoch(c), c->stream));
+    HIP_TRY(c, launch_scan(counts, order, offsets, ref_counter, scan_status, counters + kCtrScanTicket,
+                           n, next_epoch(c), c->stream));
     record(c, GS3D_STAGE_SCAN, 1);
 
     record(c, GS3D_STAGE_EMIT, 0);
     EmitArgs ea{};
     ea.projected = projected;
+    ea.order = order;
     ea.offsets = offsets;
     ea.keys = ka;
     ea.values = va;
-    ea.hist = counters + kCtrHist;
-    uint32_t* hist_partial;
-    GS3D_TRY(ensure_t(c, "radix_hist_partial",
-                      (size_t)emit_grid(n) * c->n_passes * (1u << c->digit_bits), &hist_partial));
-    ea.hist_partial = hist_partial;
+    ea.hist = tile_passes ? hist_tile : nullptr;
+    ea.hist_partial = tile_passes ? hist_partial : nullptr;
     ea.n = n;
     ea.tiles_x = c->tiles_x;
     ea.capacity = cap;
-    ea.n_passes = c->n_passes;
-    ea.digit_bits = c->digit_bits;
+    ea.n_passes = tile_passes;
+    ea.digit_bits = bits;
+    ea.hist_shift0 = 32;
     HIP_TRY(c, launch_emit(ea, c->stream));
     record(c, GS3D_STAGE_EMIT, 1);
-    if (c->keep_unsorted) {
-        uint64_t* uk;
-        uint32_t* uv;
-        GS3D_TRY(ensure_t(c, "unsorted_keys", cap, &uk));
-        GS3D_TRY(ensure_t(c, "unsorted_values", cap, &uv));
-        HIP_TRY(c, hipMemcpyAsync(uk, ka, cap * 8, hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(c, hipMemcpyAsync(uv, va, cap * 4, hipMemcpyDeviceToDevice, c->stream));
-    }
 
     record(c, GS3D_STAGE_SORT, 0);
-    GS3D_TRY(run_sort(c, ka, va, kb, vb, ref_counter, cap, counters, &c->result_in_b));
+    GS3D_TRY(run_passes(c, ka, va, kb, vb, 8, false, ref_counter, cap, c->shifts + depth_passes,
+                        tile_passes, hist_tile, counters + kCtrSortTicket + depth_passes,
+                        &c->result_in_b));
     record(c, GS3D_STAGE_SORT, 1);
     const uint64_t* sorted_keys = c->result_in_b ? kb : ka;
     const uint32_t* sorted_vals = c->result_in_b ? vb : va;
 
     record(c, GS3D_STAGE_IDENTIFY, 0);
-    HIP_TRY(c, launch_identify_tiles(sorted_keys, ref_counter, (uint32_t)cap, tile_offsets, T, c->stream));
+    HIP_TRY(c, launch_identify_tiles(sorted_keys, ref_counter, (uint32_t)cap, tile_offsets, T,
+                                     c->pinned, c->stream));
     record(c, GS3D_STAGE_IDENTIFY, 1);
 
     record(c, GS3D_STAGE_RASTER, 0);
@@ -347,32 +410,45 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     record(c, GS3D_STAGE_RASTER, 1);
     record(c, GS3D_STAGE_TOTAL, 1);
 
-    HIP_TRY(c, hipMemcpyAsync(c->pinned, ref_counter, 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipEventRecord(c->k_ready, c->stream));
+    if (c->keep_unsorted) {
+        // the reference's emit order (splat ids ascending) for parity checks
+        uint32_t* id_offsets;
+        GS3D_TRY(enqueue_id_order_scan(c, &id_offsets));
+        uint64_t* uk;
+        uint32_t* uv;
+        GS3D_TRY(ensure_t(c, "unsorted_keys", cap, &uk));
+        GS3D_TRY(ensure_t(c, "unsorted_values", cap, &uv));
+        EmitArgs eu = ea;
+        eu.order = nullptr;
+        eu.offsets = id_offsets;
+        eu.keys = uk;
+        eu.values = uv;
+        eu.hist = nullptr;
+        eu.hist_partial = nullptr;
+        eu.n_passes = 0;
+        HIP_TRY(c, launch_emit(eu, c->stream));
+    }
+
     c->k_pending = true;
-    if (c->profiled) c->ring_unread[c->cur_slot] = true;
+    if (c->profiled) {
+        c->ring_unread[c->cur_slot] = true;
+        c->ring_mask[c->cur_slot] = c->profile_mask;
+    }
     c->last_profiled = c->profiled;
     return GS3D_OK;
 }
 
-// Consume the last frame's K. With wait, block for it and, after a capacity
-// miss, grow the key buffers and re-run that frame so its results are whole.
-int settle(gs3d_ctx* c, bool wait) {
+// Consume the last frame's K (blocking: the stream is drained first). After a
+// capacity miss, grow the key buffers and re-run that frame so its results
+// are whole.
+int settle(gs3d_ctx* c) {
     for (int attempt = 0; attempt < 3; attempt++) {
         if (!c->k_pending) return GS3D_OK;
-        if (wait) {
-            HIP_TRY(c, hipEventSynchronize(c->k_ready));
-        } else {
-            hipError_t q = hipEventQuery(c->k_ready);
-            if (q == hipErrorNotReady) return GS3D_OK;
-            if (q != hipSuccess) return fail(c, GS3D_E_HIP, "hipEventQuery: %s", hipGetErrorString(q));
-        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
         c->k_pending = false;
         c->K = c->pinned[0];
         if (c->K <= c->key_capacity) return GS3D_OK;
-        // capacity miss: grow (the next ensure() frees and reallocates)
-        c->key_capacity = c->K + c->K / 4 + kSortTile;
-        if (!wait) return GS3D_OK;  // the frame about to be queued uses the new capacity
+        c->key_capacity = c->K + c->K / 4 + kSortTile;  // the next ensure() reallocates
         GS3D_TRY(enqueue_rebin(c, &c->last_cam, c->last_w, c->last_h, c->last_flags));
     }
     return fail(c, GS3D_E_STATE, "key buffer did not settle after re-runs");
@@ -409,8 +485,7 @@ int gs3d_create(int device, gs3d_ctx** out) {
     c->limits[5] = (uint32_t)prop.sharedMemPerBlock;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->pinned, 64, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreateWithFlags(&c->k_ready, hipEventDisableTiming) != hipSuccess) {
+        hipHostMalloc((void**)&c->pinned, 64, hipHostMallocMapped) != hipSuccess) {
         gs3d_destroy(c);
         return GS3D_E_HIP;
     }
@@ -435,7 +510,6 @@ int gs3d_destroy(gs3d_ctx* c) {
         for (int s = 0; s < GS3D_STAGE__COUNT; s++)
             for (int w = 0; w < 2; w++)
                 if (c->ring[r][s][w]) (void)hipEventDestroy(c->ring[r][s][w]);
-    if (c->k_ready) (void)hipEventDestroy(c->k_ready);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -445,7 +519,7 @@ int gs3d_destroy(gs3d_ctx* c) {
 int gs3d_sync(gs3d_ctx* c) {
     if (!c) return GS3D_E_INVALID_ARG;
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return GS3D_OK;
 }
@@ -467,7 +541,7 @@ int gs3d_set_splats(gs3d_ctx* c, const float* pos4, const float* scale4, const f
         return fail(c, GS3D_E_INVALID_ARG, "sh_degree %d needs f_dc and sh_rest", sh_degree);
     if ((uint64_t)n * 16 > (1ull << 40)) return fail(c, GS3D_E_INVALID_ARG, "too many splats");
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     const char* names[4] = {"splat_positions", "splat_scales", "splat_rotations", "splat_colors"};
     const float* src[4] = {pos4, scale4, rot4, color4};
     for (int k = 0; k < 4; k++) {
@@ -504,7 +578,7 @@ int gs3d_set_tile_size(gs3d_ctx* c, uint32_t x, uint32_t y, uint32_t out_xy[2]) 
     if (!((tx == 8 && ty == 8) || (tx == 16 && ty == 16) || (tx == 32 && ty == 32)))
         return fail(c, GS3D_E_UNKNOWN_NAME, "no rasterize-tiles variant for a %ux%u tile (8, 16, 32 square)",
                     tx, ty);
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     c->tile_x = tx;
     c->tile_y = ty;
     if (!raster_variant_exists(tx, c->raster_ppl)) c->raster_ppl = 1;  // 8x8 has one wavefront
@@ -517,7 +591,7 @@ int gs3d_set_tile_size(gs3d_ctx* c, uint32_t x, uint32_t y, uint32_t out_xy[2]) 
 
 int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, int value) {
     if (!c || !kernel || !param) return GS3D_E_INVALID_ARG;
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "PIXELS_PER_LANE")) {
         if (value < 0 || !raster_variant_exists(c->tile_x, (uint32_t)value))
             return fail(c, GS3D_E_UNKNOWN_NAME, "no rasterize-tiles variant with %d pixels per lane for %u px tiles",
@@ -572,7 +646,6 @@ int gs3d_rebin(gs3d_ctx* c, const gs3d_camera* camera, uint32_t width, uint32_t 
         return fail(c, GS3D_E_INVALID_ARG, "viewport %ux%u out of range", width, height);
     if (!c->bufs.count("splat_positions")) return fail(c, GS3D_E_STATE, "rebin before set_splats");
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, false));
     c->last_cam = *camera;
     c->last_w = width;
     c->last_h = height;
@@ -585,7 +658,7 @@ int gs3d_ref_count(gs3d_ctx* c, uint64_t* out) {
     if (!c || !out) return GS3D_E_INVALID_ARG;
     if (!c->have_frame) return fail(c, GS3D_E_STATE, "no rebin yet");
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     *out = c->K;
     return GS3D_OK;
 }
@@ -602,7 +675,7 @@ int gs3d_read(gs3d_ctx* c, int id, void* dst, size_t bytes, size_t* written) {
     if (!c) return GS3D_E_INVALID_ARG;
     if (!c->have_frame) return fail(c, GS3D_E_STATE, "no rebin yet");
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     const uint64_t K = c->K;
     const size_t px = (size_t)c->width * c->height;
     const void* src = nullptr;
@@ -611,7 +684,13 @@ int gs3d_read(gs3d_ctx* c, int id, void* dst, size_t bytes, size_t* written) {
     switch (id) {
         case GS3D_BUF_PROJECTED_SPLATS: src = buf<void>(c, "projected_splats"); size = (size_t)c->n * 64; break;
         case GS3D_BUF_SPLAT_REF_COUNTS: src = buf<void>(c, "splat_ref_counts"); size = (size_t)c->n * 4; break;
-        case GS3D_BUF_SPLAT_REF_OFFSETS: src = buf<void>(c, "splat_ref_offsets"); size = ((size_t)c->n + 1) * 4; break;
+        case GS3D_BUF_SPLAT_REF_OFFSETS: {
+            uint32_t* offs;
+            GS3D_TRY(enqueue_id_order_scan(c, &offs));
+            src = offs;
+            size = ((size_t)c->n + 1) * 4;
+            break;
+        }
         case GS3D_BUF_REF_COUNTER: src = buf<uint32_t>(c, "counters") + kCtrRefCounter; size = 4; break;
         case GS3D_BUF_SORT_KEYS: src = buf<void>(c, c->result_in_b ? "sort_keys_b" : "sort_keys_a"); size = K * 8; break;
         case GS3D_BUF_SORT_VALUES: src = buf<void>(c, c->result_in_b ? "sort_values_b" : "sort_values_a"); size = K * 4; break;
@@ -658,11 +737,12 @@ int gs3d_stage_times_us(gs3d_ctx* c, float* out, int n) {
     if (!c || !out) return GS3D_E_INVALID_ARG;
     if (!c->have_frame || !c->last_profiled) return fail(c, GS3D_E_STATE, "no profiled rebin yet");
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++) {
         float ms = 0.f;
-        HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[c->cur_slot][s][0], c->ring[c->cur_slot][s][1]));
+        if (c->profile_mask >> s & 1u)
+            HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[c->cur_slot][s][0], c->ring[c->cur_slot][s][1]));
         out[s] = ms * 1000.0f;
     }
     return GS3D_OK;
@@ -673,19 +753,28 @@ int gs3d_profiler_reset(gs3d_ctx* c) {
     GS3D_TRY(set_device(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int r = 0; r < gs3d_ctx::kRing; r++) c->ring_unread[r] = false;
-    for (int s = 0; s < GS3D_STAGE__COUNT; s++) c->stage_sum_us[s] = 0.0;
+    for (int s = 0; s < GS3D_STAGE__COUNT; s++) {
+        c->stage_sum_us[s] = 0.0;
+        c->stage_count[s] = 0;
+    }
     c->stage_frames = 0;
+    return GS3D_OK;
+}
+
+int gs3d_set_profile_mask(gs3d_ctx* c, uint32_t stage_mask) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    c->profile_mask = stage_mask & ((1u << GS3D_STAGE__COUNT) - 1u);
     return GS3D_OK;
 }
 
 int gs3d_stage_times_avg_us(gs3d_ctx* c, float* out, int n, uint64_t* frames) {
     if (!c || !out) return GS3D_E_INVALID_ARG;
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     for (int r = 0; r < gs3d_ctx::kRing; r++) GS3D_TRY(fold_slot(c, r));
     if (frames) *frames = c->stage_frames;
     for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++)
-        out[s] = c->stage_frames ? (float)(c->stage_sum_us[s] / (double)c->stage_frames) : 0.0f;
+        out[s] = c->stage_count[s] ? (float)(c->stage_sum_us[s] / (double)c->stage_count[s]) : 0.0f;
     return GS3D_OK;
 }
 
@@ -724,7 +813,7 @@ int gs3d_buffer_write(gs3d_ctx* c, const char* name, size_t offset, const void* 
     if (written) *written = n;
     if (n == 0) return fail(c, GS3D_E_TOO_SMALL, "write to '%s' skipped: size exceeds buffer limit", name);
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     HIP_TRY(c, hipMemcpyAsync(static_cast<char*>(it->second.ptr) + offset, src, n,
                               hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -739,7 +828,7 @@ int gs3d_buffer_read(gs3d_ctx* c, const char* name, size_t offset, void* dst, si
         return fail(c, GS3D_E_TOO_SMALL, "read of %zu bytes at %zu past '%s' (%zu bytes)", bytes, offset,
                     name, it->second.size);
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     if (bytes)
         HIP_TRY(c, hipMemcpyAsync(dst, static_cast<const char*>(it->second.ptr) + offset, bytes,
                                   hipMemcpyDeviceToHost, c->stream));
@@ -784,7 +873,7 @@ int gs3d_stage_preprocess(gs3d_ctx* c, const gs3d_camera* cam, uint32_t tiles_x,
     if (tiles_x == 0 || tiles_y == 0) return fail(c, GS3D_E_INVALID_ARG, "zero tile count");
     if (!c->bufs.count("splat_positions")) return fail(c, GS3D_E_STATE, "preprocess before set_splats");
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     const uint32_t n = c->n;
     gs3d_projected_splat* dp;
     uint32_t* dc;
@@ -818,7 +907,7 @@ int gs3d_stage_scan(gs3d_ctx* c, const uint32_t* counts, uint32_t n, uint32_t* o
                     uint32_t* total) {
     if (!c || !offsets || (n && !counts)) return GS3D_E_INVALID_ARG;
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     uint32_t *din, *dout, *ctr;
     uint64_t* st;
     GS3D_TRY(ensure_t(c, "stage_counts", n, &din));
@@ -827,8 +916,8 @@ int gs3d_stage_scan(gs3d_ctx* c, const uint32_t* counts, uint32_t n, uint32_t* o
     GS3D_TRY(ensure_status(c, "stage_scan_status", (size_t)n / kScanTile + 2, &st));
     if (n) HIP_TRY(c, hipMemcpyAsync(din, counts, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, launch_fill_u32(ctr, 0u, kCtrTotalWords, c->stream));
-    HIP_TRY(c, launch_scan(din, dout, ctr + kCtrRefCounter, st, ctr + kCtrScanTicket, n, next_epoch(c),
-                           c->stream));
+    HIP_TRY(c, launch_scan(din, nullptr, dout, ctr + kCtrRefCounter, st, ctr + kCtrScanTicket, n,
+                           next_epoch(c), c->stream));
     HIP_TRY(c, hipMemcpyAsync(offsets, dout, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, c->stream));
     uint32_t tot = 0;
     HIP_TRY(c, hipMemcpyAsync(&tot, ctr + kCtrRefCounter, 4, hipMemcpyDeviceToHost, c->stream));
@@ -844,7 +933,7 @@ int gs3d_stage_emit(gs3d_ctx* c, const gs3d_projected_splat* projected, const ui
     const uint32_t K = offsets[n];
     if (K && (!keys || !values)) return GS3D_E_INVALID_ARG;
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     gs3d_projected_splat* dp;
     uint32_t *doff, *dv;
     uint64_t* dk;
@@ -861,6 +950,7 @@ int gs3d_stage_emit(gs3d_ctx* c, const gs3d_projected_splat* projected, const ui
     ea.values = dv;
     ea.hist = nullptr;
     ea.hist_partial = nullptr;
+    ea.order = nullptr;
     ea.n = n;
     ea.tiles_x = tiles_x;
     ea.capacity = K;
@@ -881,36 +971,33 @@ int gs3d_stage_sort(gs3d_ctx* c, const uint64_t* keys_in, const uint32_t* values
     if (digit_bits != 4 && digit_bits != 8)
         return fail(c, GS3D_E_INVALID_ARG, "digit width %u not 4 or 8", digit_bits);
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     if (n == 0) return GS3D_OK;
     uint64_t *ka, *kb;
-    uint32_t *va, *vb, *ctr, *shifts_dev;
+    uint32_t *va, *vb, *ctr, *partial;
     GS3D_TRY(ensure_t(c, "stage_keys_a", n, &ka));
     GS3D_TRY(ensure_t(c, "stage_keys_b", n, &kb));
     GS3D_TRY(ensure_t(c, "stage_values_a", n, &va));
     GS3D_TRY(ensure_t(c, "stage_values_b", n, &vb));
     GS3D_TRY(ensure_t(c, "stage_counters", kCtrTotalWords, &ctr));
-    GS3D_TRY(ensure_t(c, "stage_shifts", kMaxPasses, &shifts_dev));
-    const uint32_t saved_bits = c->digit_bits, saved_passes = c->n_passes;
-    uint32_t saved_shifts[kMaxPasses];
-    std::memcpy(saved_shifts, c->shifts, sizeof saved_shifts);
-    c->digit_bits = digit_bits;
-    c->n_passes = 0;
-    for (uint32_t s = 0; s < 64; s += digit_bits) c->shifts[c->n_passes++] = s;
+    GS3D_TRY(ensure_t(c, "stage_hist_partial", (size_t)key_hist_rows(n) * kMaxPasses * 256, &partial));
+    const uint32_t passes = 64 / digit_bits;
+    uint32_t shifts[kMaxPasses];
+    for (uint32_t p = 0; p < passes; p++) shifts[p] = p * digit_bits;
     uint32_t host_ctr[kCtrWords] = {};
     host_ctr[kCtrRefCounter] = n;
     HIP_TRY(c, hipMemcpyAsync(ka, keys_in, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(va, values_in, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, launch_fill_u32(ctr, 0u, kCtrTotalWords, c->stream));
     HIP_TRY(c, hipMemcpyAsync(ctr, host_ctr, sizeof host_ctr, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(shifts_dev, c->shifts, sizeof c->shifts, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, launch_digit_histogram(ka, ctr + kCtrRefCounter, n, ctr + kCtrHist, digit_bits,
-                                      c->n_passes, shifts_dev, c->stream));
+    HIP_TRY(c, launch_key_histogram(ka, 8, ctr + kCtrRefCounter, n, ctr + kCtrHist, partial, digit_bits,
+                                    passes, 0, c->stream));
+    const uint32_t saved_bits = c->digit_bits;
+    c->digit_bits = digit_bits;
     bool in_b = false;
-    int rc = run_sort(c, ka, va, kb, vb, ctr + kCtrRefCounter, n, ctr, &in_b);
+    int rc = run_passes(c, ka, va, kb, vb, 8, false, ctr + kCtrRefCounter, n, shifts, passes,
+                        ctr + kCtrHist, ctr + kCtrSortTicket, &in_b);
     c->digit_bits = saved_bits;
-    c->n_passes = saved_passes;
-    std::memcpy(c->shifts, saved_shifts, sizeof saved_shifts);
     if (rc != GS3D_OK) return rc;
     HIP_TRY(c, hipMemcpyAsync(keys_out, in_b ? kb : ka, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipMemcpyAsync(values_out, in_b ? vb : va, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -922,7 +1009,7 @@ int gs3d_stage_identify(gs3d_ctx* c, const uint64_t* sorted_keys, uint32_t n, ui
                         uint32_t* tile_offsets) {
     if (!c || !tile_offsets || (n && !sorted_keys)) return GS3D_E_INVALID_ARG;
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     uint64_t* dk;
     uint32_t *dt, *ctr;
     GS3D_TRY(ensure_t(c, "stage_keys_a", n, &dk));
@@ -932,7 +1019,7 @@ int gs3d_stage_identify(gs3d_ctx* c, const uint64_t* sorted_keys, uint32_t n, ui
     host_ctr[kCtrRefCounter] = n;
     if (n) HIP_TRY(c, hipMemcpyAsync(dk, sorted_keys, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(ctr, host_ctr, sizeof host_ctr, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, launch_identify_tiles(dk, ctr + kCtrRefCounter, n, dt, tile_count, c->stream));
+    HIP_TRY(c, launch_identify_tiles(dk, ctr + kCtrRefCounter, n, dt, tile_count, nullptr, c->stream));
     HIP_TRY(c, hipMemcpyAsync(tile_offsets, dt, ((size_t)tile_count + 1) * 4, hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -958,7 +1045,7 @@ int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uin
     for (uint32_t i = 0; i < n_refs; i++)
         if (sorted_values[i] >= n_splats) return fail(c, GS3D_E_INVALID_ARG, "splat id %u out of range", sorted_values[i]);
     GS3D_TRY(set_device(c));
-    GS3D_TRY(settle(c, true));
+    GS3D_TRY(settle(c));
     gs3d_projected_splat* dp;
     uint32_t *dv, *dt;
     uint16_t* dimg;

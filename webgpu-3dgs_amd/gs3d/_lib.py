@@ -31,7 +31,7 @@ BUF_UNSORTED_VALUES = 10
 REBIN_PROFILE = 1
 REBIN_KEEP_UNSORTED = 2
 
-STAGES = ("preprocess-splats", "prefix-scan", "emit-tile-refs", "radix-sort",
+STAGES = ("preprocess-splats", "radix-sort-splats", "prefix-scan", "emit-tile-refs", "radix-sort",
           "identify-tile-ranges", "rasterize-tiles", "rebin-total")
 
 P = C.c_void_p
@@ -61,6 +61,7 @@ _PROTOS = {
     "gs3d_stage_times_us": (I32, [P, P, I32]),
     "gs3d_stage_label": (C.c_char_p, [I32]),
     "gs3d_profiler_reset": (I32, [P]),
+    "gs3d_set_profile_mask": (I32, [P, U32]),
     "gs3d_stage_times_avg_us": (I32, [P, P, I32, C.POINTER(C.c_uint64)]),
     "gs3d_sort_passes": (I32, [P, C.POINTER(U32), P]),
     "gs3d_buffer_create": (I32, [P, C.c_char_p, SZ]),

@@ -139,6 +139,13 @@ class GaussianSplatRenderer:
     def profiler_reset(self) -> None:
         self._check(self._lib.gs3d_profiler_reset(self._h))
 
+    def set_profile_mask(self, stages) -> None:
+        """Profile only the named stages (labels of _lib.STAGES)."""
+        mask = 0
+        for name in stages:
+            mask |= 1 << L.STAGES.index(name)
+        self._check(self._lib.gs3d_set_profile_mask(self._h, mask))
+
     def stage_times_avg_us(self) -> tuple[dict, int]:
         out = np.zeros(len(L.STAGES), np.float32)
         frames = C.c_uint64()
