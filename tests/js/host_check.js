@@ -1,0 +1,127 @@
+'use strict';
+// Test driver for the Node host (webgpu-3dgs_amd/host): reads one JSON request
+// from argv[2] (a file path), runs it and prints one JSON reply on stdout.
+// Float arrays travel as base64 of their bytes. Used by tests/test_host_js.py.
+
+const fs = require('fs');
+const path = require('path');
+
+const host = require(path.join(__dirname, '..', '..', 'webgpu-3dgs_amd', 'host'));
+
+const b64 = (ta) => (ta ? Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength).toString('base64') : null);
+const f32 = (s) => new Float32Array(new Uint8Array(Buffer.from(s, 'base64')).buffer);
+
+async function ply(req) {
+  const loader = new host.SceneLoader({ useWorker: req.worker !== false });
+  try {
+    const { splats, bounds } = await loader.loadFile(req.path);
+    return {
+      ok: true,
+      n: splats.splatCount,
+      degree: splats.sphericalHarmonicsDegree,
+      positions: b64(splats.positions),
+      colors: b64(splats.colors),
+      scales: b64(splats.scales),
+      rotations: b64(splats.rotations),
+      fDc: b64(splats.fDc),
+      sh: b64(splats.sphericalHarmonics),
+      bounds: [bounds.min.x, bounds.min.y, bounds.min.z, bounds.max.x, bounds.max.y, bounds.max.z],
+    };
+  } catch (e) {
+    return { ok: false, error: String(e.message || e) };
+  }
+}
+
+function camera(req) {
+  const out = [];
+  for (const c of req.cameras) {
+    const cam = new host.Camera(c.position, c.target, c.up, c.fov, c.aspect, c.near, c.far);
+    const v0 = cam.version;
+    for (const op of c.ops || []) cam[op[0]](...op.slice(1));
+    out.push({ uniform: b64(cam.getUniformData(c.width, c.height)), versionDelta: cam.version - v0,
+      position: Array.from(cam.position), target: Array.from(cam.target) });
+  }
+  return { ok: true, cameras: out };
+}
+
+function workgroups(req) {
+  const wm = new host.WorkgroupManager(req.limits);
+  const out = [];
+  for (const c of req.cases) {
+    const fn = host[c.strategy];
+    try {
+      wm.register({ name: c.name, problemSize: c.problemSize, strategyFn: fn, strategyArgs: c.args });
+      out.push(wm.getLayout(c.name));
+    } catch (e) {
+      out.push({ error: String(e.message) });
+    }
+  }
+  return { ok: true, layouts: out };
+}
+
+function pipelines(req) {
+  const pm = new host.PipelineManager();
+  const out = [];
+  for (const c of req.cases) {
+    try {
+      if (c.update) pm.update(c.name, { codeConstants: c.codeConstants });
+      else pm.create({ name: c.name, codeConstants: c.codeConstants });
+      out.push({ ok: true, constants: pm.get(c.name).constants });
+    } catch (e) {
+      out.push({ ok: false, error: String(e.message) });
+    }
+  }
+  return { ok: true, results: out };
+}
+
+// GPU: load a PLY, render frames through GaussianSplatRenderer and return the
+// image, ref count, sorted keys' length and the rebin decisions.
+async function render(req) {
+  const scene = new host.Scene();
+  const { splats, bounds } = await new host.SceneLoader({ useWorker: true }).loadFile(req.path);
+  scene.splats = splats;
+  scene.bounds = bounds;
+  scene.tiles = [req.tile, req.tile];
+  const r = new host.GaussianSplatRenderer(scene);
+  try {
+    r.init();
+    if (req.pixelsPerLane) {
+      r.pipelineManager.update('rasterize-tiles', { codeConstants: { PIXELS_PER_LANE: req.pixelsPerLane } });
+    }
+    const cam = new host.Camera(req.eye, [0, 0, 0], [0, 1, 0], Math.PI / 3, req.width / req.height, 0.1, 10000);
+    const frame = { width: req.width, height: req.height, camera: cam };
+    const decisions = [r.render(frame), r.render(frame)];
+    cam.rotate(0.0, 0.0);
+    decisions.push(r.render(frame));
+    const image = r.readImage(false);
+    const counts = await r.readTileCounts();
+    const variant = r.addon.getKernelVariant(r.ctx, 'rasterize-tiles', 'PIXELS_PER_LANE');
+    return {
+      ok: true, decisions, refCount: r.refCount(), rebinCount: r.rebinCount,
+      image: b64(image), tileCounts: b64(counts), tileCount: r.getTileCount(), tileSize: r.getTileSize(),
+      uniform: b64(cam.getUniformData(req.width, req.height)), variant,
+    };
+  } finally {
+    r.destroy();
+  }
+}
+
+function addonExports() {
+  const addon = host.loadAddon();
+  return { ok: true, names: Object.keys(addon).sort(), abiVersion: addon.abiVersion };
+}
+
+const HANDLERS = { ply, camera, workgroups, pipelines, render, addonExports };
+
+(async () => {
+  const req = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));
+  const fn = HANDLERS[req.cmd];
+  if (!fn) throw new Error(`unknown cmd ${req.cmd}`);
+  const reply = await fn(req);
+  process.stdout.write(JSON.stringify(reply));
+})().catch((e) => {
+  process.stderr.write(String(e && e.stack ? e.stack : e));
+  process.exit(1);
+});
+
+module.exports = { f32 };

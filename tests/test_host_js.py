@@ -1,0 +1,404 @@
+"""Node host (webgpu-3dgs_amd/host) checks, driven through tests/js/host_check.js.
+
+CPU tests: the PLY loader (ascii, binary little/big endian, extra elements,
+f_rest ordering, SH degree, header errors) against the Python restatement of
+the loader's activations (SceneWorker.ts:407-464); the Camera uniform against
+the textbook look-at / perspective closed forms (Camera.ts:69-191); the
+WorkgroupManager strategies (WorkgroupManager.ts:10-147) against their closed
+forms; PipelineManager variant validation. The GPU test renders through
+GaussianSplatRenderer.render and compares with the oracle pipeline.
+"""
+import base64
+import json
+import math
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gs3d import camera as C
+from gs3d import scene as S
+
+ROOT = Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "tests" / "js" / "host_check.js"
+ADDON = ROOT / "webgpu-3dgs_amd" / "addon" / "gs3d_napi.node"
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None, reason="node is not installed")
+
+
+def run_node(req: dict, tmp_path: Path, timeout: float = 120.0) -> dict:
+    f = tmp_path / "req.json"
+    f.write_text(json.dumps(req))
+    p = subprocess.run([NODE, str(DRIVER), str(f)], capture_output=True, text=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr[-3000:]
+    return json.loads(p.stdout)
+
+
+def f32(s):
+    return None if s is None else np.frombuffer(base64.b64decode(s), np.float32)
+
+
+# ---------------------------------------------------------------- PLY writer
+
+PROPS = ["x", "y", "z", "f_dc_0", "f_dc_1", "f_dc_2", "opacity", "scale_0", "scale_1", "scale_2",
+         "rot_0", "rot_1", "rot_2", "rot_3"]
+
+
+def raw_columns(raw: S.RawSplats) -> dict:
+    cols = {
+        "x": raw.xyz[:, 0], "y": raw.xyz[:, 1], "z": raw.xyz[:, 2],
+        "f_dc_0": raw.f_dc[:, 0], "f_dc_1": raw.f_dc[:, 1], "f_dc_2": raw.f_dc[:, 2],
+        "opacity": raw.opacity_logit,
+        "scale_0": raw.log_scale[:, 0], "scale_1": raw.log_scale[:, 1], "scale_2": raw.log_scale[:, 2],
+        "rot_0": raw.rot[:, 0], "rot_1": raw.rot[:, 1], "rot_2": raw.rot[:, 2], "rot_3": raw.rot[:, 3],
+    }
+    if raw.f_rest is not None:
+        for k in range(raw.f_rest.shape[1]):
+            cols[f"f_rest_{k}"] = raw.f_rest[:, k]
+    return {k: np.asarray(v, np.float32) for k, v in cols.items()}
+
+
+def write_ply(path: Path, raw: S.RawSplats, fmt: str, order=None, extra_before=0, extra_after=0,
+              drop=None):
+    """A Gaussian-splat PLY with float properties in `order` (default: PROPS
+    then f_rest_k ascending). Extra elements ('camera' with 2 floats) before or
+    after the vertex element exercise the element walk."""
+    cols = raw_columns(raw)
+    names = order or (PROPS + [k for k in cols if k.startswith("f_rest_")])
+    if drop:
+        names = [n for n in names if n != drop]
+    lines = ["ply", f"format {fmt} 1.0", "comment written by the host tests"]
+    if extra_before:
+        lines += [f"element camera {extra_before}", "property float cx", "property float cy"]
+    lines += [f"element vertex {raw.n}"] + [f"property float {n}" for n in names]
+    if extra_after:
+        lines += [f"element extra {extra_after}", "property uchar flag", "property double weight"]
+    lines.append("end_header")
+    head = ("\n".join(lines) + "\n").encode()
+    vert = np.stack([cols[n] for n in names], axis=1) if names else np.zeros((raw.n, 0), np.float32)
+    before = np.arange(extra_before * 2, dtype=np.float32).reshape(extra_before, 2) + 0.25
+    if fmt == "ascii":
+        body = []
+        body += [" ".join(repr(float(v)) for v in row) for row in before]
+        # repr of the f64 widening: the loader parses ascii tokens as f64, so the
+        # shortest f32 string would denote a different number than the f32 value
+        body += [" ".join(repr(float(v)) for v in row) for row in vert]
+        body += [f"{i % 7} {i * 0.5}" for i in range(extra_after)]
+        path.write_bytes(head + ("\n".join(body) + "\n").encode())
+        return
+    end = "<" if fmt == "binary_little_endian" else ">"
+    data = before.astype(end + "f4").tobytes() + vert.astype(end + "f4").tobytes()
+    rec = np.zeros(extra_after, dtype=[("flag", "u1"), ("weight", end + "f8")])
+    rec["flag"] = np.arange(extra_after) % 7
+    rec["weight"] = np.arange(extra_after) * 0.5
+    path.write_bytes(head + data + rec.tobytes())
+
+
+def check_loaded(r: dict, raw: S.RawSplats):
+    ref = S.activate(raw)
+    assert r["ok"], r.get("error")
+    assert r["n"] == raw.n
+    assert r["degree"] == raw.sh_degree
+    # f64 activations stored as f32: allow the last f32 bit for exp/sigmoid
+    # evaluated by a different libm.
+    for key, want in (("positions", ref.positions), ("colors", ref.colors), ("scales", ref.scales),
+                      ("rotations", ref.rotations)):
+        got = f32(r[key]).reshape(-1, 4)
+        np.testing.assert_allclose(got, want, rtol=2.5e-7, atol=0, err_msg=key)
+    np.testing.assert_array_equal(f32(r["fDc"]).reshape(-1, 3), raw.f_dc.astype(np.float32))
+    if raw.f_rest is None:
+        assert r["sh"] is None
+    else:
+        np.testing.assert_array_equal(f32(r["sh"]).reshape(raw.n, -1), raw.f_rest.astype(np.float32))
+    lo, hi = raw.xyz.min(axis=0), raw.xyz.max(axis=0)
+    np.testing.assert_array_equal(np.float32(r["bounds"][:3]), lo.astype(np.float32))
+    np.testing.assert_array_equal(np.float32(r["bounds"][3:]), hi.astype(np.float32))
+
+
+def small_raw(n=257, deg=0, seed=5) -> S.RawSplats:
+    raw = S.uniform_cube(seed, n, sh_degree=deg)
+    if n >= 2:
+        raw.rot[0] = 0.0  # zero quaternion: the loader keeps it as is (inv = 1)
+        raw.opacity_logit[1] = 80.0  # sigmoid saturates to 1
+    return raw
+
+
+@pytest.mark.parametrize("fmt", ["ascii", "binary_little_endian", "binary_big_endian"])
+@pytest.mark.parametrize("deg", [0, 3])
+def test_ply_formats(tmp_path, fmt, deg):
+    raw = small_raw(deg=deg)
+    p = tmp_path / "s.ply"
+    write_ply(p, raw, fmt, extra_before=2, extra_after=3)
+    check_loaded(run_node({"cmd": "ply", "path": str(p)}, tmp_path), raw)
+
+
+def test_ply_in_caller_thread_matches_worker(tmp_path):
+    raw = small_raw(n=64, deg=1)
+    p = tmp_path / "s.ply"
+    write_ply(p, raw, "binary_little_endian")
+    a = run_node({"cmd": "ply", "path": str(p), "worker": True}, tmp_path)
+    b = run_node({"cmd": "ply", "path": str(p), "worker": False}, tmp_path)
+    assert a == b
+    check_loaded(a, raw)
+
+
+def test_ply_f_rest_ordered_by_index(tmp_path):
+    """f_rest columns in shuffled header order are read back by their index
+    (SceneWorker.ts:375-388), so f_rest_10 is not placed after f_rest_1."""
+    raw = small_raw(n=50, deg=2)
+    cols = list(raw_columns(raw))
+    rest = [c for c in cols if c.startswith("f_rest_")]
+    order = PROPS[:7] + rest[::-1] + PROPS[7:]
+    p = tmp_path / "s.ply"
+    write_ply(p, raw, "binary_little_endian", order=order)
+    r = run_node({"cmd": "ply", "path": str(p)}, tmp_path)
+    check_loaded(r, raw)
+    assert r["degree"] == 2
+
+
+@pytest.mark.parametrize("n_rest,deg", [(0, 0), (8, 0), (9, 1), (23, 1), (24, 2), (44, 2), (45, 3)])
+def test_ply_sh_degree_from_rest_count(tmp_path, n_rest, deg):
+    raw = small_raw(n=8, deg=3)
+    raw.f_rest = raw.f_rest[:, :n_rest] if n_rest else None
+    p = tmp_path / "s.ply"
+    write_ply(p, raw, "binary_little_endian")
+    r = run_node({"cmd": "ply", "path": str(p)}, tmp_path)
+    assert r["degree"] == deg
+    if n_rest:
+        assert f32(r["sh"]).size == 8 * n_rest
+
+
+def test_ply_empty_vertex_element(tmp_path):
+    raw = small_raw(n=0)
+    p = tmp_path / "s.ply"
+    write_ply(p, raw, "binary_little_endian")
+    r = run_node({"cmd": "ply", "path": str(p)}, tmp_path)
+    assert r["ok"] and r["n"] == 0
+
+
+@pytest.mark.parametrize("case,msg", [
+    ("missing", "missing property 'opacity'"),
+    ("format", "Unsupported PLY format"),
+    ("magic", "must start with `ply`"),
+    ("list", "list properties"),
+    ("truncated", "outside the bounds"),
+    ("noheader", "missing end_header"),
+])
+def test_ply_errors(tmp_path, case, msg):
+    raw = small_raw(n=4)
+    p = tmp_path / "s.ply"
+    if case == "missing":
+        write_ply(p, raw, "binary_little_endian", drop="opacity")
+    else:
+        write_ply(p, raw, "binary_little_endian")
+        b = p.read_bytes()
+        if case == "format":
+            b = b.replace(b"binary_little_endian", b"binary_middle_endian")
+        elif case == "magic":
+            b = b"plx" + b[3:]
+        elif case == "list":
+            b = b.replace(b"end_header", b"property list uchar int vertex_indices\nend_header")
+        elif case == "truncated":
+            b = b[:-5]
+        elif case == "noheader":
+            b = b.replace(b"end_header", b"end_headr")
+        p.write_bytes(b)
+    r = run_node({"cmd": "ply", "path": str(p)}, tmp_path)
+    assert not r["ok"]
+    assert msg in r["error"]
+
+
+# ---------------------------------------------------------------- camera
+
+def look_at_closed_form(eye, center, up):
+    """Column-major right-handed look-at: rows of R are (s, u, -f), t = -R eye."""
+    eye, center, up = (np.asarray(v, np.float64) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = s, u, -f
+    m[:3, 3] = -m[:3, :3] @ eye
+    return m.T.reshape(-1)  # column-major storage
+
+
+def perspective_closed_form(fovy, aspect, near, far):
+    """OpenGL projection, clip z in [-1, 1]."""
+    t = 1.0 / math.tan(fovy / 2)
+    m = np.zeros((4, 4))
+    m[0, 0] = t / aspect
+    m[1, 1] = t
+    m[2, 2] = (far + near) / (near - far)
+    m[2, 3] = 2 * far * near / (near - far)
+    m[3, 2] = -1.0
+    return m.T.reshape(-1)
+
+
+CAMS = [
+    dict(position=[10, 10, 10], target=[0, 0, 0], up=[0, 1, 0], fov=math.pi / 2, aspect=1.0, near=0.1,
+         far=10000, width=800, height=800),
+    dict(position=[0.5, -2.0, 7.25], target=[0.1, 0.2, -0.3], up=[0, 1, 0], fov=math.pi / 3,
+         aspect=1920 / 1080, near=0.1, far=10000, width=1920, height=1080),
+    dict(position=[-3, 1, -4], target=[1, 0.5, 2], up=[0, 1, 0], fov=1.0, aspect=0.75, near=0.5,
+         far=200, width=600, height=800),
+]
+
+
+def test_camera_uniform_closed_forms(tmp_path):
+    r = run_node({"cmd": "camera", "cameras": CAMS}, tmp_path)
+    for c, out in zip(CAMS, r["cameras"]):
+        u = f32(out["uniform"]).astype(np.float64)
+        assert u.size == 72
+        view = look_at_closed_form(c["position"], c["target"], c["up"])
+        proj = perspective_closed_form(c["fov"], c["aspect"], c["near"], c["far"])
+        np.testing.assert_allclose(u[0:16], view, rtol=0, atol=2e-6 * max(1, np.abs(view).max()))
+        np.testing.assert_allclose(u[16:32], proj, rtol=3e-7, atol=1e-12)
+        iv = np.linalg.inv(view.reshape(4, 4).T).T.reshape(-1)
+        ip = np.linalg.inv(proj.reshape(4, 4).T).T.reshape(-1)
+        np.testing.assert_allclose(u[32:48], iv, rtol=0, atol=2e-6 * max(1, np.abs(iv).max()))
+        np.testing.assert_allclose(u[48:64], ip, rtol=1e-6, atol=1e-6)
+        np.testing.assert_array_equal(u[64:68], np.float32(c["position"] + [1.0]))
+        assert u[68] == c["width"] and u[69] == c["height"]
+        assert u[70] == np.float32(c["width"] / c["height"]) and u[71] == 0
+
+
+def test_camera_matches_python_mirror(tmp_path):
+    """The JS Camera and gs3d.camera produce the same 72 floats, up to f32
+    rounding of the inverses (the two hosts invert with different formulas)."""
+    r = run_node({"cmd": "camera", "cameras": CAMS}, tmp_path)
+    for c, out in zip(CAMS, r["cameras"]):
+        py = C.Camera(c["position"], c["target"], c["up"], c["fov"], c["aspect"], c["near"], c["far"])
+        u, v = f32(out["uniform"]), py.uniform_data(c["width"], c["height"])
+        np.testing.assert_array_equal(u[:32], v[:32])
+        np.testing.assert_array_equal(u[64:], v[64:])
+        np.testing.assert_allclose(u[32:64], v[32:64], rtol=1e-6, atol=1e-6)
+
+
+def test_camera_controls(tmp_path):
+    """rotate keeps the distance to the target, pan moves position and target by
+    the same offset, zoom stops at the near plane; each bumps the version."""
+    base = dict(CAMS[1])
+    cams = [dict(base, ops=[["rotate", 30.0, -12.0]]), dict(base, ops=[["pan", 5.0, -3.0]]),
+            dict(base, ops=[["zoom", 1e6]]), dict(base, ops=[["setPosition", [1, 2, 3]]]),
+            dict(base, ops=[["setAspect", 2.0]])]
+    r = run_node({"cmd": "camera", "cameras": cams}, tmp_path)["cameras"]
+    p0, t0 = np.array(base["position"], float), np.array(base["target"], float)
+    d0 = np.linalg.norm(p0 - t0)
+    rot = r[0]
+    assert abs(np.linalg.norm(np.array(rot["position"]) - t0) - d0) < 1e-4 * d0
+    assert not np.allclose(rot["position"], p0)
+    pan = r[1]
+    np.testing.assert_allclose(np.array(pan["position"]) - p0, np.array(pan["target"]) - t0, atol=1e-5)
+    zoom = r[2]
+    assert abs(np.linalg.norm(np.array(zoom["position"]) - t0) - base["near"]) < 1e-4
+    np.testing.assert_allclose(r[3]["position"], [1, 2, 3])
+    for out in r:
+        assert out["versionDelta"] >= 1
+    u = f32(r[4]["uniform"]).astype(np.float64)
+    np.testing.assert_allclose(u[16:32], perspective_closed_form(base["fov"], 2.0, base["near"], base["far"]),
+                               rtol=3e-7, atol=1e-12)
+
+
+# ---------------------------------------------------------------- managers
+
+LIMITS = dict(maxTotalThreads=1024, maxSizeX=1024, maxSizeY=1024, maxSizeZ=1024, maxDispatch=2147483647,
+              maxSharedMemory=65536)
+
+
+def test_workgroup_strategies(tmp_path):
+    cases = [
+        dict(name="a", strategy="linear1D", args=[256], problemSize=[1000, 1, 1]),
+        dict(name="b", strategy="linear1D", args=[256, 16], problemSize=[16_000_000, 1, 1]),
+        dict(name="c", strategy="linear1D", args=[300], problemSize=[1, 1, 1]),
+        dict(name="d", strategy="tiled1D", args=[8, 4, 256], problemSize=[1_000_000, 1, 1]),
+        dict(name="e", strategy="tiled1D", args=[8, 4096, 1024], problemSize=[100, 1, 1]),
+        dict(name="f", strategy="tile2D", args=[16, 16], problemSize=[1920, 1080, 1]),
+        dict(name="g", strategy="tile2D", args=[64, 32], problemSize=[10, 10, 1]),
+        dict(name="h", strategy="linear1D", args=[256], problemSize=[0, 1, 1]),
+    ]
+    r = run_node({"cmd": "workgroups", "limits": LIMITS, "cases": cases}, tmp_path)["layouts"]
+    assert r[0] == {"workgroupSize": [256, 1, 1], "dispatchSize": [4, 1, 1]}
+    assert r[1] == {"workgroupSize": [256, 1, 1], "dispatchSize": [3907, 1, 1]}
+    assert r[2] == {"workgroupSize": [256, 1, 1], "dispatchSize": [1, 1, 1]}  # highest power of two <= 300
+    assert r[3] == {"workgroupSize": [256, 1, 1], "dispatchSize": [489, 1, 1]}
+    # 8 elements x 4096 B x x <= 65536 forces x = 2
+    assert r[4] == {"workgroupSize": [2, 1, 1], "dispatchSize": [7, 1, 1]}
+    assert r[5] == {"workgroupSize": [16, 16, 1], "dispatchSize": [120, 68, 1]}
+    assert r[6] == {"error": "Workgroup layout exceeds hardware limits."}  # 64 x 32 = 2048 threads
+    assert r[7] == {"workgroupSize": [256, 1, 1], "dispatchSize": [1, 1, 1]}
+
+
+def test_workgroup_dispatch_clamp(tmp_path):
+    lim = dict(LIMITS, maxDispatch=65535)
+    cases = [dict(name="a", strategy="linear1D", args=[256], problemSize=[256 * 70000, 1, 1])]
+    r = run_node({"cmd": "workgroups", "limits": lim, "cases": cases}, tmp_path)["layouts"]
+    assert r[0]["dispatchSize"] == [65535, 1, 1]
+
+
+def test_pipeline_variants(tmp_path):
+    cases = [
+        dict(name="radix-sort", codeConstants={"RADIX_BITS": 8}),
+        dict(name="radix-sort", codeConstants={"RADIX_BITS": 4}, update=True),
+        dict(name="radix-sort", codeConstants={"RADIX_BITS": 5}, update=True),
+        dict(name="rasterize-tiles", codeConstants={"TILE_X": 16, "TILE_Y": 16, "TILE_AREA": 256}),
+        dict(name="rasterize-tiles", codeConstants={"PIXELS_PER_LANE": 2}, update=True),
+        dict(name="rasterize-tiles", codeConstants={"TILE_X": 8, "TILE_Y": 16}, update=True),
+        dict(name="no-such-kernel", codeConstants={}),
+        dict(name="prefix-scan", codeConstants={"FOO": 1}),
+    ]
+    r = run_node({"cmd": "pipelines", "cases": cases}, tmp_path)["results"]
+    assert r[0] == {"ok": True, "constants": {"RADIX_BITS": 8}}
+    assert r[1] == {"ok": True, "constants": {"RADIX_BITS": 4}}
+    assert not r[2]["ok"] and "RADIX_BITS=5" in r[2]["error"]
+    assert r[3]["ok"]
+    assert r[4] == {"ok": True, "constants": {"TILE_X": 16, "TILE_Y": 16, "TILE_AREA": 256, "PIXELS_PER_LANE": 2}}
+    assert not r[5]["ok"] and "square" in r[5]["error"]
+    assert not r[6]["ok"] and "No compiled kernel" in r[6]["error"]
+    assert not r[7]["ok"]
+
+
+@pytest.mark.skipif(not ADDON.exists(), reason="addon not built")
+def test_addon_exports(tmp_path):
+    """The addon loads and exports one function per bound C entry point."""
+    r = run_node({"cmd": "addonExports"}, tmp_path)
+    want = {"create", "destroy", "lastError", "sync", "deviceLimits", "setSplats", "setTileSize",
+            "getTileSize", "getTileCount", "rebin", "refCount", "read", "readTileCounts", "stageTimes",
+            "stageTimesAvg", "profilerReset", "stageLabel", "bufferCreate", "bufferResize", "bufferSize",
+            "bufferWrite", "bufferRead", "bufferDestroy", "bufferList", "setKernelVariant",
+            "getKernelVariant", "setProfileMask", "sortPasses", "abiVersion"}
+    assert set(r["names"]) == want
+    assert r["abiVersion"] >= 1
+
+
+# ---------------------------------------------------------------- GPU render through node
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile,ppl", [(16, 1), (8, 1), (32, 2)])
+def test_node_render_matches_oracle(tmp_path, tile, ppl):
+    import oracle_lib as O
+
+    raw = S.uniform_cube(11, 20_000, sh_degree=3)
+    p = tmp_path / "scene.ply"
+    write_ply(p, raw, "binary_little_endian")
+    W, H = 320, 240
+    eye = [3.0, 2.0, 4.0]
+    r = run_node({"cmd": "render", "path": str(p), "tile": tile, "pixelsPerLane": ppl, "eye": eye,
+                  "width": W, "height": H}, tmp_path, timeout=300)
+    assert r["decisions"] == [True, False, True]  # the rebin gate: unchanged camera skips the rebin
+    assert r["rebinCount"] == 2
+    assert r["tileSize"] == [tile, tile]
+    assert r["variant"] == ppl
+    s = S.activate(raw)
+    cam = f32(r["uniform"])
+    ref = O.pipeline_f32(cam, s, W, H, tile)
+    assert r["refCount"] == ref["k"]
+    img = np.frombuffer(base64.b64decode(r["image"]), np.float16).astype(np.float32).reshape(H, W, 4)
+    err = np.abs(img - ref["image"].reshape(H, W, 4)).max()
+    assert err <= 1.0 / 255.0, err
+    counts = np.frombuffer(base64.b64decode(r["tileCounts"]), np.uint32)
+    np.testing.assert_array_equal(counts, np.diff(ref["tile_offsets"].astype(np.int64)).clip(0))

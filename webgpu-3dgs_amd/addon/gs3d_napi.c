@@ -453,6 +453,63 @@ static napi_value js_buffer_list(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* setKernelVariant(ctx, kernel, param, value): PipelineManager.update with new
+ * codeConstants (PipelineManager.ts:66-105). */
+static napi_value js_set_kernel_variant(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char kernel[64], param[64];
+    int32_t value = 0;
+    if (!get_name(env, argv[1], kernel, sizeof kernel) || !get_name(env, argv[2], param, sizeof param) ||
+        napi_get_value_int32(env, argv[3], &value) != napi_ok) {
+        napi_throw_type_error(env, NULL, "setKernelVariant(ctx, kernel, param, value)");
+        return NULL;
+    }
+    CHECK(env, c, gs3d_set_kernel_variant(c, kernel, param, value), "gs3d_set_kernel_variant");
+    return NULL;
+}
+
+static napi_value js_get_kernel_variant(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char kernel[64], param[64];
+    if (!get_name(env, argv[1], kernel, sizeof kernel) || !get_name(env, argv[2], param, sizeof param)) {
+        napi_throw_type_error(env, NULL, "getKernelVariant(ctx, kernel, param)");
+        return NULL;
+    }
+    int value = 0;
+    CHECK(env, c, gs3d_get_kernel_variant(c, kernel, param, &value), "gs3d_get_kernel_variant");
+    napi_value r;
+    napi_create_int32(env, value, &r);
+    return r;
+}
+
+/* setProfileMask(ctx, mask): which stages a profiled rebin brackets with events. */
+static napi_value js_set_profile_mask(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t mask = 0;
+    if (!get_u32(env, argv[1], &mask)) {
+        napi_throw_type_error(env, NULL, "setProfileMask(ctx, mask)");
+        return NULL;
+    }
+    CHECK(env, c, gs3d_set_profile_mask(c, mask), "gs3d_set_profile_mask");
+    return NULL;
+}
+
+/* sortPasses(ctx): bit offsets of the key-sort passes the last rebin ran. */
+static napi_value js_sort_passes(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t n = 0, offs[16];
+    CHECK(env, c, gs3d_sort_passes(c, &n, offs), "gs3d_sort_passes");
+    return make_typed(env, napi_uint32_array, offs, n, 4);
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor d[] = {
 #define FN(js, c) {js, NULL, c, NULL, NULL, NULL, napi_enumerable, NULL}
@@ -480,6 +537,10 @@ static napi_value init(napi_env env, napi_value exports) {
         FN("bufferRead", js_buffer_read),
         FN("bufferDestroy", js_buffer_destroy),
         FN("bufferList", js_buffer_list),
+        FN("setKernelVariant", js_set_kernel_variant),
+        FN("getKernelVariant", js_get_kernel_variant),
+        FN("setProfileMask", js_set_profile_mask),
+        FN("sortPasses", js_sort_passes),
 #undef FN
     };
     NAPI_CALL(env, napi_define_properties(env, exports, sizeof d / sizeof d[0], d));

@@ -29,7 +29,7 @@ class GaussianSplatRenderer {
     this.ctx = addon.create(device);
     this.scene = scene;
     this.bufferManager = new BufferManager(addon, this.ctx);
-    this.pipelineManager = new PipelineManager();
+    this.pipelineManager = new PipelineManager(addon, this.ctx);
     this.workgroups = WorkgroupManager.fromDevice(addon, this.ctx);
     this.profiler = profiler;
     this.tileSizeX = 16;

@@ -169,11 +169,21 @@ const KERNEL_VARIANTS = {
   'emit-tile-refs': { constants: {} },
   'radix-sort': { constants: { RADIX_BITS: [4, 8] } },
   'identify-tile-ranges': { constants: {} },
-  'rasterize-tiles': { constants: { TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024] } },
+  'rasterize-tiles': {
+    constants: { TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4] },
+  },
 };
 
+// Constants the library selects by gs3d_set_kernel_variant; the tile constants
+// follow gs3d_set_tile_size instead.
+const LIBRARY_CONSTANTS = new Set(['RADIX_BITS', 'PIXELS_PER_LANE']);
+
 class PipelineManager {
-  constructor() {
+  // With an addon and context, creating or updating a pipeline also selects
+  // the matching compiled variant in the library.
+  constructor(addon = null, ctx = null) {
+    this.addon = addon;
+    this.ctx = ctx;
     this.pipelines = new Map();
     this.pipelineConfigs = new Map();
   }
@@ -191,6 +201,11 @@ class PipelineManager {
     }
     if ('TILE_X' in constants && constants.TILE_X !== constants.TILE_Y) {
       throw new Error(`Pipeline "${config.name}": tiles are square (8, 16 or 32)`);
+    }
+    if (this.addon) {
+      for (const [k, v] of Object.entries(constants)) {
+        if (LIBRARY_CONSTANTS.has(k)) this.addon.setKernelVariant(this.ctx, kernel, k, v);
+      }
     }
     const pipeline = { label: `pipeline-${config.name}`, kernel, constants };
     this.pipelines.set(config.name, pipeline);
