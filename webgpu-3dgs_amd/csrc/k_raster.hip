@@ -13,17 +13,18 @@
 // (pixel centre +0.5; skip if |d|^2 > r^2; power = -1/2 d^T conic d, skip if
 // power > 0; alpha = min(0.99, o e^power), skip if alpha < 1/255; C += T a c,
 // T *= 1-a, THEN stop when T < 1e-4; out = (C, 1-T) as rgba16float). The
-// schedule is CDNA-native: each wavefront owns 64 pixels of the tile and walks
-// the tile's sorted list on its own, 64 splats per step: one lane gathers one
-// splat (coalesced id load + three 16-B loads), splats that cannot reach the
-// wavefront's pixel rows are dropped by a ballot, and the survivors are
-// compacted into the wavefront's LDS slots and broadcast from there. No
-// workgroup barriers, and a wavefront leaves as soon as all its pixels are
-// saturated. Splats that cannot reach alpha >= 1/255 inside the wavefront's
-// rows (circle and opacity-aware ellipse bounds, see may_touch) are skipped
-// before the broadcast; this drops no blend the per-pixel rules would keep.
-// exp is the hardware exp2 (v_exp_f32) of x*log2(e); results stay within one
-// fp16 step of the fp32 restatement on the CPU.
+// schedule is CDNA-native: each wavefront owns a block of the tile's pixels
+// (8 x 8 at one pixel per lane) and walks the tile's sorted list on its own,
+// 64 splats per step: one lane gathers one splat (coalesced id load + three
+// 16-B loads), splats that cannot reach the block are dropped by a ballot, and
+// the survivors are compacted into the wavefront's LDS slots and broadcast from
+// there. No workgroup barriers, and a wavefront leaves as soon as all its
+// pixels are saturated. Splats that cannot reach alpha >= 1/255 inside the
+// block (radius circle and the exact minimum of the conic over the block, see
+// may_touch) are skipped before the broadcast; this drops no blend the
+// per-pixel rules would keep.
+// exp is the hardware exp2 (v_exp_f32) of the power pre-scaled by log2(e);
+// results stay within one fp16 step of the fp32 restatement on the CPU.
 #include <hip/hip_fp16.h>
 
 #include "gs3d_internal.h"
@@ -60,86 +61,138 @@ __global__ __launch_bounds__(256) void k_identify(const uint64_t* __restrict__ k
 
 // Conservative test of one gathered splat against a wavefront's rectangle of
 // pixel centres [rx0, rx1] x [ry0, ry1]. A pixel can only be blended when it is
-// inside the radius circle AND alpha = o e^{-q/2} >= 1/255, i.e. inside the
-// ellipse q = d^T conic d <= 2 ln(255 o); that ellipse's bounding box has
-// half-extents sqrt(t Sigma_xx), sqrt(t Sigma_yy) with Sigma = conic^-1. Both
-// shapes are widened by far more than float rounding (1% + 0.05 px), so the
-// test only drops splats that cannot pass the exact per-pixel rules.
+// inside the radius circle AND alpha = o e^{-q/2} >= 1/255, i.e. when
+// q = d^T conic d <= t = 2 ln(255 o). The circle test uses the distance from
+// the mean to the rectangle. For the ellipse, q is convex, so when the mean is
+// outside the rectangle the minimum of q over it lies on an edge facing the
+// mean; on such an edge q is a 1-D parabola whose minimum is the clamped
+// vertex. Both facing edges are evaluated (an edge that does not face gives a
+// value at a point of the rectangle, never below the minimum), so qmin is the
+// exact minimum over the rectangle. Both bounds are widened by far more than
+// float rounding (1-2 % + 0.05), so the test only drops splats that cannot
+// pass the exact per-pixel rules at any pixel centre of the rectangle.
 __device__ __forceinline__ bool may_touch(float4 f0, float4 f1, float rx0, float rx1, float ry0,
                                           float ry1) {
     const float mx = f0.x, my = f0.y, rad = f0.w;
-    const float cx = f1.x, cy = f1.y, cz = f1.z, op = f1.w;
+    const float ca = f1.x, cb = f1.y, cc = f1.z, op = f1.w;
     if (!(op * 255.0f >= 0.99f)) return false;  // alpha <= o < 1/255 everywhere
     const float qx = fminf(fmaxf(mx, rx0), rx1);
     const float qy = fminf(fmaxf(my, ry0), ry1);
-    const float ex = fabsf(mx - qx), ey = fabsf(my - qy);
+    const float ex = qx - mx, ey = qy - my;
     const float rr = rad * 1.01f + 0.05f;
     if (ex * ex + ey * ey > rr * rr) return false;
-    const float det = cx * cz - cy * cy;
+    if (!(ca > 0.0f && cc > 0.0f && ca * cc - cb * cb > 0.0f)) return true;  // degenerate: keep
+    // edge x = qx: vertex at dy = -cb ex / cc, clamped to the rectangle
+    const float vy = fminf(fmaxf(my - cb * ex / cc, ry0), ry1) - my;
+    const float q1 = ca * ex * ex + 2.0f * cb * ex * vy + cc * vy * vy;
+    // edge y = qy: vertex at dx = -cb ey / ca, clamped
+    const float vx = fminf(fmaxf(mx - cb * ey / ca, rx0), rx1) - mx;
+    const float q2 = ca * vx * vx + 2.0f * cb * vx * ey + cc * ey * ey;
     const float t = 2.0f * __logf(255.0f * op) * 1.02f + 0.05f;
-    if (!(det > 0.0f)) return true;  // degenerate conic: keep (exact rules decide)
-    const float hx = sqrtf(t * cz / det) * 1.01f + 0.05f;
-    const float hy = sqrtf(t * cx / det) * 1.01f + 0.05f;
-    return ex <= hx && ey <= hy;
+    return fminf(q1, q2) * 0.99f <= t;
 }
 
-// TW x TH tile; each lane owns PPL pixels of one column, so a wavefront covers
-// TW columns x (64 PPL / TW) rows and a workgroup has TW TH / PPL threads.
+// Pixel block of one wavefront: BW columns x BH rows, each lane owning PPL
+// pixels of one column spaced 64/BW rows apart. 8-pixel-wide blocks (8 x 8 at
+// one pixel per lane) touch fewer splats than full-width rows of the same area.
+template <int TW, int PPL>
+struct RasterShape {
+    static constexpr int BW = (8 * PPL <= TW) ? 8 : 16;
+    static constexpr int BH = kWave * PPL / BW;
+    static constexpr int kLaneStride = kWave / BW;  // rows between a lane's pixels
+    static constexpr int kBlocksX = TW / BW;
+    static constexpr int kWaves = (TW / BW) * (TW / BH);
+    static_assert(BH <= TW && TW % BW == 0 && TW % BH == 0, "blocks tile the tile");
+};
+
+// One survivor as the composite loop reads it: {mean.x, mean.y, r^2, A},
+// {B, C, opacity, red}, {green, blue} with A, B, C the conic scaled so that
+// A dx^2 + B dx dy + C dy^2 = log2(e) * power, the argument of v_exp_f32.
+struct alignas(16) Survivor {
+    float4 a;
+    float4 b;
+    float2 c;
+    float2 pad;
+};
+
+// TW x TW tile; RasterShape gives the wavefronts' pixel blocks.
 //
 // Per batch of 64 list entries a wavefront gathers one splat per lane, keeps
-// the ones that may touch its rows (may_touch) and compacts them, in list
-// order, into its own LDS slots as {mean.x, mean.y, r^2, conic.x},
-// {2 conic.y, conic.z, opacity, red}, {green, blue}. The composite loop then
-// reads slot j with uniform-address LDS loads (a broadcast) and blends
-// branch-free: a splat that fails a rule is blended with alpha 0, which leaves
-// C and T bit-identical (T * (1 - 0) = T, C + T * 0 * c = C).
-template <int TW, int TH, int PPL>
-__global__ __launch_bounds__(TW * TH / PPL) void k_rasterize(const float4* __restrict__ proj,
+// the ones that may touch its block (may_touch) and compacts them, in list
+// order, into its own LDS slots; the count is padded to a multiple of 4 with
+// zero-opacity entries, which fail alpha >= 1/255 and blend nothing. The
+// composite loop reads 4 slots at a time with uniform-address LDS loads (a
+// broadcast) and blends branch-free: a splat that fails a rule is blended with
+// alpha 0, which leaves C and T bit-identical (T * (1 - 0) = T,
+// C + T * 0 * c = C). The all-saturated exit is checked once per 4 splats.
+template <int TW, int PPL>
+__global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __restrict__ proj,
                                                              const uint32_t* __restrict__ values,
                                                              const uint32_t* __restrict__ tile_offsets,
                                                              uint32_t tiles_x, uint32_t width,
                                                              uint32_t height, uint2* __restrict__ image) {
-    static_assert(kWave % TW == 0 && (TW * TH) % (kWave * PPL) == 0, "wavefronts cover whole rows");
-    constexpr int kLaneRows = kWave / TW;         // rows covered by one pixel slot
-    constexpr int kWaveRows = kLaneRows * PPL;    // rows covered by one wavefront
-    constexpr int kWaves = TW * TH / (PPL * kWave);
-    __shared__ float4 s_a[kWaves][kWave];
-    __shared__ float4 s_b[kWaves][kWave];
-    __shared__ float2 s_c[kWaves][kWave];
+    using Sh = RasterShape<TW, PPL>;
+    constexpr float kLog2e = 1.4426950408889634f;
+    __shared__ Survivor s_slot[Sh::kWaves][kWave];
 
     const uint32_t tile = blockIdx.x;
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
-    const uint32_t px = tx * TW + (lane % TW);
-    const uint32_t py0 = ty * TH + wave * kWaveRows + lane / TW;
+    const uint32_t bx0 = tx * TW + (wave % Sh::kBlocksX) * Sh::BW;
+    const uint32_t by0 = ty * TW + (wave / Sh::kBlocksX) * Sh::BH;
+    const uint32_t px = bx0 + (lane % Sh::BW);
+    const uint32_t py0 = by0 + lane / Sh::BW;
     const float pxf = (float)px + 0.5f;
 
-    // pixel-centre rectangle of this wavefront's rows
-    const float rx0 = (float)(tx * TW) + 0.5f, rx1 = rx0 + (float)(TW - 1);
-    const float ry0 = (float)(ty * TH + wave * kWaveRows) + 0.5f;
-    const float ry1 = ry0 + (float)(kWaveRows - 1);
+    // pixel-centre rectangle of this wavefront's block
+    const float rx0 = (float)bx0 + 0.5f, rx1 = rx0 + (float)(Sh::BW - 1);
+    const float ry0 = (float)by0 + 0.5f, ry1 = ry0 + (float)(Sh::BH - 1);
 
+    // A pixel is live while T >= 1e-4: T only changes on a blend and the pixel
+    // stops at the first blend that takes T below 1e-4, so that test is the
+    // termination flag. Pixels outside the image start at T = 0 (never live).
     float c0[PPL], c1[PPL], c2[PPL], tr[PPL], pyf[PPL];
-    bool done[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-        const uint32_t py = py0 + k * kLaneRows;
+        const uint32_t py = py0 + k * Sh::kLaneStride;
         c0[k] = c1[k] = c2[k] = 0.0f;
-        tr[k] = 1.0f;
+        tr[k] = (px < width && py < height) ? 1.0f : 0.0f;
         pyf[k] = (float)py + 0.5f;
-        done[k] = !(px < width && py < height);
     }
+    constexpr float kTMin = 1.0e-4f;
     auto all_done = [&]() {
-        bool d = true;
+        bool l = false;
 #pragma unroll
-        for (int k = 0; k < PPL; k++) d = d && done[k];
-        return __ballot(!d) == 0ull;
+        for (int k = 0; k < PPL; k++) l = l || tr[k] >= kTMin;
+        return __ballot(l) == 0ull;
     };
+    auto blend = [&](const Survivor& s) {
+        const float dx = pxf - s.a.x;
+        const float dxa = s.a.w * dx;
+        const float dx2 = dx * dx;
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+            const float dy = pyf[k] - s.a.y;
+            const float u = fmaf(s.b.x, dy, dxa);         // A dx + B dy
+            const float p = fmaf(s.b.y * dy, dy, u * dx);  // log2(e) * power
+            const float d2 = fmaf(dy, dy, dx2);
+            const float alpha = fminf(s.b.z * __builtin_amdgcn_exp2f(p), 0.99f);
+            // non-short-circuit: all four tests are evaluated, no branch per splat
+            const bool ok = (tr[k] >= kTMin) & (d2 <= s.a.z) & (p <= 0.0f) & (alpha >= 1.0f / 255.0f);
+            const float ae = ok ? alpha : 0.0f;
+            const float w = tr[k] * ae;
+            c0[k] = fmaf(w, s.b.w, c0[k]);
+            c1[k] = fmaf(w, s.c.x, c1[k]);
+            c2[k] = fmaf(w, s.c.y, c2[k]);
+            tr[k] = tr[k] * (1.0f - ae);
+        }
+    };
+
     const uint32_t start = tile_offsets[tile];
     const uint32_t end = tile_offsets[tile + 1];
-    const float min_alpha = 1.0f / 255.0f;
+    Survivor* slots = s_slot[wave];
 
     // software pipeline: batch b+1 is gathered while batch b is composited
     float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
@@ -163,48 +216,41 @@ __global__ __launch_bounds__(TW * TH / PPL) void k_rasterize(const float4* __res
         const bool hit = have && may_touch(f0, f1, rx0, rx1, ry0, ry1);
         const uint64_t mask = __ballot(hit);
         const int nhit = __popcll(mask);
+        const int nhit4 = (nhit + 3) & ~3;
         if (hit) {
             const int slot = (int)__popcll(mask & ((1ull << lane) - 1ull));
-            s_a[wave][slot] = make_float4(f0.x, f0.y, f0.w * f0.w, f1.x);
-            s_b[wave][slot] = make_float4(2.0f * f1.y, f1.z, f1.w, f2.x);
-            s_c[wave][slot] = make_float2(f2.y, f2.z);
+            Survivor s;
+            s.a = make_float4(f0.x, f0.y, f0.w * f0.w, -0.5f * kLog2e * f1.x);
+            s.b = make_float4(-kLog2e * f1.y, -0.5f * kLog2e * f1.z, f1.w, f2.x);
+            s.c = make_float2(f2.y, f2.z);
+            s.pad = make_float2(0.f, 0.f);
+            slots[slot] = s;
+        }
+        if (lane >= nhit && lane < nhit4) {
+            // pad slots [nhit, nhit4), written by lane = slot whether or not that
+            // lane hit (a hit lane's own survivor went to a slot below nhit).
+            // Zero opacity: alpha = 0 < 1/255, blends nothing.
+            Survivor s;
+            s.a = make_float4(0.f, 0.f, 0.f, 0.f);
+            s.b = s.a;
+            s.c = make_float2(0.f, 0.f);
+            s.pad = s.c;
+            slots[lane] = s;
         }
         __builtin_amdgcn_wave_barrier();
-        // slot j+1 is read while slot j is blended
-        float4 na = s_a[wave][0], nb = s_b[wave][0];
-        float2 nc = s_c[wave][0];
-        for (int j = 0; j < nhit; j++) {
-            const float4 a = na, b = nb;
-            const float2 cc = nc;
-            if (j + 1 < nhit) {
-                na = s_a[wave][j + 1];
-                nb = s_b[wave][j + 1];
-                nc = s_c[wave][j + 1];
-            }
-            const float dx = pxf - a.x;
-#pragma unroll
-            for (int k = 0; k < PPL; k++) {
-                const float dy = pyf[k] - a.y;
-                const float d2 = dx * dx + dy * dy;
-                const float power = -0.5f * (a.w * dx * dx + b.x * dx * dy + b.y * dy * dy);
-                float alpha = b.z * __expf(power);
-                alpha = alpha > 0.99f ? 0.99f : alpha;
-                const bool ok = !done[k] && !(d2 > a.z) && !(power > 0.0f) && !(alpha < min_alpha);
-                const float ae = ok ? alpha : 0.0f;
-                const float w = tr[k] * ae;
-                c0[k] = c0[k] + w * b.w;
-                c1[k] = c1[k] + w * cc.x;
-                c2[k] = c2[k] + w * cc.y;
-                tr[k] = tr[k] * (1.0f - ae);
-                done[k] = done[k] || (ok && tr[k] < 1.0e-4f);
-            }
+        for (int j = 0; j < nhit4; j += 4) {
+            const Survivor s0 = slots[j], s1 = slots[j + 1], s2 = slots[j + 2], s3 = slots[j + 3];
+            blend(s0);
+            blend(s1);
+            blend(s2);
+            blend(s3);
             if (all_done()) break;
         }
         __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-        const uint32_t py = py0 + k * kLaneRows;
+        const uint32_t py = py0 + k * Sh::kLaneStride;
         if (px < width && py < height) {
             const uint32_t h0 = __half_as_ushort(__float2half_rn(c0[k]));
             const uint32_t h1 = __half_as_ushort(__float2half_rn(c1[k]));
@@ -236,7 +282,7 @@ template <int TW, int PPL>
 static void launch_raster_variant(uint32_t tiles, const float4* p, const uint32_t* values,
                                   const uint32_t* tile_offsets, uint32_t tiles_x, uint32_t width,
                                   uint32_t height, uint2* img, hipStream_t s) {
-    hipLaunchKernelGGL((k_rasterize<TW, TW, PPL>), dim3(tiles), dim3(TW * TW / PPL), 0, s, p, values,
+    hipLaunchKernelGGL((k_rasterize<TW, PPL>), dim3(tiles), dim3(TW * TW / PPL), 0, s, p, values,
                        tile_offsets, tiles_x, width, height, img);
 }
 
