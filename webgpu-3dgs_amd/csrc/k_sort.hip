@@ -8,10 +8,10 @@
 //
 // Here a pass is ONE kernel (a "onesweep" pass):
 //   1. a ticket gives the workgroup its 4096-key tile in launch order;
-//   2. each wavefront ranks its 1024 keys with a wave-level multisplit: per
-//      digit bit a 64-lane ballot narrows the mask of lanes holding the same
-//      digit; popcount below the lane = stable rank, the lowest lane bumps the
-//      wave's private LDS histogram;
+//   2. each wavefront ranks its keys (1024 of 64 bits or 512 of 32 bits) with
+//      a wave-level multisplit: per digit bit a 64-lane ballot narrows the
+//      mask of lanes holding the same digit; popcount below the lane = stable
+//      rank, the lowest lane bumps the wave's private LDS histogram;
 //   3. per digit: the tile's count is published as an aggregate, the workgroup
 //      looks back over predecessor tiles' status words (decoupled lookback,
 //      up to 8 predecessors per round trip) for its exclusive prefix, then
@@ -19,7 +19,8 @@
 //      the digit histograms fused into emit;
 //   4. keys, then values, are placed digit-sorted in LDS and written out in
 //      LDS order, so runs of equal digits leave as contiguous stores.
-// HBM traffic per pass: 12 B read + 12 B written per key. Tiles beyond
+// HBM traffic per pass: 12 B read + 12 B written per 64-bit key (8 + 8 for
+// 32-bit keys). Tiles beyond
 // n = *n_dev exit at once, so a graph-captured launch can size the grid to the
 // buffer capacity while the key count lives on the device.
 #include "gs3d_internal.h"
@@ -35,8 +36,9 @@ __device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan over the 256 threads of the block; all threads must call.
-__device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t x, uint32_t* s_tmp) {
+// Exclusive scan over the THREADS threads of the block; all threads must call.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     uint32_t v = x;
@@ -49,7 +51,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t x, uint32_
     __syncthreads();
     uint32_t prefix = 0;
 #pragma unroll
-    for (int w = 0; w < kSortThreads / kWave; w++)
+    for (int w = 0; w < THREADS / kWave; w++)
         if (w < wave) prefix += s_tmp[w];
     __syncthreads();
     return prefix + v - x;
@@ -61,43 +63,49 @@ __device__ __forceinline__ uint32_t block_exclusive_scan_256(uint32_t x, uint32_
 // a tile publishes all its digits with one coalesced store.
 constexpr int kLookbackGroup = 8;
 
-template <int BITS, typename KeyT, bool IOTA>
-__global__ __launch_bounds__(kSortThreads) void k_radix_pass(
+// THREADS x ITEMS keys per tile. LDS: the staging buffer (one KeyT per key;
+// values reuse it as u32), the per-wavefront digit counts, which the digit byte
+// of every sorted position later overwrites, and one signed per-digit delta:
+// global output index = delta[d] + position in the tile.
+template <int BITS, typename KeyT, bool IOTA, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_radix_pass(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
     const uint32_t* __restrict__ hist_pass, uint64_t* status, uint32_t* ticket, uint32_t n_max,
     uint32_t shift, uint32_t epoch) {
     constexpr int RADIX = 1 << BITS;
-    constexpr int WAVES = kSortThreads / kWave;
+    constexpr int WAVES = THREADS / kWave;
+    constexpr int TILE = THREADS * ITEMS;
     constexpr uint32_t MASK = RADIX - 1;
-    // keys are staged first, then values, through the same 32 KB; the digit of
-    // every sorted position is kept so the value pass needs no extra registers
-    __shared__ uint64_t s_stage[kSortTile];
-    __shared__ uint8_t s_digit[kSortTile];
-    __shared__ uint32_t s_wave_hist[WAVES][RADIX];
-    __shared__ uint32_t s_bin_local[RADIX];
-    __shared__ uint32_t s_bin_global[RADIX];
+    static_assert(THREADS >= RADIX && TILE >= kSortTile, "one thread per digit; status sized for kSortTile");
+    constexpr int kCountBytes = WAVES * RADIX * 4;
+    constexpr int kShared = kCountBytes > TILE ? kCountBytes : TILE;
+    __shared__ KeyT s_stage[TILE];
+    __shared__ __align__(16) uint8_t s_shared[kShared];
+    __shared__ int32_t s_delta[RADIX];
     __shared__ uint32_t s_tmp[WAVES];
     __shared__ uint32_t s_part;
+    uint32_t (*s_wave_hist)[RADIX] = reinterpret_cast<uint32_t (*)[RADIX]>(s_shared);
+    uint8_t* s_digit = s_shared;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
     if (tid == 0) s_part = atomicAdd(ticket, 1u);
-    for (int k = tid; k < WAVES * RADIX; k += kSortThreads) (&s_wave_hist[0][0])[k] = 0u;
+    for (int k = tid; k < WAVES * RADIX; k += THREADS) (&s_wave_hist[0][0])[k] = 0u;
     __syncthreads();
     const uint32_t part = s_part;
     const uint32_t n = *n_dev;
-    const size_t tile_base = (size_t)part * kSortTile;
+    const size_t tile_base = (size_t)part * TILE;
     // uniform over the block: past the keys, or more keys than the buffers hold
     // (the host re-runs the frame with larger buffers in that case)
     if (tile_base >= n || n > n_max) return;
 
-    const size_t wave_base = tile_base + (size_t)wave * (kWave * kSortItems);
-    KeyT k[kSortItems];
-    uint32_t rank[kSortItems];
+    const size_t wave_base = tile_base + (size_t)wave * (kWave * ITEMS);
+    KeyT k[ITEMS];
+    uint32_t rank[ITEMS];
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
+    for (int it = 0; it < ITEMS; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
         k[it] = idx < n ? kin[idx] : (KeyT)~(KeyT)0;
     }
@@ -107,7 +115,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     // of the group below this one; the group's lowest lane bumps the count.
     const uint64_t lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
+    for (int it = 0; it < ITEMS; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
         const bool valid = idx < n;
         const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
@@ -124,7 +132,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
     }
     __syncthreads();
 
-    // Per digit: counts across waves -> wave offsets, tile count; publish.
+    // Per digit: counts across waves -> exclusive wave offsets and the tile
+    // count; publish the count; the tile-local digit start is added to the
+    // wave offsets once the block scan has it.
     uint32_t tile_count = 0;
     if (tid < RADIX) {
 #pragma unroll
@@ -136,11 +146,38 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
         store_status(&status[(size_t)part * RADIX + tid],
                      status_word(epoch, part == 0 ? kFlagPrefix : kFlagAggregate, tile_count));
     }
-    const uint32_t local_start = block_exclusive_scan_256(tid < RADIX ? tile_count : 0u, s_tmp);
-    const uint32_t global_start = block_exclusive_scan_256(tid < RADIX ? hist_pass[tid] : 0u, s_tmp);
+    const uint32_t local_start = block_exclusive_scan<THREADS>(tid < RADIX ? tile_count : 0u, s_tmp);
+    const uint32_t global_start = block_exclusive_scan<THREADS>(tid < RADIX ? hist_pass[tid] : 0u, s_tmp);
     if (tid < RADIX) {
-        s_bin_local[tid] = local_start;
-        // decoupled lookback, up to 8 predecessors per round trip
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) s_wave_hist[w][tid] += local_start;
+    }
+    __syncthreads();
+
+    // Position in the tile (tile-local, known before the lookback); then the
+    // counts' memory is free for the digit bytes. Keys go to LDS now and the
+    // value loads are issued now, so both leave the registers / hide their
+    // latency while the lookback waits on predecessor tiles.
+    uint32_t dig[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        dig[it] = (uint32_t)(k[it] >> shift) & MASK;
+        rank[it] += s_wave_hist[wave][dig[it]];
+    }
+    __syncthreads();
+    uint32_t val[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const size_t idx = wave_base + (size_t)it * kWave + lane;
+        if (idx < n) {
+            s_stage[rank[it]] = k[it];
+            s_digit[rank[it]] = (uint8_t)dig[it];
+            val[it] = IOTA ? (uint32_t)idx : vin[idx];
+        }
+    }
+
+    if (tid < RADIX) {
+        // decoupled lookback over kLookbackGroup predecessors per round trip
         uint32_t excl = 0;
         int64_t q = (int64_t)part - 1;  // nearest predecessor not yet folded in
         while (q >= 0) {
@@ -167,48 +204,30 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
         if (part != 0)
             store_status(&status[(size_t)part * RADIX + tid],
                          status_word(epoch, kFlagPrefix, excl + tile_count));
-        s_bin_global[tid] = global_start + excl;
+        s_delta[tid] = (int32_t)(global_start + excl) - (int32_t)local_start;
     }
     __syncthreads();
 
-    // keys: digit-sorted into LDS, then out in LDS order
-    KeyT* s_keys = reinterpret_cast<KeyT*>(s_stage);
-    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);
+    // keys out in LDS order (runs of equal digits are contiguous stores)
+    const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - tile_base);
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        const size_t idx = wave_base + (size_t)it * kWave + lane;
-        if (idx < n) {
-            const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
-            rank[it] += s_bin_local[d] + s_wave_hist[wave][d];  // position in the tile
-            s_keys[rank[it]] = k[it];
-            s_digit[rank[it]] = (uint8_t)d;
-        }
-    }
-    __syncthreads();
-    const uint32_t tile_n = (uint32_t)min((size_t)kSortTile, (size_t)n - tile_base);
-#pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        const uint32_t j = (uint32_t)(it * kSortThreads + tid);
-        if (j < tile_n) {
-            const uint32_t d = s_digit[j];
-            kout[s_bin_global[d] + (j - s_bin_local[d])] = s_keys[j];
-        }
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t j = (uint32_t)(it * THREADS + tid);
+        if (j < tile_n) kout[s_delta[s_digit[j]] + (int32_t)j] = s_stage[j];
     }
     __syncthreads();
     // values: same permutation, reusing the staging memory
+    uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
+    for (int it = 0; it < ITEMS; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
-        if (idx < n) s_vals[rank[it]] = IOTA ? (uint32_t)idx : vin[idx];
+        if (idx < n) s_vals[rank[it]] = val[it];
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        const uint32_t j = (uint32_t)(it * kSortThreads + tid);
-        if (j < tile_n) {
-            const uint32_t d = s_digit[j];
-            vout[s_bin_global[d] + (j - s_bin_local[d])] = s_vals[j];
-        }
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t j = (uint32_t)(it * THREADS + tid);
+        if (j < tile_n) vout[s_delta[s_digit[j]] + (int32_t)j] = s_vals[j];
     }
 }
 
@@ -216,12 +235,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_pass(
 
 uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / kSortTile + 1; }
 
+// Tile shapes (threads x keys per thread) for 64-bit and 32-bit keys.
+constexpr int kSort64Threads = 1024, kSort64Items = 16;
+constexpr int kSort32Threads = 1024, kSort32Items = 8;
+
 template <int BITS, typename KeyT, bool IOTA>
-static void launch_pass(uint32_t grid, const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
+static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                         const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass,
                         uint64_t* status, uint32_t* ticket, uint32_t shift, uint32_t epoch,
                         hipStream_t s) {
-    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA>), dim3(grid), dim3(kSortThreads), 0, s,
+    constexpr int kThreads = sizeof(KeyT) == 8 ? kSort64Threads : kSort32Threads;
+    constexpr int kItems = sizeof(KeyT) == 8 ? kSort64Items : kSort32Items;
+    const uint32_t grid = (n_max + kThreads * kItems - 1) / (kThreads * kItems);
+    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA, kThreads, kItems>), dim3(grid), dim3(kThreads), 0, s,
                        static_cast<const KeyT*>(kin), vin, static_cast<KeyT*>(kout), vout, n_dev,
                        hist_pass, status, ticket, n_max, shift, epoch);
 }
@@ -232,11 +258,10 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
                              hipStream_t s) {
     if (n_max == 0) return hipSuccess;
-    const uint32_t grid = (n_max + kSortTile - 1) / kSortTile;
     const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
                     (iota_values ? 1 : 0);
 #define GS3D_PASS(B, K, I) \
-    launch_pass<B, K, I>(grid, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, s)
+    launch_pass<B, K, I>(kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, s)
     switch (key) {
         case 110: GS3D_PASS(8, uint64_t, false); break;
         case 111: GS3D_PASS(8, uint64_t, true); break;
