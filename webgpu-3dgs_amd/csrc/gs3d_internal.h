@@ -30,15 +30,14 @@ __host__ __device__ inline uint64_t status_word(uint32_t epoch, uint64_t flag, u
     return ((uint64_t)epoch << 34) | (flag << 32) | (uint64_t)value;
 }
 
-// Scan: 256 threads x 8 counts per thread.
-constexpr int kScanThreads = 256;
+// Scan: 1024 threads x 8 counts per thread.
+constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// Radix: 256 threads (4 waves) x 16 keys per thread.
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;
+// Radix: the smallest tile (keys per workgroup) any pass shape uses; it sizes
+// the lookback status words and the key-capacity slack.
+constexpr int kSortTile = 4096;
 constexpr int kMaxPasses = 16;
 
 // Counters buffer slots (u32), cleared at the start of every rebin.
@@ -128,12 +127,13 @@ hipError_t launch_key_histogram(const void* keys, uint32_t key_bytes, const uint
 uint32_t radix_status_tiles(uint32_t n_max);
 // One stable LSD pass over n = *n_dev (key, value) pairs, keys of key_bytes
 // (4 or 8); with iota_values the input values are the positions 0..n-1 and vin
-// is not read. The grid covers n_max; n > n_max is a no-op.
+// is not read. The grid covers n_max; n > n_max is a no-op. With aux_in and
+// aux_out (32-bit keys, values read from vin): aux_out[p] = aux_in[vout[p]].
 hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                              uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
-                             hipStream_t s);
+                             const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s);
 // tile_offsets[0..tile_count] from n = *n_dev sorted keys (n > n_max: all 0).
 // If k_host is set (host-mapped memory), *k_host = *n_dev.
 hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,

@@ -58,15 +58,12 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
     s_sid[tid] = sid;
 
     const uint32_t total = block_end - block_start;
-    const int lane = tid & (kWave - 1);
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
     for (uint32_t w0 = 0; w0 < total; w0 += kOwnerWindow) {
         const uint32_t wn = min(kOwnerWindow, total - w0);
         __syncthreads();  // previous window's owners consumed (and the staging above visible)
         const uint32_t lo = max(my_start, w0), hi = min(my_start + my_count, w0 + wn);
         for (uint32_t x = lo; x < hi; x++) s_owner[x - w0] = (uint8_t)tid;
         __syncthreads();
-        // every lane of the wavefront runs the same number of rounds (ballots below)
         const uint32_t rounds = (wn + kEmitThreads - 1) / kEmitThreads;
         for (uint32_t round = 0; round < rounds; round++) {
             const uint32_t jw = round * kEmitThreads + tid;
@@ -86,16 +83,12 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
                     a.values[pos] = s_sid[o];
                 }
             }
-            if (a.hist_partial) {
-                // Lanes with equal digits are grouped by ballots (a wave-level
-                // multisplit); only each group's lowest lane touches LDS, so the
-                // digits that repeat across a wavefront (depth exponent, tile
-                // row) do not serialize on one LDS address.
+            if (a.hist_partial && live) {
+                // plain LDS atomics: a wavefront's keys come from ~18 splats in
+                // depth order, so their tile digits rarely collide
                 for (uint32_t p = 0; p < a.n_passes; p++) {
                     const uint32_t d = (uint32_t)(key >> (a.hist_shift0 + p * a.digit_bits)) & (radix - 1u);
-                    const uint64_t peers = match_digit_rt(d, live, a.digit_bits);
-                    if (live && (peers & lt_mask) == 0ull)
-                        atomicAdd(&s_hist[p * radix + d], (uint32_t)__popcll(peers));
+                    atomicAdd(&s_hist[p * radix + d], 1u);
                 }
             }
         }
@@ -129,23 +122,46 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
     }
 }
 
-// hist[w] += sum over rows r of partial[r][w], rows split over gridDim.y
-// slices (one atomic per word per slice).
+// hist[w] += sum over rows r of partial[r][w]. A workgroup folds a block of
+// kReduceWords words x kReduceRows rows: four groups of 64 threads each sum
+// 16 rows (independent coalesced loads), LDS adds the groups, and one atomic
+// per word and block leaves.
+constexpr uint32_t kReduceWords = 64, kReduceRows = 64;
+
 __global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* __restrict__ partial,
                                                      uint32_t rows, uint32_t words,
                                                      uint32_t* __restrict__ hist) {
-    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
-    if (w >= words) return;
-    const uint32_t per = (rows + gridDim.y - 1) / gridDim.y;
-    const uint32_t r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
+    __shared__ uint32_t s_sum[3][kReduceWords];
+    const uint32_t wl = threadIdx.x % kReduceWords, g = threadIdx.x / kReduceWords;
+    const uint32_t w = blockIdx.x * kReduceWords + wl;
+    const uint32_t r0 = blockIdx.y * kReduceRows + g * (kReduceRows / 4);
     uint32_t sum = 0;
-    for (uint32_t r = r0; r < r1; r++) sum += partial[(size_t)r * words + w];
-    if (sum) atomicAdd(&hist[w], sum);
+    if (w < words) {
+#pragma unroll
+        for (uint32_t i = 0; i < kReduceRows / 4; i++) {
+            const uint32_t r = r0 + i;
+            if (r < rows) sum += partial[(size_t)r * words + w];
+        }
+    }
+    if (g > 0) s_sum[g - 1][wl] = sum;
+    __syncthreads();
+    if (g == 0 && w < words) {
+        sum += s_sum[0][wl] + s_sum[1][wl] + s_sum[2][wl];
+        if (sum) atomicAdd(&hist[w], sum);
+    }
+}
+
+void launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t words, uint32_t* hist,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_hist_reduce, dim3((words + kReduceWords - 1) / kReduceWords,
+                                           (rows + kReduceRows - 1) / kReduceRows),
+                       dim3(256), 0, s, partial, rows, words, hist);
 }
 
 // Digit histograms of n = *n_dev keys for every pass, one plain row per
-// workgroup (grid-stride; equal digits within a wavefront aggregated by
-// ballots as in k_emit).
+// workgroup (grid-stride, kHistKeys keys per thread per step, LDS atomics).
+constexpr uint32_t kHistKeys = 4;
+
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ keys,
                                                        const uint32_t* __restrict__ n_dev,
@@ -159,19 +175,20 @@ __global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ 
     __syncthreads();
     uint32_t n = *n_dev;
     if (n > n_max) n = 0;  // keys did not fit: count nothing, the frame is re-run
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t rounds = (n + stride - 1) / stride;
-    for (size_t r = 0; r < rounds; r++) {
-        const size_t i = r * stride + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-        const bool live = i < n;
-        const KeyT key = live ? keys[i] : (KeyT)0;
-        for (uint32_t p = 0; p < n_passes; p++) {
-            const uint32_t d = (uint32_t)(key >> (shift0 + p * digit_bits)) & (radix - 1u);
-            const uint64_t peers = match_digit_rt(d, live, digit_bits);
-            if (live && (peers & lt_mask) == 0ull)
-                atomicAdd(&s_hist[p * radix + d], (uint32_t)__popcll(peers));
+    const size_t stride = (size_t)gridDim.x * blockDim.x * kHistKeys;
+    for (size_t base = (size_t)blockIdx.x * blockDim.x * kHistKeys; base < n; base += stride) {
+        KeyT k[kHistKeys];
+#pragma unroll
+        for (uint32_t j = 0; j < kHistKeys; j++) {
+            const size_t i = base + j * blockDim.x + threadIdx.x;
+            k[j] = i < n ? keys[i] : (KeyT)0;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kHistKeys; j++) {
+            if (base + j * blockDim.x + threadIdx.x >= n) continue;
+            for (uint32_t p = 0; p < n_passes; p++)
+                atomicAdd(&s_hist[p * radix + ((uint32_t)(k[j] >> (shift0 + p * digit_bits)) & (radix - 1u))],
+                          1u);
         }
     }
     __syncthreads();
@@ -191,18 +208,14 @@ hipError_t launch_emit(const EmitArgs& a, hipStream_t s) {
     if (a.n_passes > kMaxPasses || (a.digit_bits != 4 && a.digit_bits != 8)) return hipErrorInvalidValue;
     const uint32_t grid = emit_grid(a.n);
     hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kEmitThreads), 0, s, a);
-    if (a.hist_partial && a.hist) {
-        const uint32_t words = a.n_passes * (1u << a.digit_bits);
-        const uint32_t slices = grid < 32u ? grid : 32u;
-        hipLaunchKernelGGL(k_hist_reduce, dim3((words + 255u) / 256u, slices), dim3(256), 0, s,
-                           a.hist_partial, grid, words, a.hist);
-    }
+    if (a.hist_partial && a.hist)
+        launch_hist_reduce(a.hist_partial, grid, a.n_passes * (1u << a.digit_bits), a.hist, s);
     return hipGetLastError();
 }
 
 uint32_t key_hist_rows(uint32_t n_max) {
-    const uint32_t blocks = (n_max + 255u) / 256u;
-    return blocks < 512u ? (blocks ? blocks : 1u) : 512u;
+    const uint32_t blocks = (n_max + 256u * kHistKeys - 1u) / (256u * kHistKeys);
+    return blocks < 1024u ? (blocks ? blocks : 1u) : 1024u;
 }
 
 hipError_t launch_key_histogram(const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
@@ -223,10 +236,7 @@ hipError_t launch_key_histogram(const void* keys, uint32_t key_bytes, const uint
     } else {
         return hipErrorInvalidValue;
     }
-    const uint32_t words = n_passes * (1u << digit_bits);
-    const uint32_t slices = grid < 32u ? grid : 32u;
-    hipLaunchKernelGGL(k_hist_reduce, dim3((words + 255u) / 256u, slices), dim3(256), 0, s,
-                       hist_partial, grid, words, hist);
+    launch_hist_reduce(hist_partial, grid, n_passes * (1u << digit_bits), hist, s);
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@
 // per 2S chunk), prefix_scan_blocks.wgsl:28-43 (one thread scanning chunk sums;
 // writes ref_counter[0] and the sentinel offsets[N]) and prefix_scan_add.wgsl
 // :22-37 by one decoupled-lookback launch: each workgroup takes a ticket (so
-// tiles are processed in launch order), scans its 2048 counts, publishes its
+// tiles are processed in launch order), scans its 8192 counts, publishes its
 // aggregate, looks back over its predecessors' status words a wavefront (64
 // predecessors) at a time, then writes final offsets. The result is the same
 // exclusive sum mod 2^32 with offsets[N] = ref_counter[0] = total.

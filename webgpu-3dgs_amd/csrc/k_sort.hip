@@ -67,12 +67,15 @@ constexpr int kLookbackGroup = 8;
 // values reuse it as u32), the per-wavefront digit counts, which the digit byte
 // of every sorted position later overwrites, and one signed per-digit delta:
 // global output index = delta[d] + position in the tile.
-template <int BITS, typename KeyT, bool IOTA, int THREADS, int ITEMS>
+// AUX: also write aux_out[position] = aux_in[value] (a gather through the
+// sorted values, overlapped with the key and value write-out).
+template <int BITS, typename KeyT, bool IOTA, int THREADS, int ITEMS, bool AUX>
 __global__ __launch_bounds__(THREADS) void k_radix_pass(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
     const uint32_t* __restrict__ hist_pass, uint64_t* status, uint32_t* ticket, uint32_t n_max,
-    uint32_t shift, uint32_t epoch) {
+    uint32_t shift, uint32_t epoch, const uint32_t* __restrict__ aux_in,
+    uint32_t* __restrict__ aux_out) {
     constexpr int RADIX = 1 << BITS;
     constexpr int WAVES = THREADS / kWave;
     constexpr int TILE = THREADS * ITEMS;
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         rank[it] += s_wave_hist[wave][dig[it]];
     }
     __syncthreads();
-    uint32_t val[ITEMS];
+    uint32_t val[ITEMS], aux[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const size_t idx = wave_base + (size_t)it * kWave + lane;
@@ -208,6 +211,15 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     }
     __syncthreads();
 
+    if constexpr (AUX) {
+        // issued after the lookback: loads return in order, so an earlier
+        // gather would hold up the lookback's status loads
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const size_t idx = wave_base + (size_t)it * kWave + lane;
+            if (idx < n) aux[it] = aux_in[val[it]];
+        }
+    }
     // keys out in LDS order (runs of equal digits are contiguous stores)
     const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - tile_base);
 #pragma unroll
@@ -229,6 +241,20 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         const uint32_t j = (uint32_t)(it * THREADS + tid);
         if (j < tile_n) vout[s_delta[s_digit[j]] + (int32_t)j] = s_vals[j];
     }
+    if constexpr (AUX) {
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const size_t idx = wave_base + (size_t)it * kWave + lane;
+            if (idx < n) s_vals[rank[it]] = aux[it];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t j = (uint32_t)(it * THREADS + tid);
+            if (j < tile_n) aux_out[s_delta[s_digit[j]] + (int32_t)j] = s_vals[j];
+        }
+    }
 }
 
 }  // namespace
@@ -239,38 +265,44 @@ uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / k
 constexpr int kSort64Threads = 1024, kSort64Items = 16;
 constexpr int kSort32Threads = 1024, kSort32Items = 8;
 
-template <int BITS, typename KeyT, bool IOTA>
+template <int BITS, typename KeyT, bool IOTA, bool AUX>
 static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                         const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass,
                         uint64_t* status, uint32_t* ticket, uint32_t shift, uint32_t epoch,
-                        hipStream_t s) {
+                        const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s) {
     constexpr int kThreads = sizeof(KeyT) == 8 ? kSort64Threads : kSort32Threads;
     constexpr int kItems = sizeof(KeyT) == 8 ? kSort64Items : kSort32Items;
     const uint32_t grid = (n_max + kThreads * kItems - 1) / (kThreads * kItems);
-    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA, kThreads, kItems>), dim3(grid), dim3(kThreads), 0, s,
-                       static_cast<const KeyT*>(kin), vin, static_cast<KeyT*>(kout), vout, n_dev,
-                       hist_pass, status, ticket, n_max, shift, epoch);
+    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA, kThreads, kItems, AUX>), dim3(grid), dim3(kThreads),
+                       0, s, static_cast<const KeyT*>(kin), vin, static_cast<KeyT*>(kout), vout, n_dev,
+                       hist_pass, status, ticket, n_max, shift, epoch, aux_in, aux_out);
 }
 
 hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                              uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
-                             hipStream_t s) {
+                             const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s) {
     if (n_max == 0) return hipSuccess;
+    const bool aux = aux_in != nullptr && aux_out != nullptr;
+    // the gather rides on 32-bit key passes whose values are read from vin
+    if (aux && (key_bytes != 4 || iota_values)) return hipErrorInvalidValue;
     const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
-                    (iota_values ? 1 : 0);
-#define GS3D_PASS(B, K, I) \
-    launch_pass<B, K, I>(kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, s)
+                    (iota_values ? 1 : 0) + (aux ? 2 : 0);
+#define GS3D_PASS(B, K, I, A)                                                                      \
+    launch_pass<B, K, I, A>(kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, \
+                            aux_in, aux_out, s)
     switch (key) {
-        case 110: GS3D_PASS(8, uint64_t, false); break;
-        case 111: GS3D_PASS(8, uint64_t, true); break;
-        case 120: GS3D_PASS(8, uint32_t, false); break;
-        case 121: GS3D_PASS(8, uint32_t, true); break;
-        case 210: GS3D_PASS(4, uint64_t, false); break;
-        case 211: GS3D_PASS(4, uint64_t, true); break;
-        case 220: GS3D_PASS(4, uint32_t, false); break;
-        case 221: GS3D_PASS(4, uint32_t, true); break;
+        case 110: GS3D_PASS(8, uint64_t, false, false); break;
+        case 111: GS3D_PASS(8, uint64_t, true, false); break;
+        case 120: GS3D_PASS(8, uint32_t, false, false); break;
+        case 121: GS3D_PASS(8, uint32_t, true, false); break;
+        case 122: GS3D_PASS(8, uint32_t, false, true); break;
+        case 210: GS3D_PASS(4, uint64_t, false, false); break;
+        case 211: GS3D_PASS(4, uint64_t, true, false); break;
+        case 220: GS3D_PASS(4, uint32_t, false, false); break;
+        case 221: GS3D_PASS(4, uint32_t, true, false); break;
+        case 222: GS3D_PASS(4, uint32_t, false, true); break;
         default: return hipErrorInvalidValue;
     }
 #undef GS3D_PASS

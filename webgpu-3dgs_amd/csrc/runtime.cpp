@@ -240,10 +240,12 @@ int set_device(gs3d_ctx* c) {
 // Run LSD passes shifts[0..passes) over n = *n_dev pairs (grid sized for
 // n_max), ping-ponging a <-> b. hist: [passes][radix] digit counts of the
 // input; ticket: one counter per pass; first_iota: pass 0 takes the
-// positions as values. Sets *in_b when the result ends in the b buffers.
+// positions as values; aux_in/aux_out (or null): the last pass also writes
+// aux_out[p] = aux_in[value at p]. Sets *in_b when the result ends in b.
 int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint32_t key_bytes,
                bool first_iota, const uint32_t* n_dev, uint64_t n_max, const uint32_t* shifts,
-               uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b) {
+               uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b,
+               const uint32_t* aux_in = nullptr, uint32_t* aux_out = nullptr) {
     uint64_t* radix_status = nullptr;
     const uint32_t radix = 1u << c->digit_bits;
     GS3D_TRY(ensure_status(c, "radix_status",
@@ -253,7 +255,8 @@ int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va,
                                      key_bytes, first_iota && p == 0, n_dev, (uint32_t)n_max,
                                      hist + p * radix, radix_status, tickets + p, shifts[p],
-                                     c->digit_bits, next_epoch(c), c->stream));
+                                     c->digit_bits, next_epoch(c), p + 1 == passes ? aux_in : nullptr,
+                                     p + 1 == passes ? aux_out : nullptr, c->stream));
     }
     *in_b = (passes % 2) == 1;
     return GS3D_OK;
@@ -305,11 +308,12 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     const uint32_t radix = 1u << bits;
 
     gs3d_projected_splat* projected;
-    uint32_t *counts, *offsets, *counters, *tile_offsets, *va, *vb, *dka, *dkb, *ida, *idb;
+    uint32_t *counts, *counts_sorted, *offsets, *counters, *tile_offsets, *va, *vb, *dka, *dkb, *ida, *idb;
     uint64_t *scan_status, *ka, *kb;
     uint16_t* image;
     GS3D_TRY(ensure_t(c, "projected_splats", n, &projected));
     GS3D_TRY(ensure_t(c, "splat_ref_counts", n, &counts));
+    GS3D_TRY(ensure_t(c, "splat_ref_counts_depth_order", n, &counts_sorted));
     GS3D_TRY(ensure_t(c, "splat_ref_offsets_depth_order", (size_t)n + 1, &offsets));
     GS3D_TRY(ensure_t(c, "counters", kCtrTotalWords, &counters));
     GS3D_TRY(ensure_status(c, "scan_status", (size_t)n / kScanTile + 2, &scan_status));
@@ -362,15 +366,17 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     HIP_TRY(c, launch_key_histogram(dka, 4, counters + kCtrSplatCount, n, hist_depth, hist_partial,
                                     bits, depth_passes, 0, c->stream));
     bool order_in_b = false;
-    // pass 0 takes the positions as values (ida is not read); odd passes write ida
+    // pass 0 takes the positions as values (ida is not read); odd passes write
+    // ida; the last pass also lays the tile counts out in depth order
     GS3D_TRY(run_passes(c, dka, ida, dkb, idb, 4, true, counters + kCtrSplatCount, n, c->shifts,
-                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b));
+                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b, counts,
+                        counts_sorted));
     const uint32_t* order = order_in_b ? idb : ida;
     record(c, GS3D_STAGE_DEPTH_SORT, 1);
 
     record(c, GS3D_STAGE_SCAN, 0);
-    HIP_TRY(c, launch_scan(counts, order, offsets, ref_counter, scan_status, counters + kCtrScanTicket,
-                           n, next_epoch(c), c->stream));
+    HIP_TRY(c, launch_scan(counts_sorted, nullptr, offsets, ref_counter, scan_status,
+                           counters + kCtrScanTicket, n, next_epoch(c), c->stream));
     record(c, GS3D_STAGE_SCAN, 1);
 
     record(c, GS3D_STAGE_EMIT, 0);
