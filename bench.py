@@ -38,8 +38,9 @@ def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
     preprocess: vec4 inputs + SH coefficients in, 64-B projected splat + count
     + depth key out; depth sort: per pass a u32 key + u32 id read and written
     (the first pass reads keys only), plus one read of the keys to histogram;
-    scan: order + gathered count in, offset out; emit: offsets, order and 32 B
-    of the projected splat in, 12-B (key, id) out; key sort: 24 B per key per
+    scan: the sorted id and depth key and the 8-B tile rectangle of each
+    splat in, a 16-B record per splat out; emit: the records and ids in,
+    12-B (key, id) out; key sort: 24 B per key per
     pass; identify: one (T+1)-entry write plus the sorted tile words it
     searches; raster: the 48 B used of each listed splat, tile ranges, 8-B
     pixel out.
@@ -48,8 +49,8 @@ def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
     return {
         "preprocess-splats": n * (pre_in + 68 + 4),
         "radix-sort-splats": n * (4 + 12 + 16 * (depth_passes - 1)) if depth_passes else 0,
-        "prefix-scan": 12 * n + 4,
-        "emit-tile-refs": 44 * n + 12 * k,
+        "prefix-scan": 16 * n + 16 * (n + 1),
+        "emit-tile-refs": 16 * (n + 1) + 4 * n + 12 * k,
         "radix-sort": tile_passes * 24 * k,
         "identify-tile-ranges": 8 * k + 4 * (tiles + 1),
         "rasterize-tiles": 48 * k + 8 * (tiles + 1) + 8 * pixels,

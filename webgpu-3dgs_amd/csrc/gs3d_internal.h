@@ -55,6 +55,7 @@ struct PreprocessArgs {
     gs3d_projected_splat* out;
     uint32_t* counts;
     uint32_t* depth_keys;  // bits(-depth) of visible splats, 0xFFFFFFFF for culled; or null
+    uint2* rects;          // tile rectangle (x0 | y0 << 16, x1 | y1 << 16), zero for culled; or null
     uint32_t* counters;    // if set: counters[0..counter_words) = 0 except [count_slot] = n
     uint32_t counter_words;
     uint32_t count_slot;
@@ -67,7 +68,8 @@ struct PreprocessArgs {
 struct EmitArgs {
     const gs3d_projected_splat* projected;
     const uint32_t* order;    // position -> splat id (null: identity)
-    const uint32_t* offsets;  // n + 1 entries, per position
+    const uint32_t* offsets;  // n + 1 entries, per position (when recs is null)
+    const uint4* recs;        // n + 1 records of k_scan_rects, or null
     uint64_t* keys;
     uint32_t* values;
     uint32_t* hist;           // [passes][radix] digit histograms (accumulated), or null
@@ -84,6 +86,23 @@ struct EmitArgs {
 // which bounds the per-workgroup histogram rows to fold.
 constexpr uint32_t kEmitMaxBlocks = 1024;
 uint32_t emit_grid(uint32_t n);
+
+// Exclusive scan of the tile counts of the splats in depth order, counted from
+// their tile rectangles: recs[p] = (first key, first tile id, width in tiles,
+// depth key) for depth position p, recs[n].x = K = ref_counter[0].
+struct ScanRectsArgs {
+    const uint32_t* order;       // depth position -> splat id
+    const uint32_t* depth_keys;  // depth position -> depth key, 0xFFFFFFFF for a culled splat
+    const uint2* rects;          // per splat id: tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
+    uint4* recs;                 // n + 1 records
+    uint32_t* ref_counter;
+    uint64_t* status;            // lookback words, n / kScanTile + 2
+    uint32_t* ticket;
+    uint32_t n;
+    uint32_t tiles_x;
+    uint32_t epoch;
+};
+hipError_t launch_scan_rects(const ScanRectsArgs& a, hipStream_t s);
 
 // Lanes of the wavefront whose BITS-bit digit equals this lane's, among lanes
 // with `valid` set (wave-level match by one ballot per digit bit). Written on

@@ -7,7 +7,10 @@
 // owns 256 consecutive splats and therefore one contiguous output range
 // [offsets[i0], offsets[i0+256]); thread j of the range finds its splat by a
 // direct lookup in an LDS owner table (one byte store per key to build), so
-// every store of keys and ids is coalesced.
+// every store of keys and ids is coalesced. In the frame the splats come in
+// depth order with per-position records from k_scan_rects (first key, first
+// tile, width, depth key), read coalesced; with offsets instead, the tile
+// rectangle and depth are read from the projected splats.
 //
 // Fused here: the digit histograms of every radix pass that the sort will run,
 // so the sort does not re-read the keys to count them. Keys past `capacity`
@@ -27,31 +30,41 @@ constexpr int kEmitThreads = 256;
 constexpr uint32_t kOwnerWindow = 4096;
 
 __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint32_t* s_start,
-                                           uint32_t* s_tmin_x, uint32_t* s_tmin_y,
-                                           uint32_t* s_width, uint32_t* s_keylo, uint32_t* s_sid,
-                                           uint8_t* s_owner, uint32_t* s_hist) {
+                                           uint32_t* s_tile0, uint32_t* s_width, uint32_t* s_keylo,
+                                           uint32_t* s_sid, uint8_t* s_owner, uint32_t* s_hist) {
     const int tid = threadIdx.x;
     const uint32_t i = i0 + tid;
     const uint32_t last = min(i0 + kEmitThreads, a.n);
     const uint32_t radix = 1u << a.digit_bits;
-    const uint32_t block_start = a.offsets[i0];
-    const uint32_t block_end = a.offsets[last];
+    // first key of position p: the records' start column or the offsets
+    auto first_key = [&](uint32_t p) { return a.recs ? a.recs[p].x : a.offsets[p]; };
+    const uint32_t block_start = first_key(i0);
+    const uint32_t block_end = first_key(last);
     uint32_t my_start = 0, my_count = 0, sid = i;
     if (i < a.n) {
-        const uint32_t off = a.offsets[i];
-        my_count = a.offsets[i + 1] - off;  // > 0 exactly for valid splats
-        my_start = off - block_start;
         if (a.order) sid = a.order[i];
-        if (my_count) {
-            const float4* ps = reinterpret_cast<const float4*>(a.projected + sid);
-            const float4 f0 = ps[0];
-            const float4 f3 = ps[3];
-            const uint32_t t0x = __float_as_uint(f3.x), t0y = __float_as_uint(f3.y);
-            const uint32_t t1x = __float_as_uint(f3.z);
-            s_tmin_x[tid] = t0x;
-            s_tmin_y[tid] = t0y;
-            s_width[tid] = t1x - t0x + 1u;
-            s_keylo[tid] = __float_as_uint(-f0.z);
+        if (a.recs) {
+            // (first key, first tile, width, depth key) written by the scan
+            const uint4 rec = a.recs[i];
+            my_count = a.recs[i + 1].x - rec.x;
+            my_start = rec.x - block_start;
+            s_tile0[tid] = rec.y;
+            s_width[tid] = rec.z;
+            s_keylo[tid] = rec.w;
+        } else {
+            const uint32_t off = a.offsets[i];
+            my_count = a.offsets[i + 1] - off;  // > 0 exactly for valid splats
+            my_start = off - block_start;
+            if (my_count) {
+                const float4* ps = reinterpret_cast<const float4*>(a.projected + sid);
+                const float4 f0 = ps[0];
+                const float4 f3 = ps[3];
+                const uint32_t t0x = __float_as_uint(f3.x), t0y = __float_as_uint(f3.y);
+                const uint32_t t1x = __float_as_uint(f3.z);
+                s_tile0[tid] = t0y * a.tiles_x + t0x;
+                s_width[tid] = t1x - t0x + 1u;
+                s_keylo[tid] = __float_as_uint(-f0.z);
+            }
         }
     }
     s_start[tid] = my_start;
@@ -75,7 +88,7 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
                 const uint32_t w = s_width[o];
                 const uint32_t ry = r / w;
                 const uint32_t rx = r - ry * w;
-                const uint32_t tile = (s_tmin_y[o] + ry) * a.tiles_x + s_tmin_x[o] + rx;
+                const uint32_t tile = s_tile0[o] + ry * a.tiles_x + rx;
                 key = ((uint64_t)tile << 32) | (uint64_t)s_keylo[o];
                 const uint64_t pos = (uint64_t)block_start + w0 + jw;
                 if (pos < a.capacity) {
@@ -101,8 +114,7 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
 // (hist_partial[block][pass][digit]); k_hist_reduce folds the rows.
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
     __shared__ uint32_t s_start[kEmitThreads];
-    __shared__ uint32_t s_tmin_x[kEmitThreads];
-    __shared__ uint32_t s_tmin_y[kEmitThreads];
+    __shared__ uint32_t s_tile0[kEmitThreads];
     __shared__ uint32_t s_width[kEmitThreads];
     __shared__ uint32_t s_keylo[kEmitThreads];
     __shared__ uint32_t s_sid[kEmitThreads];
@@ -113,8 +125,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
     for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) s_hist[k] = 0u;
     const uint32_t chunks = (a.n + kEmitThreads - 1) / kEmitThreads;
     for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x)
-        emit_chunk(a, c * kEmitThreads, s_start, s_tmin_x, s_tmin_y, s_width, s_keylo, s_sid, s_owner,
-                   s_hist);
+        emit_chunk(a, c * kEmitThreads, s_start, s_tile0, s_width, s_keylo, s_sid, s_owner, s_hist);
     if (a.hist_partial) {
         __syncthreads();
         uint32_t* row = a.hist_partial + (size_t)blockIdx.x * hist_words;

@@ -207,6 +207,10 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     out[3] = o3;
     a.counts[i] = count;
     if (a.depth_keys) a.depth_keys[i] = count ? __float_as_uint(-o0.z) : 0xFFFFFFFFu;
+    // tile coordinates are < 2048 (viewport <= 16384 px, tiles >= 8 px)
+    if (a.rects)
+        a.rects[i] = make_uint2(__float_as_uint(o3.x) | (__float_as_uint(o3.y) << 16),
+                                __float_as_uint(o3.z) | (__float_as_uint(o3.w) << 16));
 }
 
 // f_dc (N x 3) + rest (N x 3R, channel-major per splat) -> coefficient-major SoA.

@@ -135,6 +135,117 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restric
     }
 }
 
+// The frame's scan: the counts are not read from memory but counted from the
+// tile rectangles of the splats in depth order (gathered through `order`, 8 B
+// per visible splat; culled splats have depth key 0xFFFFFFFF and count 0).
+// Besides the offsets it writes what emit needs per position, so emit reads
+// records in order instead of gathering splats: recs[p] = (first key, first
+// tile id, width in tiles, depth key); recs[n].x = total = ref_counter[0].
+__global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wave_sum[kScanThreads / kWave];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    const uint32_t n = a.n;
+    if (tid == 0) s_part = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const uint32_t part = s_part;
+    const uint32_t parts = n == 0 ? 1u : (n + kScanTile - 1) / kScanTile;
+    if (part >= parts) return;
+    const size_t base = (size_t)part * kScanTile + (size_t)tid * kScanItems;
+
+    uint32_t sid[kScanItems], dk[kScanItems];
+    if (base + kScanItems <= n) {
+        const uint4* o = reinterpret_cast<const uint4*>(a.order + base);
+        const uint4* d = reinterpret_cast<const uint4*>(a.depth_keys + base);
+        const uint4 o0 = o[0], o1 = o[1], d0 = d[0], d1 = d[1];
+        sid[0] = o0.x; sid[1] = o0.y; sid[2] = o0.z; sid[3] = o0.w;
+        sid[4] = o1.x; sid[5] = o1.y; sid[6] = o1.z; sid[7] = o1.w;
+        dk[0] = d0.x; dk[1] = d0.y; dk[2] = d0.z; dk[3] = d0.w;
+        dk[4] = d1.x; dk[5] = d1.y; dk[6] = d1.z; dk[7] = d1.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            sid[k] = base + k < n ? a.order[base + k] : 0u;
+            dk[k] = base + k < n ? a.depth_keys[base + k] : 0xFFFFFFFFu;
+        }
+    }
+    uint32_t v[kScanItems], t0[kScanItems], wd[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = 0u;
+        t0[k] = 0u;
+        wd[k] = 0u;
+        if (dk[k] != 0xFFFFFFFFu) {
+            const uint2 r = a.rects[sid[k]];
+            const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+            wd[k] = x1 - x0 + 1u;
+            v[k] = wd[k] * (y1 - y0 + 1u);
+            t0[k] = y0 * a.tiles_x + x0;
+        }
+    }
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) local += v[k];
+    const uint32_t incl = wave_inclusive_scan(local);
+    if (lane == kWave - 1) s_wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t wave_prefix = 0, tile_total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / kWave; w++) {
+        const uint32_t ws = s_wave_sum[w];
+        if (w < wave) wave_prefix += ws;
+        tile_total += ws;
+    }
+
+    if (wave == 0) {
+        // Publish, then look back 64 predecessors at a time.
+        if (lane == 0) {
+            store_status(&a.status[part],
+                         status_word(a.epoch, part == 0 ? kFlagPrefix : kFlagAggregate, tile_total));
+        }
+        uint32_t excl = 0;
+        int64_t p = (int64_t)part - 1;
+        while (p >= 0) {
+            const int64_t q = p - lane;
+            uint64_t w = 0;
+            uint32_t flag = 0;
+            if (q >= 0) {
+                do {
+                    w = load_status(&a.status[q]);
+                    flag = ((uint32_t)(w >> 34) == a.epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
+                } while (flag == 0);
+            }
+            // lanes with q < 0 contribute nothing and act as a prefix stop.
+            const uint64_t stop = __ballot(q < 0 || flag == kFlagPrefix);
+            const int first = stop ? __ffsll((unsigned long long)stop) - 1 : kWave;
+            uint32_t val = (q >= 0 && lane <= first) ? (uint32_t)w : 0u;
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1) val += __shfl_xor(val, d, kWave);
+            excl += val;
+            if (stop) break;
+            p -= kWave;
+        }
+        if (lane == 0) {
+            if (part != 0) store_status(&a.status[part], status_word(a.epoch, kFlagPrefix, excl + tile_total));
+            s_prefix = excl;
+            if (part == parts - 1) {
+                a.recs[n] = make_uint4(excl + tile_total, 0u, 0u, 0u);
+                a.ref_counter[0] = excl + tile_total;
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wave_prefix + (incl - local);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) a.recs[base + k] = make_uint4(run, t0[k], wd[k], dk[k]);
+        run += v[k];
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t value, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -157,6 +268,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
     size_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_rects(const ScanRectsArgs& a, hipStream_t s) {
+    const uint32_t parts = a.n == 0 ? 1u : (a.n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_rects, dim3(parts), dim3(kScanThreads), 0, s, a);
     return hipGetLastError();
 }
 

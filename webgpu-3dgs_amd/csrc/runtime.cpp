@@ -282,8 +282,10 @@ int enqueue_id_order_scan(gs3d_ctx* c, uint32_t** out) {
 // stream, no host round trip:
 //   preprocess -> depth keys of the N splats
 //   LSD over the N depth keys (digits of bits 0..31), values = splat ids
-//   scan of the tile counts in depth order -> K, write offsets
-//   emit (tile, depth) keys in depth order (+ tile-digit histograms)
+//   scan of the tile counts in depth order, counted from the tile rectangles
+//   (-> K, one record per depth position)
+//   emit (tile, depth) keys in depth order from the records (+ tile-digit
+//   histograms)
 //   LSD over the K keys, tile digits only (bits 32..)
 //   identify tile ranges, rasterize
 // Sorting the 32 depth bits on the splats instead of on the keys is
@@ -308,15 +310,17 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     const uint32_t radix = 1u << bits;
 
     gs3d_projected_splat* projected;
-    uint32_t *counts, *counts_sorted, *offsets, *counters, *tile_offsets, *va, *vb, *dka, *dkb, *ida, *idb;
+    uint32_t *counts, *counters, *tile_offsets, *va, *vb, *dka, *dkb, *ida, *idb;
     uint64_t *scan_status, *ka, *kb;
+    uint4* recs;
     uint16_t* image;
     GS3D_TRY(ensure_t(c, "projected_splats", n, &projected));
     GS3D_TRY(ensure_t(c, "splat_ref_counts", n, &counts));
-    GS3D_TRY(ensure_t(c, "splat_ref_counts_depth_order", n, &counts_sorted));
-    GS3D_TRY(ensure_t(c, "splat_ref_offsets_depth_order", (size_t)n + 1, &offsets));
+    uint2* rects;
+    GS3D_TRY(ensure_t(c, "splat_tile_rects", n, &rects));
     GS3D_TRY(ensure_t(c, "counters", kCtrTotalWords, &counters));
     GS3D_TRY(ensure_status(c, "scan_status", (size_t)n / kScanTile + 2, &scan_status));
+    GS3D_TRY(ensure_t(c, "splat_emit_records", (size_t)n + 1, &recs));
     GS3D_TRY(ensure_t(c, "depth_keys_a", n, &dka));
     GS3D_TRY(ensure_t(c, "depth_keys_b", n, &dkb));
     GS3D_TRY(ensure_t(c, "splat_order_a", n, &ida));
@@ -352,6 +356,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     pa.out = projected;
     pa.counts = counts;
     pa.depth_keys = dka;
+    pa.rects = rects;
     pa.n = n;
     pa.sh_degree = c->sh_degree;
     pa.tiles_x = c->tiles_x;
@@ -366,24 +371,35 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     HIP_TRY(c, launch_key_histogram(dka, 4, counters + kCtrSplatCount, n, hist_depth, hist_partial,
                                     bits, depth_passes, 0, c->stream));
     bool order_in_b = false;
-    // pass 0 takes the positions as values (ida is not read); odd passes write
-    // ida; the last pass also lays the tile counts out in depth order
+    // pass 0 takes the positions as values (ida is not read); odd passes write ida
     GS3D_TRY(run_passes(c, dka, ida, dkb, idb, 4, true, counters + kCtrSplatCount, n, c->shifts,
-                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b, counts,
-                        counts_sorted));
+                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b));
     const uint32_t* order = order_in_b ? idb : ida;
+    const uint32_t* depth_sorted = order_in_b ? dkb : dka;
     record(c, GS3D_STAGE_DEPTH_SORT, 1);
 
+    // scan of the tile counts in depth order, counted from the tile
+    // rectangles; it leaves one record per depth position for emit
     record(c, GS3D_STAGE_SCAN, 0);
-    HIP_TRY(c, launch_scan(counts_sorted, nullptr, offsets, ref_counter, scan_status,
-                           counters + kCtrScanTicket, n, next_epoch(c), c->stream));
+    ScanRectsArgs sa{};
+    sa.order = order;
+    sa.depth_keys = depth_sorted;
+    sa.rects = rects;
+    sa.recs = recs;
+    sa.ref_counter = ref_counter;
+    sa.status = scan_status;
+    sa.ticket = counters + kCtrScanTicket;
+    sa.n = n;
+    sa.tiles_x = c->tiles_x;
+    sa.epoch = next_epoch(c);
+    HIP_TRY(c, launch_scan_rects(sa, c->stream));
     record(c, GS3D_STAGE_SCAN, 1);
 
     record(c, GS3D_STAGE_EMIT, 0);
     EmitArgs ea{};
     ea.projected = projected;
     ea.order = order;
-    ea.offsets = offsets;
+    ea.recs = recs;
     ea.keys = ka;
     ea.values = va;
     ea.hist = tile_passes ? hist_tile : nullptr;
@@ -424,14 +440,16 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
         uint32_t* uv;
         GS3D_TRY(ensure_t(c, "unsorted_keys", cap, &uk));
         GS3D_TRY(ensure_t(c, "unsorted_values", cap, &uv));
-        EmitArgs eu = ea;
+        EmitArgs eu{};
+        eu.projected = projected;
         eu.order = nullptr;
         eu.offsets = id_offsets;
         eu.keys = uk;
         eu.values = uv;
-        eu.hist = nullptr;
-        eu.hist_partial = nullptr;
-        eu.n_passes = 0;
+        eu.n = n;
+        eu.tiles_x = c->tiles_x;
+        eu.capacity = cap;
+        eu.digit_bits = bits;
         HIP_TRY(c, launch_emit(eu, c->stream));
     }
 
@@ -747,7 +765,7 @@ int gs3d_stage_times_us(gs3d_ctx* c, float* out, int n) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++) {
         float ms = 0.f;
-        if (c->profile_mask >> s & 1u)
+        if (c->ring_mask[c->cur_slot] >> s & 1u)
             HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[c->cur_slot][s][0], c->ring[c->cur_slot][s][1]));
         out[s] = ms * 1000.0f;
     }
