@@ -6,7 +6,10 @@
 //                     k = 0 (f_dc) .. (deg+1)^2-1; one coalesced load per
 //                     (thread, coefficient)
 //   projected_splats  N x 64 B gs3d_projected_splat
-//   splat_ref_counts  N x u32;  splat_ref_offsets (N+1) x u32
+//   splat_ref_counts  N x u32 (id order)
+//   splat_tile_rects  N x uint2 tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
+//   depth keys/order  N x u32 each, a and b (the depth presort)
+//   splat_emit_records (N+1) x uint4 (first key, first tile, width, depth key)
 //   sort_keys_a/b     cap x u64 (tile << 32 | bits(-depth)); values cap x u32
 //   tile_offsets      (T+1) x u32
 //   splat_target      W*H x 4 x f16 (premultiplied colour, 1 - T)
