@@ -98,6 +98,9 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--composite", type=int, default=0,
+                    help="after the timed region, run this many composites per display mode "
+                         "(plain, tile heat map) and report their wall time per call")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path runs on a single MI355X "
@@ -213,6 +216,24 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if args.composite > 0:
+        # The display step (not part of frames/s): premultiplied over onto the
+        # resident unorm8 framebuffer, without and with the tile heat map. Wall
+        # time per call includes the host side of each call; the kernel time is
+        # in the rocprof kernel trace of this same command.
+        disp = {}
+        for mode, label in ((0, "composite"), (1, "composite_heat_map")):
+            r.set_debug_mode(mode, 256)
+            r.composite(W, H, clear=None)  # allocate / warm
+            r.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.composite):
+                r.composite_resident()
+            r.sync()
+            disp[label] = {"wall_us_per_call": round((time.perf_counter() - t0) * 1e6 / args.composite, 2),
+                           "algo_bytes": (24 if mode else 16) * W * H}
+        r.set_debug_mode(0, 256)
+        out["display"] = disp
     print(json.dumps(out), flush=True)
     r.close()
     return 0

@@ -86,7 +86,17 @@ enum gs3d_buffer_id {
     GS3D_BUF_IMAGE_RGBA32F     = 8,  /* same image widened to f32 on readback       */
     GS3D_BUF_UNSORTED_KEYS     = 9,  /* K x u64 emit order (before the sort)        */
     GS3D_BUF_UNSORTED_VALUES   = 10, /* K x u32 emit order                          */
-    GS3D_BUF__COUNT            = 11
+    GS3D_BUF_DEBUG_TARGET_RGBA16F = 11, /* W*H x 4 x f16: target + heat map (last
+                                           composite in debug mode 1)              */
+    GS3D_BUF_FRAMEBUFFER       = 12, /* W*H x 4 unorm8 of the last gs3d_composite  */
+    GS3D_BUF__COUNT            = 13
+};
+
+/* Framebuffer formats of gs3d_composite (the swapchain format the reference
+ * gets from navigator.gpu.getPreferredCanvasFormat, DeviceManager.ts:15). */
+enum gs3d_format {
+    GS3D_FORMAT_RGBA8UNORM = 0,
+    GS3D_FORMAT_BGRA8UNORM = 1
 };
 
 /* rebin flags */
@@ -217,6 +227,30 @@ int gs3d_buffer_destroy(gs3d_ctx* ctx, const char* name);
  * `count` gets the total number (Profiler.getBuffersSortedBySize). */
 int gs3d_buffer_list(gs3d_ctx* ctx, char* names, uint64_t* sizes, size_t n_max, size_t* count);
 
+/* ---- display: debug heat map and composite ---------------------------------- */
+
+/* setDebugMode (GaussianSplatRenderer.ts:319-323): 0 = normal, 1 = the
+ * splats-per-tile heat map (debug_tile_overlay.wgsl:35-58) over the target.
+ * debug_ref is the splat count drawn full red (the reference's debugRef, 256,
+ * :75). Any mode other than 0 or 1 is GS3D_E_INVALID_ARG. */
+int gs3d_set_debug_mode(gs3d_ctx* ctx, uint32_t mode, uint32_t debug_ref);
+int gs3d_get_debug_mode(const gs3d_ctx* ctx, uint32_t* mode, uint32_t* debug_ref);
+
+/* The composite pass of render() (GaussianSplatRenderer.ts:1075-1097): the
+ * last rebin's premultiplied target, through the heat map in debug mode 1,
+ * blended "over" (one, one-minus-src-alpha on colour and alpha, :939-945)
+ * onto a W*H unorm8 framebuffer in `format` byte order (composite_splats.wgsl:12-29).
+ * The framebuffer is the context's resident buffer "framebuffer":
+ *   clear_rgba non-null: it is first cleared to that colour (the viewport's
+ *     clear pass, Viewport.ts:146-154; the reference clears to 0.12, 0.12, 0.13, 1);
+ *   else, framebuffer non-null: the caller's W*H*4 bytes are uploaded first;
+ *   else the resident contents are blended onto (undefined after the
+ *     viewport size changed).
+ * With `framebuffer` non-null the result is copied back into it (`bytes` must
+ * hold W*H*4). Unorm stores round to nearest: floor(clamp(v, 0, 1)*255 + 0.5). */
+int gs3d_composite(gs3d_ctx* ctx, int format, const float* clear_rgba,
+                   uint8_t* framebuffer, size_t bytes);
+
 /* ---- stage entry points (each WGSL pass fed caller data) ---------------- */
 
 /* preprocess_splats.wgsl:75-184 over caller splats. projected: N x 64 B,
@@ -258,6 +292,17 @@ int gs3d_stage_rasterize(gs3d_ctx* ctx, const gs3d_projected_splat* projected,
                          uint32_t n_splats, const uint32_t* sorted_values, uint32_t n_refs,
                          const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                          uint32_t width, uint32_t height, float* image);
+
+/* The heat map and composite over caller data: target_f16 W*H*4 binary16
+ * halves (premultiplied), tile_offsets (T+1) u32 for tile_x x tile_y tiles
+ * (read only when debug is 1), framebuffer W*H*4 unorm8 read and written (its
+ * input ignored when clear_rgba is set). debug_out (nullable) receives the
+ * W*H*4 f16 heat-map target when debug is 1. */
+int gs3d_stage_composite(gs3d_ctx* ctx, const uint16_t* target_f16,
+                         const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
+                         uint32_t width, uint32_t height, uint32_t debug, uint32_t debug_ref,
+                         int format, const float* clear_rgba, uint8_t* framebuffer,
+                         uint16_t* debug_out);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,9 @@
  *   identify     identify_tile_ranges.wgsl:30-52 (trailing tiles stay 0)
  *   rasterize    rasterize_tiles.wgsl:43-110 (accumulate, then stop if T<1e-4)
  *   fp16 store   rgba16float target (GaussianSplatRenderer.ts:233-238), RNE
+ *   heat map     debug_tile_overlay.wgsl:35-58 (tint over the target, f16 store)
+ *   composite    composite_splats.wgsl:25-29 with the premultiplied "over"
+ *                blend state of GaussianSplatRenderer.ts:939-945, unorm8 store
  *
  * The body in gs3d_oracle_real.inc is compiled twice: REAL=float (the fp32
  * mirror: operation order of the WGSL, no FMA contraction — built with
@@ -31,7 +34,11 @@
  * (centred isotropic splat, culling cases, radius floor, tile clamp), scipy
  * Rotation for the quaternion matrix, eigenvalues of cov3d; SH basis vs
  * scipy.special.sph_harm_y; raster vs single-splat and occlusion KATs and a
- * brute-force per-pixel composite; fp16 vs numpy float16.
+ * brute-force per-pixel composite; fp16 vs numpy float16; heat map vs its
+ * colour-ramp end points (green, yellow, red), the underflow guard and the
+ * clamp of debug_ref; composite vs the "over" identities (transparent source
+ * keeps the framebuffer, opaque source replaces it, associativity) and a
+ * float64 blend rounded to unorm8.
  */
 #include <math.h>
 #include <stdint.h>
@@ -270,5 +277,80 @@ void orc_rasterize_f32(const gs3d_projected_splat* ps, const uint32_t* values,
     if (quantize_f16) {
         size_t m = (size_t)W * H * 4;
         for (size_t i = 0; i < m; i++) image[i] = orc_f16_to_f32(orc_f32_to_f16(image[i]));
+    }
+}
+
+/* ---- debug heat map (debug_tile_overlay.wgsl:35-58) ---------------------- */
+/* base: W*H*4 premultiplied target (f16 values widened to f32); out: the
+ * overlay, rounded through binary16 (the rgba16float debug target). The pixel
+ * centre's tile is min(px / tile, tiles - 1) (:36-38); the count keeps the
+ * underflow guard of :42-44; opacity 0.4 (:52). */
+static float orc_clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+void orc_tile_overlay_f32(const float* base, const uint32_t* tile_offsets, uint32_t tile_x,
+                          uint32_t tile_y, uint32_t W, uint32_t H, uint32_t debug_ref, float* out) {
+    const uint32_t tiles_x = (W + tile_x - 1) / tile_x, tiles_y = (H + tile_y - 1) / tile_y;
+    const float ref = (float)debug_ref > 1.0f ? (float)debug_ref : 1.0f;
+    const float ao = 0.4f, keep = 1.0f - ao;
+    for (uint32_t y = 0; y < H; y++) {
+        for (uint32_t x = 0; x < W; x++) {
+            uint32_t tx = x / tile_x, ty = y / tile_y;
+            if (tx > tiles_x - 1) tx = tiles_x - 1;
+            if (ty > tiles_y - 1) ty = tiles_y - 1;
+            const uint32_t tile = ty * tiles_x + tx;
+            const uint32_t lo = tile_offsets[tile], hi = tile_offsets[tile + 1];
+            const uint32_t count = hi >= lo ? hi - lo : 0u;
+            const float t = orc_clamp01((float)count / ref);
+            const float tint[3] = {orc_clamp01(t * 2.0f), orc_clamp01(2.0f - t * 2.0f), 0.0f};
+            const float* b = base + ((size_t)y * W + x) * 4;
+            float* o = out + ((size_t)y * W + x) * 4;
+            for (int ch = 0; ch < 3; ch++) {
+                const float a = tint[ch] * ao, m = b[ch] * keep;
+                o[ch] = orc_f16_to_f32(orc_f32_to_f16(a + m));
+            }
+            const float m = b[3] * keep;
+            o[3] = orc_f16_to_f32(orc_f32_to_f16(ao + m));
+        }
+    }
+}
+
+/* ---- composite (composite_splats.wgsl:25-29, blend GaussianSplatRenderer.ts:939-945) */
+/* fb: W*H*4 unorm8 bytes, R,G,B,A order (B,G,R,A when bgra), read unless
+ * clear_rgba is set (then every pixel starts as the clear colour stored to
+ * unorm8, the clear pass), and written. Per channel:
+ * out = src + dst * (1 - src.a), then clamp to [0, 1], *255, round to nearest. */
+static uint8_t orc_unorm8(float v) {
+    v = v > 0.0f ? (v < 1.0f ? v : 1.0f) : 0.0f;
+    const float s = v * 255.0f;
+    return (uint8_t)(uint32_t)(s + 0.5f);
+}
+
+void orc_composite_f32(const float* src, uint32_t W, uint32_t H, int bgra, const float* clear_rgba,
+                       uint8_t* fb) {
+    const int ri = bgra ? 2 : 0, bi = bgra ? 0 : 2;
+    uint8_t clear[4] = {0, 0, 0, 0};
+    if (clear_rgba) {
+        clear[ri] = orc_unorm8(clear_rgba[0]);
+        clear[1] = orc_unorm8(clear_rgba[1]);
+        clear[bi] = orc_unorm8(clear_rgba[2]);
+        clear[3] = orc_unorm8(clear_rgba[3]);
+    }
+    const size_t px = (size_t)W * H;
+    for (size_t p = 0; p < px; p++) {
+        uint8_t* d = fb + p * 4;
+        const uint8_t* in = clear_rgba ? clear : d;
+        const float dst[4] = {(float)in[ri] / 255.0f, (float)in[1] / 255.0f, (float)in[bi] / 255.0f,
+                              (float)in[3] / 255.0f};
+        const float* s4 = src + p * 4;
+        const float inv_a = 1.0f - s4[3];
+        uint8_t o[4];
+        for (int ch = 0; ch < 4; ch++) {
+            const float m = dst[ch] * inv_a;
+            o[ch] = orc_unorm8(s4[ch] + m);
+        }
+        d[ri] = o[0];
+        d[1] = o[1];
+        d[bi] = o[2];
+        d[3] = o[3];
     }
 }

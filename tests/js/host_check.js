@@ -96,10 +96,16 @@ async function render(req) {
     const image = r.readImage(false);
     const counts = await r.readTileCounts();
     const variant = r.addon.getKernelVariant(r.ctx, 'rasterize-tiles', 'PIXELS_PER_LANE');
+    // display step: heat map + composite onto a cleared framebuffer (the camera
+    // is unchanged, so this render only composites)
+    r.setDebugMode(1);
+    const framebuffer = new Uint8Array(req.width * req.height * 4);
+    const composited = r.render({ ...frame, colorTexture: framebuffer, clearValue: { r: 0.12, g: 0.12, b: 0.13, a: 1 } });
     return {
       ok: true, decisions, refCount: r.refCount(), rebinCount: r.rebinCount,
       image: b64(image), tileCounts: b64(counts), tileCount: r.getTileCount(), tileSize: r.getTileSize(),
       uniform: b64(cam.getUniformData(req.width, req.height)), variant,
+      framebuffer: b64(framebuffer), compositeRebinned: composited, debugRef: r.debugRef,
     };
   } finally {
     r.destroy();

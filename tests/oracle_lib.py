@@ -29,6 +29,8 @@ _PROTOS = {
     "orc_sh_basis_f64": (None, [F64, F64, F64, I32, P]),
     "orc_render_f32": (C.c_int64, [P, U32, P, P, P, P, P, P, I32, U32, U32, U32, U32, P, P, P]),
     "orc_render_f64": (C.c_int64, [P, U32, P, P, P, P, P, P, I32, U32, U32, U32, U32, P, P, P]),
+    "orc_tile_overlay_f32": (None, [P, P, U32, U32, U32, U32, U32, P]),
+    "orc_composite_f32": (None, [P, U32, U32, I32, P, P]),
 }
 
 _lib = None
@@ -159,3 +161,22 @@ def pipeline_f32(cam, s, width, height, tile=16):
     return dict(projected=proj, counts=counts, offsets=offsets, k=k, keys=keys, values=vals,
                 sorted_keys=skeys, sorted_values=svals, tile_offsets=toff, image=img,
                 tiles=(tiles_x, tiles_y), visited=visited, blended=blended)
+
+
+def tile_overlay(base, tile_offsets, tile, width, height, debug_ref=256):
+    """Heat-map target (H x W x 4 f32, f16-representable) over `base` (same shape)."""
+    base = np.ascontiguousarray(base, np.float32)
+    tile_offsets = np.ascontiguousarray(tile_offsets, np.uint32)
+    out = np.zeros((height, width, 4), np.float32)
+    lib().orc_tile_overlay_f32(_p(base), _p(tile_offsets), tile, tile, width, height, debug_ref, _p(out))
+    return out
+
+
+def composite(src, width, height, clear=None, framebuffer=None, bgra=False):
+    """Premultiplied "over" of src (H x W x 4 f32) onto an unorm8 framebuffer."""
+    src = np.ascontiguousarray(src, np.float32)
+    fb = (np.zeros((height, width, 4), np.uint8) if framebuffer is None
+          else np.ascontiguousarray(framebuffer, np.uint8).copy())
+    cl = None if clear is None else np.ascontiguousarray(clear, np.float32)
+    lib().orc_composite_f32(_p(src), width, height, 1 if bgra else 0, _p(cl), _p(fb))
+    return fb

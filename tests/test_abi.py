@@ -38,6 +38,9 @@ def test_null_arguments_rejected_without_device():
     assert lib.gs3d_destroy(None) == -1
     assert lib.gs3d_sync(None) == -1
     assert lib.gs3d_rebin(None, None, 1, 1, 0) == -1
+    assert lib.gs3d_set_debug_mode(None, 1, 256) == -1
+    assert lib.gs3d_composite(None, 0, None, None, 0) == -1
+    assert lib.gs3d_stage_composite(None, None, None, 16, 16, 1, 1, 0, 256, 0, None, None, None) == -1
     assert lib.gs3d_last_error(None) == b"null context"
 
 

@@ -305,3 +305,66 @@ def test_kernel_variant_errors(renderer):
         renderer.set_kernel_variant("nope", "X", 1)
     renderer.set_tile_size(16, 16)
     assert renderer.get_kernel_variant("rasterize-tiles", "TILE_X") == 16
+
+
+# ------------------------------------------------- heat map + composite ----
+def random_target_f16(rng, H, W):
+    a = rng.uniform(0, 1, (H, W, 1))
+    rgb = rng.uniform(0, 1, (H, W, 3)) * a
+    return np.concatenate([rgb, a], axis=2).astype(np.float16)
+
+
+@pytest.mark.parametrize("debug", [0, 1])
+@pytest.mark.parametrize("fmt", [L.FORMAT_RGBA8UNORM, L.FORMAT_BGRA8UNORM])
+@pytest.mark.parametrize("clear", [None, (0.12, 0.12, 0.13, 1.0)])
+@pytest.mark.parametrize("W,H,tile", [(333, 77, 16), (1030, 9, 8)])
+def test_composite_stage_bit_exact(renderer, debug, fmt, clear, W, H, tile):
+    say(f"composite stage {W}x{H} tile={tile} debug={debug} fmt={fmt} clear={clear is not None}")
+    rng = np.random.default_rng(W + H + debug)
+    tgt = random_target_f16(rng, H, W)
+    T = (-(-W // tile)) * (-(-H // tile))
+    to = np.concatenate([[0], np.cumsum(rng.integers(0, 400, T))]).astype(np.uint32)
+    to[-4:-1] = 0
+    to[3] = to[4] + 5  # one tile whose end is below its start (the underflow guard)
+    fb = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    got, dbg = renderer.stage_composite(tgt.view(np.uint16), to if debug else None, tile, W, H,
+                                        debug, 200, clear, fb, fmt)
+    src = tgt.astype(np.float32)
+    if debug:
+        src = O.tile_overlay(src, to, tile, W, H, 200)
+        assert np.array_equal(dbg, src.astype(np.float16).view(np.uint16))
+    bgra = fmt == L.FORMAT_BGRA8UNORM
+    want = O.composite(src, W, H, clear=clear, framebuffer=fb, bgra=bgra)
+    assert np.array_equal(got, want)
+
+
+def test_composite_after_rebin(renderer):
+    """render()'s display step on a rebinned frame: heat map + composite vs the oracle on the
+    frame's own target and tile ranges, then a plain composite onto a caller framebuffer."""
+    say("composite after rebin (uniform-100k-sh0-720p)")
+    wl = sc.workload(0)
+    s = sc.activate(wl.raw)
+    (eye, W, H) = wl.views[0]
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    renderer.rebin(cam_mod.view_camera(eye, W, H), W, H)
+    tgt = renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16).reshape(H, W, 4)
+    to = renderer.read(L.BUF_TILE_OFFSETS, np.uint32)
+    src = tgt.view(np.float16).astype(np.float32)
+    try:
+        renderer.set_debug_mode(1, 64)
+        assert renderer.get_debug_mode() == (1, 64)
+        got = renderer.composite(W, H, clear=(0.12, 0.12, 0.13, 1.0))
+        heat = O.tile_overlay(src, to, 16, W, H, 64)
+        dbg = renderer.read(L.BUF_DEBUG_TARGET_RGBA16F, np.uint16).reshape(H, W, 4)
+        assert np.array_equal(dbg, heat.astype(np.float16).view(np.uint16))
+        assert np.array_equal(got, O.composite(heat, W, H, clear=(0.12, 0.12, 0.13, 1.0)))
+        assert np.array_equal(renderer.read(L.BUF_FRAMEBUFFER, np.uint8).reshape(H, W, 4), got)
+    finally:
+        renderer.set_debug_mode(0, 256)
+    fb = np.random.default_rng(9).integers(0, 256, (H, W, 4), dtype=np.uint8)
+    got = renderer.composite(W, H, clear=None, framebuffer=fb, fmt=L.FORMAT_BGRA8UNORM)
+    assert np.array_equal(got, O.composite(src, W, H, framebuffer=fb, bgra=True))
+    # resident framebuffer: blending the same frame onto the previous result again
+    again = renderer.composite(W, H, clear=None)
+    assert np.array_equal(again, O.composite(src, W, H, framebuffer=got, bgra=False))

@@ -370,7 +370,8 @@ def test_addon_exports(tmp_path):
             "getTileSize", "getTileCount", "rebin", "refCount", "read", "readTileCounts", "stageTimes",
             "stageTimesAvg", "profilerReset", "stageLabel", "bufferCreate", "bufferResize", "bufferSize",
             "bufferWrite", "bufferRead", "bufferDestroy", "bufferList", "setKernelVariant",
-            "getKernelVariant", "setProfileMask", "sortPasses", "abiVersion"}
+            "getKernelVariant", "setProfileMask", "sortPasses", "setDebugMode", "composite",
+            "abiVersion"}
     assert set(r["names"]) == want
     assert r["abiVersion"] >= 1
 
@@ -402,3 +403,8 @@ def test_node_render_matches_oracle(tmp_path, tile, ppl):
     assert err <= 1.0 / 255.0, err
     counts = np.frombuffer(base64.b64decode(r["tileCounts"]), np.uint32)
     np.testing.assert_array_equal(counts, np.diff(ref["tile_offsets"].astype(np.int64)).clip(0))
+    # render() with a colour target: heat map (debug mode 1) + composite over the cleared target
+    assert r["compositeRebinned"] is False
+    fb = np.frombuffer(base64.b64decode(r["framebuffer"]), np.uint8).reshape(H, W, 4)
+    heat = O.tile_overlay(img, ref["tile_offsets"], tile, W, H, r["debugRef"])
+    np.testing.assert_array_equal(fb, O.composite(heat, W, H, clear=(0.12, 0.12, 0.13, 1.0)))

@@ -452,3 +452,110 @@ def test_render_f32_close_to_f64():
     i64, k64, _, _ = O.render(cu, s, W, H, 16, "f64")
     assert abs(k32 - k64) <= max(4, k64 // 200)
     assert np.abs(i32 - i64).max() < 1.0 / 255.0
+
+
+# ------------------------------------------------- heat map + composite ----
+def f16_image(rng, H, W):
+    """Random premultiplied f16-representable target: alpha in [0, 1], rgb <= alpha."""
+    a = rng.uniform(0, 1, (H, W, 1))
+    rgb = rng.uniform(0, 1, (H, W, 3)) * a
+    img = np.concatenate([rgb, a], axis=2).astype(np.float16).astype(np.float32)
+    img[0, 0] = 0.0          # fully transparent
+    img[0, 1] = [0.25, 0.5, 0.75, 1.0]  # opaque
+    return img
+
+
+def test_composite_transparent_keeps_and_opaque_replaces():
+    rng = np.random.default_rng(3)
+    H, W = 16, 64
+    fb = np.arange(H * W * 4, dtype=np.uint32).reshape(H, W, 4).astype(np.uint8)  # every byte value
+    out = O.composite(np.zeros((H, W, 4), np.float32), W, H, framebuffer=fb)
+    assert np.array_equal(out, fb)  # src alpha 0 leaves every stored byte as it was
+    src = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    src[..., 3] = 1.0
+    out = O.composite(src, W, H, framebuffer=fb)
+    assert np.array_equal(out, np.floor(src.astype(np.float64) * 255 + 0.5).astype(np.uint8))
+
+
+def test_composite_clear_colour_and_over_formula():
+    H, W = 8, 8
+    out = O.composite(np.zeros((H, W, 4), np.float32), W, H, clear=(0.12, 0.12, 0.13, 1.0))
+    assert np.all(out == np.array([31, 31, 33, 255], np.uint8))  # round(0.12*255)=31, round(0.13*255)=33
+    rng = np.random.default_rng(4)
+    src = f16_image(rng, 40, 50)
+    fb = rng.integers(0, 256, (40, 50, 4), dtype=np.uint8)
+    out = O.composite(src, 50, 40, framebuffer=fb)
+    s64 = src.astype(np.float64)
+    want = np.clip(s64 + fb / 255.0 * (1.0 - s64[..., 3:4]), 0, 1) * 255
+    diff = np.abs(out.astype(np.int32) - np.floor(want + 0.5).astype(np.int32))
+    assert diff.max() <= 1 and (diff == 0).mean() > 0.99
+    # a blend weighted by dst alpha or by (1 - src colour) would miss by far more
+    wrong = np.floor(np.clip(s64 + fb / 255.0 * (1.0 - fb[..., 3:4] / 255.0), 0, 1) * 255 + 0.5)
+    assert np.abs(out - wrong).max() > 10
+
+
+def test_composite_bgra_swaps_red_and_blue():
+    rng = np.random.default_rng(5)
+    src = f16_image(rng, 12, 20)
+    fb = rng.integers(0, 256, (12, 20, 4), dtype=np.uint8)
+    rgba = O.composite(src, 20, 12, framebuffer=fb)
+    bgra = O.composite(src, 20, 12, framebuffer=fb[..., [2, 1, 0, 3]], bgra=True)
+    assert np.array_equal(bgra, rgba[..., [2, 1, 0, 3]])
+    c = O.composite(np.zeros((2, 2, 4), np.float32), 2, 2, clear=(1.0, 0.5, 0.0, 1.0), bgra=True)
+    assert np.all(c == np.array([0, 128, 255, 255], np.uint8))
+
+
+def overlay_f64(base, tile_offsets, tile, W, H, ref):
+    tx = np.minimum(np.arange(W) // tile, -(-W // tile) - 1)
+    ty = np.minimum(np.arange(H) // tile, -(-H // tile) - 1)
+    tid = ty[:, None] * (-(-W // tile)) + tx[None, :]
+    lo, hi = tile_offsets[tid].astype(np.int64), tile_offsets[tid + 1].astype(np.int64)
+    count = np.where(hi >= lo, hi - lo, 0)
+    t = np.clip(count / max(1.0, float(ref)), 0, 1)
+    tint = np.stack([np.clip(2 * t, 0, 1), np.clip(2 - 2 * t, 0, 1), np.zeros_like(t)], axis=2)
+    out = np.empty_like(base, dtype=np.float64)
+    out[..., :3] = tint * 0.4 + base[..., :3] * 0.6
+    out[..., 3] = 0.4 + base[..., 3] * 0.6
+    return out
+
+
+def test_overlay_colour_ramp_end_points():
+    """count 0 -> green, count = ref/2 -> yellow, count >= ref -> red, all at opacity 0.4."""
+    W, H, tile, ref = 64, 16, 16, 256
+    tiles = 4
+    to = np.zeros(tiles + 1, np.uint32)
+    counts = [0, 128, 256, 1000]
+    to[1:] = np.cumsum(counts)
+    out = O.tile_overlay(np.zeros((H, W, 4), np.float32), to, tile, W, H, ref)
+    f16 = lambda v: float(np.float16(v))
+    want = [(0, f16(0.4), 0), (f16(0.4), f16(0.4), 0), (f16(0.4), 0, 0), (f16(0.4), 0, 0)]
+    for t in range(tiles):
+        blk = out[:, t * tile:(t + 1) * tile]
+        assert np.all(blk[..., 0] == want[t][0]) and np.all(blk[..., 1] == want[t][1])
+        assert np.all(blk[..., 2] == 0) and np.all(blk[..., 3] == f16(0.4))
+
+
+def test_overlay_underflow_guard_and_ref_clamp():
+    W, H, tile = 32, 16, 16
+    # tile 0: offsets 5 -> 0 (hi < lo: reads as 0, green); tile 1: 0 -> 7
+    to = np.array([5, 0, 7], np.uint32)
+    out = O.tile_overlay(np.zeros((H, W, 4), np.float32), to, tile, W, H, 0)
+    assert np.all(out[:, :16, 1] == np.float16(0.4)) and np.all(out[:, :16, 0] == 0)
+    # debug_ref 0 is clamped to 1: any count >= 1 is full red
+    assert np.all(out[:, 16:, 0] == np.float16(0.4)) and np.all(out[:, 16:, 1] == 0)
+
+
+@pytest.mark.parametrize("W,H,tile", [(100, 37, 16), (33, 70, 8), (64, 64, 32)])
+def test_overlay_matches_float64_on_ragged_viewports(W, H, tile):
+    rng = np.random.default_rng(W * 1000 + H)
+    T = (-(-W // tile)) * (-(-H // tile))
+    to = np.concatenate([[0], np.cumsum(rng.integers(0, 600, T))]).astype(np.uint32)
+    to[-3:-1] = 0  # trailing tiles past the last populated one stay 0 (identify's contract)
+    base = f16_image(rng, H, W)
+    out = O.tile_overlay(base, to, tile, W, H, 256)
+    want = overlay_f64(base.astype(np.float64), to, tile, W, H, 256)
+    # each channel within one binary16 step of the float64 value
+    step = np.abs(want.astype(np.float16).astype(np.float64) - np.nextafter(
+        want.astype(np.float16), np.float16(np.inf)).astype(np.float64))
+    assert np.all(np.abs(out - want) <= step + 1e-12)
+    assert np.array_equal(out, out.astype(np.float16).astype(np.float32))

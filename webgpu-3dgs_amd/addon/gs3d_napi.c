@@ -510,6 +510,38 @@ static napi_value js_sort_passes(napi_env env, napi_callback_info info) {
     return make_typed(env, napi_uint32_array, offs, n, 4);
 }
 
+/* setDebugMode(ctx, mode, debugRef) */
+static napi_value js_set_debug_mode(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t mode = 0, ref = 256;
+    if (!get_u32(env, argv[1], &mode) || (argc > 2 && !get_u32(env, argv[2], &ref))) {
+        napi_throw_type_error(env, NULL, "setDebugMode(ctx, mode, debugRef)");
+        return NULL;
+    }
+    CHECK(env, c, gs3d_set_debug_mode(c, mode, ref), "gs3d_set_debug_mode");
+    return NULL;
+}
+
+/* composite(ctx, format, clear Float32Array(4) | null, framebuffer Uint8Array | null):
+ * blends the last frame onto the framebuffer in place (W*H*4 bytes). */
+static napi_value js_composite(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    int32_t format = 0;
+    void *clear, *fb;
+    size_t cb, fbb;
+    if (napi_get_value_int32(env, argv[1], &format) != napi_ok || !get_bytes(env, argv[2], &clear, &cb) ||
+        !get_bytes(env, argv[3], &fb, &fbb) || (clear && cb < 16)) {
+        napi_throw_type_error(env, NULL, "composite(ctx, format, clear Float32Array(4)|null, framebuffer Uint8Array|null)");
+        return NULL;
+    }
+    CHECK(env, c, gs3d_composite(c, format, (const float*)clear, (uint8_t*)fb, fbb), "gs3d_composite");
+    return NULL;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor d[] = {
 #define FN(js, c) {js, NULL, c, NULL, NULL, NULL, napi_enumerable, NULL}
@@ -541,6 +573,8 @@ static napi_value init(napi_env env, napi_value exports) {
         FN("getKernelVariant", js_get_kernel_variant),
         FN("setProfileMask", js_set_profile_mask),
         FN("sortPasses", js_sort_passes),
+        FN("setDebugMode", js_set_debug_mode),
+        FN("composite", js_composite),
 #undef FN
     };
     NAPI_CALL(env, napi_define_properties(env, exports, sizeof d / sizeof d[0], d));

@@ -170,4 +170,23 @@ hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_
                             uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
 
+// Composite of the f16 target over an unorm8 framebuffer, through the
+// splats-per-tile heat map when debug is set (then also written to
+// debug_target).
+struct CompositeParams {
+    const uint16_t* target;
+    uint16_t* debug_target;
+    uint32_t* framebuffer;
+    const uint32_t* tile_offsets;
+    uint32_t width, height, tile_x, tile_y, tiles_x, tiles_y;
+    uint32_t debug_ref;
+    uint32_t clear;  // packed unorm8 clear colour (framebuffer byte order)
+    int use_clear;
+    int bgra;
+    int debug;
+};
+// Packs an RGBA float colour into one unorm8 framebuffer word (the clear pass).
+uint32_t pack_unorm8_clear(const float rgba[4], int bgra);
+hipError_t launch_composite(const CompositeParams& p, hipStream_t s);
+
 }  // namespace gs3d

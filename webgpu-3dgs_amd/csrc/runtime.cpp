@@ -73,6 +73,9 @@ struct gs3d_ctx {
     bool result_in_b = false;
     bool keep_unsorted = false;
     uint32_t epoch = 0;
+    uint32_t debug_mode = 0, debug_ref = 256;  // setDebugMode, debugRef
+    size_t framebuffer_px = 0;                 // pixels the resident framebuffer holds (0: none yet)
+    size_t debug_target_px = 0;                // pixels of the last heat-map target
 
     // last frame (for re-runs after a capacity miss)
     bool have_frame = false;
@@ -727,6 +730,12 @@ int gs3d_read(gs3d_ctx* c, int id, void* dst, size_t bytes, size_t* written) {
         case GS3D_BUF_UNSORTED_VALUES:
             if (!c->keep_unsorted) return fail(c, GS3D_E_STATE, "rebin without GS3D_REBIN_KEEP_UNSORTED");
             src = buf<void>(c, "unsorted_values"); size = K * 4; break;
+        case GS3D_BUF_DEBUG_TARGET_RGBA16F:
+            if (c->debug_target_px != px) return fail(c, GS3D_E_STATE, "no composite in debug mode at this viewport");
+            src = buf<void>(c, "debug_target"); size = px * 8; break;
+        case GS3D_BUF_FRAMEBUFFER:
+            if (c->framebuffer_px != px) return fail(c, GS3D_E_STATE, "no composite at this viewport");
+            src = buf<void>(c, "framebuffer"); size = px * 4; break;
         default: return fail(c, GS3D_E_UNKNOWN_NAME, "unknown buffer id %d", id);
     }
     if (written) *written = size;
@@ -754,6 +763,61 @@ int gs3d_read_tile_counts(gs3d_ctx* c, uint32_t* counts, size_t n, size_t* writt
     std::vector<uint32_t> off(T + 1);
     GS3D_TRY(gs3d_read(c, GS3D_BUF_TILE_OFFSETS, off.data(), off.size() * 4, nullptr));
     for (size_t t = 0; t < T; t++) counts[t] = off[t + 1] >= off[t] ? off[t + 1] - off[t] : 0u;
+    return GS3D_OK;
+}
+
+int gs3d_set_debug_mode(gs3d_ctx* c, uint32_t mode, uint32_t debug_ref) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (mode > 1) return fail(c, GS3D_E_INVALID_ARG, "debug mode %u (0 = normal, 1 = tile heat map)", mode);
+    c->debug_mode = mode;
+    c->debug_ref = debug_ref;
+    return GS3D_OK;
+}
+
+int gs3d_get_debug_mode(const gs3d_ctx* c, uint32_t* mode, uint32_t* debug_ref) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (mode) *mode = c->debug_mode;
+    if (debug_ref) *debug_ref = c->debug_ref;
+    return GS3D_OK;
+}
+
+int gs3d_composite(gs3d_ctx* c, int format, const float* clear_rgba, uint8_t* framebuffer, size_t bytes) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (format != GS3D_FORMAT_RGBA8UNORM && format != GS3D_FORMAT_BGRA8UNORM)
+        return fail(c, GS3D_E_INVALID_ARG, "framebuffer format %d", format);
+    if (!c->have_frame) return fail(c, GS3D_E_STATE, "no rebin yet");
+    const size_t px = (size_t)c->width * c->height;
+    if (framebuffer && bytes < px * 4)
+        return fail(c, GS3D_E_TOO_SMALL, "framebuffer needs %zu bytes, got %zu", px * 4, bytes);
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    uint32_t* fb;
+    GS3D_TRY(ensure_t(c, "framebuffer", px, &fb));
+    if (!clear_rgba && framebuffer)
+        HIP_TRY(c, hipMemcpyAsync(fb, framebuffer, px * 4, hipMemcpyHostToDevice, c->stream));
+    CompositeParams p{};
+    p.target = buf<uint16_t>(c, "splat_target");
+    p.framebuffer = fb;
+    p.tile_offsets = buf<uint32_t>(c, "tile_offsets");
+    p.width = c->width;
+    p.height = c->height;
+    p.tile_x = c->tile_x;
+    p.tile_y = c->tile_y;
+    p.tiles_x = c->tiles_x;
+    p.tiles_y = c->tiles_y;
+    p.debug_ref = c->debug_ref;
+    p.bgra = format == GS3D_FORMAT_BGRA8UNORM;
+    p.use_clear = clear_rgba != nullptr;
+    p.clear = clear_rgba ? pack_unorm8_clear(clear_rgba, p.bgra) : 0u;
+    p.debug = c->debug_mode == 1;
+    if (p.debug) GS3D_TRY(ensure_t(c, "debug_target", px * 4, &p.debug_target));
+    HIP_TRY(c, launch_composite(p, c->stream));
+    c->framebuffer_px = px;
+    if (p.debug) c->debug_target_px = px;
+    if (framebuffer) {
+        HIP_TRY(c, hipMemcpyAsync(framebuffer, fb, px * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
     return GS3D_OK;
 }
 
@@ -1089,6 +1153,56 @@ int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uin
                                 c->stream));
     HIP_TRY(c, launch_f16_to_f32(dimg, dimg32, (size_t)width * height * 4, c->stream));
     HIP_TRY(c, hipMemcpyAsync(image, dimg32, (size_t)width * height * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_stage_composite(gs3d_ctx* c, const uint16_t* target_f16, const uint32_t* tile_offsets,
+                         uint32_t tile_x, uint32_t tile_y, uint32_t width, uint32_t height,
+                         uint32_t debug, uint32_t debug_ref, int format, const float* clear_rgba,
+                         uint8_t* framebuffer, uint16_t* debug_out) {
+    if (!c || !target_f16 || !framebuffer || (debug && !tile_offsets)) return GS3D_E_INVALID_ARG;
+    if (width == 0 || height == 0 || tile_x == 0 || tile_y == 0)
+        return fail(c, GS3D_E_INVALID_ARG, "zero viewport or tile size");
+    if ((uint64_t)width * height > (1ull << 31)) return fail(c, GS3D_E_INVALID_ARG, "viewport too large");
+    if (debug > 1) return fail(c, GS3D_E_INVALID_ARG, "debug mode %u", debug);
+    if (format != GS3D_FORMAT_RGBA8UNORM && format != GS3D_FORMAT_BGRA8UNORM)
+        return fail(c, GS3D_E_INVALID_ARG, "framebuffer format %d", format);
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    const uint32_t tiles_x = (width + tile_x - 1) / tile_x, tiles_y = (height + tile_y - 1) / tile_y;
+    const size_t px = (size_t)width * height, T = (size_t)tiles_x * tiles_y;
+    uint16_t *dt, *dd = nullptr;
+    uint32_t *dfb, *dto = nullptr;
+    GS3D_TRY(ensure_t(c, "stage_image", px * 4, &dt));
+    GS3D_TRY(ensure_t(c, "stage_framebuffer", px, &dfb));
+    HIP_TRY(c, hipMemcpyAsync(dt, target_f16, px * 8, hipMemcpyHostToDevice, c->stream));
+    if (!clear_rgba) HIP_TRY(c, hipMemcpyAsync(dfb, framebuffer, px * 4, hipMemcpyHostToDevice, c->stream));
+    if (debug) {
+        GS3D_TRY(ensure_t(c, "stage_tile_offsets", T + 1, &dto));
+        GS3D_TRY(ensure_t(c, "stage_debug_target", px * 4, &dd));
+        HIP_TRY(c, hipMemcpyAsync(dto, tile_offsets, (T + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    CompositeParams p{};
+    p.target = dt;
+    p.debug_target = dd;
+    p.framebuffer = dfb;
+    p.tile_offsets = dto;
+    p.width = width;
+    p.height = height;
+    p.tile_x = tile_x;
+    p.tile_y = tile_y;
+    p.tiles_x = tiles_x;
+    p.tiles_y = tiles_y;
+    p.debug_ref = debug_ref;
+    p.bgra = format == GS3D_FORMAT_BGRA8UNORM;
+    p.use_clear = clear_rgba != nullptr;
+    p.clear = clear_rgba ? pack_unorm8_clear(clear_rgba, p.bgra) : 0u;
+    p.debug = (int)debug;
+    HIP_TRY(c, launch_composite(p, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(framebuffer, dfb, px * 4, hipMemcpyDeviceToHost, c->stream));
+    if (debug && debug_out)
+        HIP_TRY(c, hipMemcpyAsync(debug_out, dd, px * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return GS3D_OK;
 }

@@ -27,6 +27,11 @@ BUF_IMAGE_RGBA16F = 7
 BUF_IMAGE_RGBA32F = 8
 BUF_UNSORTED_KEYS = 9
 BUF_UNSORTED_VALUES = 10
+BUF_DEBUG_TARGET_RGBA16F = 11
+BUF_FRAMEBUFFER = 12
+
+FORMAT_RGBA8UNORM = 0
+FORMAT_BGRA8UNORM = 1
 
 REBIN_PROFILE = 1
 REBIN_KEEP_UNSORTED = 2
@@ -77,6 +82,10 @@ _PROTOS = {
     "gs3d_stage_sort": (I32, [P, P, P, U32, U32, P, P]),
     "gs3d_stage_identify": (I32, [P, P, U32, U32, P]),
     "gs3d_stage_rasterize": (I32, [P, P, U32, P, U32, P, U32, U32, U32, U32, P]),
+    "gs3d_set_debug_mode": (I32, [P, U32, U32]),
+    "gs3d_get_debug_mode": (I32, [P, C.POINTER(U32), C.POINTER(U32)]),
+    "gs3d_composite": (I32, [P, I32, P, P, SZ]),
+    "gs3d_stage_composite": (I32, [P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, P]),
 }
 
 _lib = None

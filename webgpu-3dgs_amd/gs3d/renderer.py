@@ -131,6 +131,36 @@ class GaussianSplatRenderer:
             return None
         return out
 
+    # -- display: heat map + composite -----------------------------------------
+    def set_debug_mode(self, mode: int, debug_ref: int = 256) -> None:
+        """setDebugMode (GaussianSplatRenderer.ts:319-323); debug_ref = count drawn full red."""
+        self._check(self._lib.gs3d_set_debug_mode(self._h, mode, debug_ref))
+
+    def get_debug_mode(self) -> tuple[int, int]:
+        m, r = C.c_uint32(), C.c_uint32()
+        self._check(self._lib.gs3d_get_debug_mode(self._h, C.byref(m), C.byref(r)))
+        return int(m.value), int(r.value)
+
+    def composite(self, width: int, height: int, clear=(0.12, 0.12, 0.13, 1.0), framebuffer=None,
+                  fmt: int = L.FORMAT_RGBA8UNORM) -> np.ndarray:
+        """Composite the last frame over a framebuffer; returns it as H x W x 4 uint8.
+
+        clear: RGBA floats the framebuffer is cleared to first, or None to blend
+        onto `framebuffer` (H x W x 4 uint8, or None for the resident one)."""
+        out = (np.zeros((height, width, 4), np.uint8) if framebuffer is None
+               else np.ascontiguousarray(framebuffer, np.uint8).copy())
+        cl = None if clear is None else np.asarray(clear, np.float32)
+        if cl is None and framebuffer is None:
+            self._check(self._lib.gs3d_composite(self._h, fmt, None, None, 0))
+            return self.read(L.BUF_FRAMEBUFFER, np.uint8).reshape(height, width, 4)
+        self._check(self._lib.gs3d_composite(self._h, fmt, L.ptr(cl) if cl is not None else None,
+                                             L.ptr(out), out.nbytes))
+        return out
+
+    def composite_resident(self, fmt: int = L.FORMAT_RGBA8UNORM) -> None:
+        """Composite onto the resident framebuffer, no host copies (asynchronous)."""
+        self._check(self._lib.gs3d_composite(self._h, fmt, None, None, 0))
+
     def stage_times_us(self) -> dict:
         out = np.zeros(len(L.STAGES), np.float32)
         self._check(self._lib.gs3d_stage_times_us(self._h, L.ptr(out), out.size))
@@ -214,3 +244,19 @@ class GaussianSplatRenderer:
             self._h, L.ptr(projected), projected.shape[0], L.ptr(sorted_values),
             sorted_values.size, L.ptr(tile_offsets), tile, tile, width, height, L.ptr(img)))
         return img
+
+    def stage_composite(self, target_f16, tile_offsets, tile, width, height, debug=0, debug_ref=256,
+                        clear=None, framebuffer=None, fmt: int = L.FORMAT_RGBA8UNORM):
+        """Heat map + composite over caller data; returns (framebuffer, debug target or None)."""
+        target_f16 = np.ascontiguousarray(target_f16, np.uint16)
+        assert target_f16.size == width * height * 4
+        to = None if tile_offsets is None else np.ascontiguousarray(tile_offsets, np.uint32)
+        fb = (np.zeros((height, width, 4), np.uint8) if framebuffer is None
+              else np.ascontiguousarray(framebuffer, np.uint8).copy())
+        cl = None if clear is None else np.asarray(clear, np.float32)
+        dbg = np.zeros((height, width, 4), np.uint16) if debug else None
+        self._check(self._lib.gs3d_stage_composite(
+            self._h, L.ptr(target_f16), L.ptr(to) if to is not None else None, tile, tile, width,
+            height, debug, debug_ref, fmt, L.ptr(cl) if cl is not None else None, L.ptr(fb),
+            L.ptr(dbg) if dbg is not None else None))
+        return fb, dbg

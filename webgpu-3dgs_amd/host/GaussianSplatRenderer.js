@@ -2,8 +2,9 @@
 // GaussianSplatRenderer with the reference's API (src/renderers/GaussianSplatRenderer.ts):
 // init (:331-471), render with the rebin gate (:1033-1098), rebin (:1104-1266),
 // setTileSize / clampTileSize (:301-317, :162-167), setDebugMode (:321-323),
-// setAlwaysRebin (:327-329), getTileCount / getTileSize (:1268-1274) and
-// readTileCounts (:1278-1310). The passes run in libgs3d.so on one HIP stream;
+// setAlwaysRebin (:327-329), getTileCount / getTileSize (:1268-1274),
+// readTileCounts (:1278-1310) and the display step of render (heat map +
+// composite, :1075-1097). The passes run in libgs3d.so on one HIP stream;
 // this class keeps the host-side state machine and the resource registries.
 
 const { WorkgroupManager, linear1D, tiled1D, tile2D, BufferManager, PipelineManager } = require('./managers');
@@ -12,7 +13,10 @@ const REBIN_PROFILE = 1;
 const BUF = {
   PROJECTED_SPLATS: 0, SPLAT_REF_COUNTS: 1, SPLAT_REF_OFFSETS: 2, REF_COUNTER: 3, SORT_KEYS: 4,
   SORT_VALUES: 5, TILE_OFFSETS: 6, IMAGE_RGBA16F: 7, IMAGE_RGBA32F: 8,
+  UNSORTED_KEYS: 9, UNSORTED_VALUES: 10, DEBUG_TARGET_RGBA16F: 11, FRAMEBUFFER: 12,
 };
+// Framebuffer byte orders of composite (the swapchain format handed to init).
+const FORMATS = { rgba8unorm: 0, bgra8unorm: 1 };
 const MAX_TILES_PER_SPLAT = 16;  // Config.MAX_TILES_PER_SPLAT: the reference's key budget per splat
 
 function loadAddon() {
@@ -63,7 +67,8 @@ class GaussianSplatRenderer {
     return [tx, ty];
   }
 
-  init(format = 'rgba16float') {
+  init(format = 'rgba8unorm') {
+    if (!(format in FORMATS)) throw new Error(`GaussianSplatRenderer.init: unsupported format ${format}`);
     this.format = format;
     const splatCount = Math.max(1, (this.scene.splats && this.scene.splats.splatCount) || 1);
     this.registerBinningLayouts(splatCount);
@@ -101,15 +106,21 @@ class GaussianSplatRenderer {
     return [tx, ty];
   }
 
+  // 0 = normal, 1 = splats-per-tile heat map (any other value draws normally,
+  // as in the reference, which tests debugMode === 1).
   setDebugMode(mode) {
     this.debugMode = mode | 0;
+    this.addon.setDebugMode(this.ctx, this.debugMode === 1 ? 1 : 0, this.debugRef);
   }
 
   setAlwaysRebin(on) {
     this.alwaysRebin = !!on;
   }
 
-  // frame: {width, height, camera (Camera), cameraVersion?}
+  // frame: {width, height, camera (Camera), cameraVersion?, colorTexture?, clearValue?}
+  // colorTexture: a Uint8Array of width*height*4 bytes in the init format; when
+  // given, the frame is composited onto it (premultiplied over, in place), after
+  // clearing it to clearValue ({r, g, b, a}) if that is set.
   render(frame) {
     const splats = this.scene.splats;
     if (!splats) return false;
@@ -133,7 +144,14 @@ class GaussianSplatRenderer {
     this.lastCameraVersion = version;
     const needsRebin = this.alwaysRebin || cameraMoved || splatCountChanged || viewportChanged;
     if (needsRebin) this.rebin(frame.camera, splatCount, tilesX, tilesY, width, height);
+    if (frame.colorTexture) this.composite(frame.colorTexture, frame.clearValue);
     return needsRebin;
+  }
+
+  // Heat map (debug mode 1) + composite of the current frame onto `target`.
+  composite(target, clearValue = null) {
+    const clear = clearValue ? new Float32Array([clearValue.r, clearValue.g, clearValue.b, clearValue.a]) : null;
+    this.addon.composite(this.ctx, FORMATS[this.format || 'rgba8unorm'], clear, target);
   }
 
   rebin(camera, splatCount, tilesX, tilesY, width, height) {
@@ -183,4 +201,4 @@ class GaussianSplatRenderer {
   }
 }
 
-module.exports = { GaussianSplatRenderer, loadAddon, BUF, REBIN_PROFILE };
+module.exports = { GaussianSplatRenderer, loadAddon, BUF, FORMATS, REBIN_PROFILE };
