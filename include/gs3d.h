@@ -40,6 +40,8 @@ extern "C" {
 #define GS3D_E_STATE         -5   /* call order: e.g. rebin before set_splats   */
 #define GS3D_E_UNKNOWN_NAME  -6   /* registry miss (buffer / kernel variant)    */
 #define GS3D_E_TOO_SMALL     -7   /* destination buffer smaller than the data   */
+#define GS3D_E_FORMAT        -8   /* malformed or unsupported PLY file          */
+#define GS3D_E_IO            -9   /* a file could not be opened or read         */
 
 /* ---- byte-exact data contract ------------------------------------------- */
 
@@ -250,6 +252,48 @@ int gs3d_get_debug_mode(const gs3d_ctx* ctx, uint32_t* mode, uint32_t* debug_ref
  * hold W*H*4). Unorm stores round to nearest: floor(clamp(v, 0, 1)*255 + 0.5). */
 int gs3d_composite(gs3d_ctx* ctx, int format, const float* clear_rgba,
                    uint8_t* framebuffer, size_t bytes);
+
+/* ---- native PLY ingest (SceneWorker.ts:146-549, SceneSyncer.ts:25-43) ---------
+ * The reference parses PLY files in a Web Worker in JS (parsePlyHeader,
+ * createPlyValueReader, parseGaussianSplatBody, writeGaussianSplat). These
+ * entry points restate it in C++: same header rules and error messages, same
+ * activations (f64, stored as f32), binary bodies parsed by several threads.
+ * Divergences: element counts must be whole numbers below 2^32; ascii tokens
+ * are split on ASCII whitespace only. */
+enum gs3d_ply_format {
+    GS3D_PLY_ASCII     = 0,
+    GS3D_PLY_BINARY_LE = 1,
+    GS3D_PLY_BINARY_BE = 2
+};
+
+typedef struct gs3d_ply_info {
+    uint64_t splat_count;    /* rows of the vertex element                         */
+    uint32_t f_rest_count;   /* f_rest_k properties (sorted by k)                  */
+    int32_t  sh_degree;      /* inferSphericalHarmonicsDegree(f_rest_count), :501  */
+    int32_t  format;         /* gs3d_ply_format                                    */
+    uint32_t reserved;
+    uint64_t header_bytes;   /* body offset                                        */
+    double   bounds_min[3];  /* min / max of x, y, z (after gs3d_ply_parse)        */
+    double   bounds_max[3];
+} gs3d_ply_info;
+
+/* Header only: fills splat_count, f_rest_count, sh_degree, format, header_bytes.
+ * GS3D_E_FORMAT with the reference's message in `err` on a bad header. */
+int gs3d_ply_probe(const void* data, size_t bytes, gs3d_ply_info* info,
+                   char* err, size_t err_bytes);
+
+/* Whole file into the loader's arrays (GaussianSplatData, SceneWorker.ts:389-405):
+ * positions/scales/rotations/colors N x 4 f32, f_dc N x 3 f32 (raw values), and
+ * sh_rest N x f_rest_count f32 (null allowed when f_rest_count is 0).
+ * threads: 0 = up to 16. No device or context is needed. */
+int gs3d_ply_parse(const void* data, size_t bytes, float* positions, float* scales,
+                   float* rotations, float* colors, float* f_dc, float* sh_rest,
+                   int threads, gs3d_ply_info* info, char* err, size_t err_bytes);
+
+/* Load a PLY file and upload it (gs3d_set_splats) through pinned staging
+ * memory. SH degree d uses the first (d+1)^2-1 coefficients of each colour
+ * channel when the file holds more. */
+int gs3d_load_ply(gs3d_ctx* ctx, const char* path, int threads, gs3d_ply_info* info);
 
 /* ---- stage entry points (each WGSL pass fed caller data) ---------------- */
 

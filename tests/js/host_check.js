@@ -12,7 +12,7 @@ const b64 = (ta) => (ta ? Buffer.from(ta.buffer, ta.byteOffset, ta.byteLength).t
 const f32 = (s) => new Float32Array(new Uint8Array(Buffer.from(s, 'base64')).buffer);
 
 async function ply(req) {
-  const loader = new host.SceneLoader({ useWorker: req.worker !== false });
+  const loader = new host.SceneLoader({ useWorker: req.worker !== false, native: !!req.native });
   try {
     const { splats, bounds } = await loader.loadFile(req.path);
     return {
@@ -30,6 +30,15 @@ async function ply(req) {
   } catch (e) {
     return { ok: false, error: String(e.message || e) };
   }
+}
+
+// Time the loader on a file (no arrays returned).
+async function plyTime(req) {
+  const loader = new host.SceneLoader({ useWorker: true, native: !!req.native });
+  const t0 = process.hrtime.bigint();
+  const { splats } = await loader.loadFile(req.path);
+  const seconds = Number(process.hrtime.bigint() - t0) / 1e9;
+  return { ok: true, n: splats.splatCount, seconds };
 }
 
 function camera(req) {
@@ -117,7 +126,7 @@ function addonExports() {
   return { ok: true, names: Object.keys(addon).sort(), abiVersion: addon.abiVersion };
 }
 
-const HANDLERS = { ply, camera, workgroups, pipelines, render, addonExports };
+const HANDLERS = { ply, plyTime, camera, workgroups, pipelines, render, addonExports };
 
 (async () => {
   const req = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));

@@ -371,7 +371,7 @@ def test_addon_exports(tmp_path):
             "stageTimesAvg", "profilerReset", "stageLabel", "bufferCreate", "bufferResize", "bufferSize",
             "bufferWrite", "bufferRead", "bufferDestroy", "bufferList", "setKernelVariant",
             "getKernelVariant", "setProfileMask", "sortPasses", "setDebugMode", "composite",
-            "abiVersion"}
+            "parsePly", "loadPly", "abiVersion"}
     assert set(r["names"]) == want
     assert r["abiVersion"] >= 1
 

@@ -7,11 +7,16 @@
  * external whose finalizer destroys it. Built against N-API 8 headers; the
  * napi_* symbols resolve from the node binary at load time.
  */
+#define _GNU_SOURCE /* MAP_POPULATE */
+#include <fcntl.h>
 #include <node_api.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "gs3d.h"
 
@@ -542,6 +547,149 @@ static napi_value js_composite(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+static napi_value f32_array(napi_env env, size_t count, float** data) {
+    napi_value ab, out;
+    if (napi_create_arraybuffer(env, count * 4, (void**)data, &ab) != napi_ok) return NULL;
+    if (napi_create_typedarray(env, napi_float32_array, count, ab, 0, &out) != napi_ok) return NULL;
+    return out;
+}
+
+static napi_value xyz_object(napi_env env, const double v[3]) {
+    napi_value o, x;
+    napi_create_object(env, &o);
+    const char* names[3] = {"x", "y", "z"};
+    for (int i = 0; i < 3; i++) {
+        napi_create_double(env, v[i], &x);
+        napi_set_named_property(env, o, names[i], x);
+    }
+    return o;
+}
+
+static napi_value parse_ply_bytes(napi_env env, const void* data, size_t bytes, int32_t threads);
+
+/* parsePly(bytes | path, threads?) -> {splats: GaussianSplatData (+ fDc), bounds}:
+ * the loader's result (SceneWorker.ts:336-405) parsed natively; throws the
+ * loader's error message on a malformed file. A path string is memory-mapped. */
+static napi_value js_parse_ply(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    int32_t threads = 0;
+    if (argc > 1) napi_get_value_int32(env, argv[1], &threads);
+    napi_valuetype t;
+    napi_typeof(env, argv[0], &t);
+    if (t == napi_string) {
+        char path[4096];
+        size_t len = 0;
+        if (napi_get_value_string_utf8(env, argv[0], path, sizeof path, &len) != napi_ok || len >= sizeof path - 1) {
+            napi_throw_type_error(env, NULL, "parsePly: path too long");
+            return NULL;
+        }
+        const int fd = open(path, O_RDONLY);
+        struct stat st;
+        if (fd < 0 || fstat(fd, &st) != 0) {
+            char msg[4200];
+            snprintf(msg, sizeof msg, "parsePly: cannot open %s", path);
+            if (fd >= 0) close(fd);
+            napi_throw_error(env, NULL, msg);
+            return NULL;
+        }
+        if (st.st_size == 0) {
+            close(fd);
+            static const char empty = 0;
+            return parse_ply_bytes(env, &empty, 0, threads);
+        }
+        void* map = mmap(NULL, (size_t)st.st_size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+        close(fd);
+        if (map == MAP_FAILED) {
+            napi_throw_error(env, NULL, "parsePly: mmap failed");
+            return NULL;
+        }
+        napi_value r = parse_ply_bytes(env, map, (size_t)st.st_size, threads);
+        munmap(map, (size_t)st.st_size);
+        return r;
+    }
+    void* data;
+    size_t bytes;
+    if (!get_bytes(env, argv[0], &data, &bytes) || !data) {
+        napi_throw_type_error(env, NULL, "parsePly(bytes: Buffer | ArrayBuffer | TypedArray | path, threads?)");
+        return NULL;
+    }
+    return parse_ply_bytes(env, data, bytes, threads);
+}
+
+static napi_value parse_ply_bytes(napi_env env, const void* data, size_t bytes, int32_t threads) {
+    gs3d_ply_info pi;
+    char err[512] = {0};
+    if (gs3d_ply_probe(data, bytes, &pi, err, sizeof err) != GS3D_OK) {
+        napi_throw_error(env, NULL, err);
+        return NULL;
+    }
+    const size_t n = (size_t)pi.splat_count, F = pi.f_rest_count;
+    float *pos, *sc, *rot, *col, *dc, *rest = NULL;
+    napi_value jpos = f32_array(env, n * 4, &pos), jcol = f32_array(env, n * 4, &col);
+    napi_value jsc = f32_array(env, n * 4, &sc), jrot = f32_array(env, n * 4, &rot);
+    napi_value jdc = f32_array(env, n * 3, &dc), jrest;
+    if (F) jrest = f32_array(env, n * F, &rest);
+    else napi_get_null(env, &jrest);
+    if (!jpos || !jcol || !jsc || !jrot || !jdc || !jrest) {
+        napi_throw_error(env, NULL, "parsePly: out of memory");
+        return NULL;
+    }
+    if (gs3d_ply_parse(data, bytes, pos, sc, rot, col, dc, rest, threads, &pi, err, sizeof err) != GS3D_OK) {
+        napi_throw_error(env, NULL, err);
+        return NULL;
+    }
+    napi_value splats, bounds, result, v;
+    napi_create_object(env, &splats);
+    napi_set_named_property(env, splats, "positions", jpos);
+    napi_set_named_property(env, splats, "colors", jcol);
+    napi_set_named_property(env, splats, "scales", jsc);
+    napi_set_named_property(env, splats, "rotations", jrot);
+    napi_set_named_property(env, splats, "fDc", jdc);
+    napi_set_named_property(env, splats, "sphericalHarmonics", jrest);
+    napi_create_int32(env, pi.sh_degree, &v);
+    napi_set_named_property(env, splats, "sphericalHarmonicsDegree", v);
+    napi_create_double(env, (double)n, &v);
+    napi_set_named_property(env, splats, "splatCount", v);
+    napi_create_object(env, &bounds);
+    napi_set_named_property(env, bounds, "min", xyz_object(env, pi.bounds_min));
+    napi_set_named_property(env, bounds, "max", xyz_object(env, pi.bounds_max));
+    napi_create_object(env, &result);
+    napi_set_named_property(env, result, "splats", splats);
+    napi_set_named_property(env, result, "bounds", bounds);
+    return result;
+}
+
+/* loadPly(ctx, path, threads?) -> {splatCount, sphericalHarmonicsDegree, fRestCount, bounds}:
+ * parse natively and upload (SceneSyncer.setSplatData). */
+static napi_value js_load_ply(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    char path[4096];
+    size_t len = 0;
+    if (napi_get_value_string_utf8(env, argv[1], path, sizeof path, &len) != napi_ok || len >= sizeof path - 1) {
+        napi_throw_type_error(env, NULL, "loadPly(ctx, path, threads?)");
+        return NULL;
+    }
+    int32_t threads = 0;
+    if (argc > 2) napi_get_value_int32(env, argv[2], &threads);
+    gs3d_ply_info pi;
+    CHECK(env, c, gs3d_load_ply(c, path, threads, &pi), "gs3d_load_ply");
+    napi_value r, v, bounds;
+    napi_create_object(env, &r);
+    napi_create_double(env, (double)pi.splat_count, &v);
+    napi_set_named_property(env, r, "splatCount", v);
+    napi_create_int32(env, pi.sh_degree, &v);
+    napi_set_named_property(env, r, "sphericalHarmonicsDegree", v);
+    napi_create_uint32(env, pi.f_rest_count, &v);
+    napi_set_named_property(env, r, "fRestCount", v);
+    napi_create_object(env, &bounds);
+    napi_set_named_property(env, bounds, "min", xyz_object(env, pi.bounds_min));
+    napi_set_named_property(env, bounds, "max", xyz_object(env, pi.bounds_max));
+    napi_set_named_property(env, r, "bounds", bounds);
+    return r;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor d[] = {
 #define FN(js, c) {js, NULL, c, NULL, NULL, NULL, napi_enumerable, NULL}
@@ -575,6 +723,8 @@ static napi_value init(napi_env env, napi_value exports) {
         FN("sortPasses", js_sort_passes),
         FN("setDebugMode", js_set_debug_mode),
         FN("composite", js_composite),
+        FN("parsePly", js_parse_ply),
+        FN("loadPly", js_load_ply),
 #undef FN
     };
     NAPI_CALL(env, napi_define_properties(env, exports, sizeof d / sizeof d[0], d));

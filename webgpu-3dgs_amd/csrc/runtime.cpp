@@ -483,6 +483,12 @@ int settle(gs3d_ctx* c) {
 
 }  // namespace
 
+// Used by ply.cpp, which has no view of the context's layout.
+__attribute__((visibility("hidden"))) int gs3d_internal_fail(gs3d_ctx* c, int code, const char* msg) {
+    return fail(c, code, "%s", msg);
+}
+__attribute__((visibility("hidden"))) int gs3d_internal_set_device(gs3d_ctx* c) { return set_device(c); }
+
 extern "C" {
 
 int gs3d_abi_version(void) { return GS3D_ABI_VERSION; }

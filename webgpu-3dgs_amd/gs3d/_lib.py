@@ -13,7 +13,8 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgs3d.so"
 GS3D_OK = 0
 ERRORS = {
     -1: "GS3D_E_INVALID_ARG", -2: "GS3D_E_NO_DEVICE", -3: "GS3D_E_HIP", -4: "GS3D_E_OOM",
-    -5: "GS3D_E_STATE", -6: "GS3D_E_UNKNOWN_NAME", -7: "GS3D_E_TOO_SMALL",
+    -5: "GS3D_E_STATE", -6: "GS3D_E_UNKNOWN_NAME", -7: "GS3D_E_TOO_SMALL", -8: "GS3D_E_FORMAT",
+    -9: "GS3D_E_IO",
 }
 
 BUF_PROJECTED_SPLATS = 0
@@ -32,6 +33,8 @@ BUF_FRAMEBUFFER = 12
 
 FORMAT_RGBA8UNORM = 0
 FORMAT_BGRA8UNORM = 1
+
+PLY_ASCII, PLY_BINARY_LE, PLY_BINARY_BE = 0, 1, 2
 
 REBIN_PROFILE = 1
 REBIN_KEEP_UNSORTED = 2
@@ -86,7 +89,17 @@ _PROTOS = {
     "gs3d_get_debug_mode": (I32, [P, C.POINTER(U32), C.POINTER(U32)]),
     "gs3d_composite": (I32, [P, I32, P, P, SZ]),
     "gs3d_stage_composite": (I32, [P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, P]),
+    "gs3d_ply_probe": (I32, [P, SZ, P, C.c_char_p, SZ]),
+    "gs3d_ply_parse": (I32, [P, SZ, P, P, P, P, P, P, I32, P, C.c_char_p, SZ]),
+    "gs3d_load_ply": (I32, [P, C.c_char_p, I32, P]),
 }
+
+
+class PlyInfo(C.Structure):
+    """gs3d_ply_info"""
+    _fields_ = [("splat_count", C.c_uint64), ("f_rest_count", C.c_uint32), ("sh_degree", C.c_int32),
+                ("format", C.c_int32), ("reserved", C.c_uint32), ("header_bytes", C.c_uint64),
+                ("bounds_min", C.c_double * 3), ("bounds_max", C.c_double * 3)]
 
 _lib = None
 

@@ -71,6 +71,13 @@ class GaussianSplatRenderer:
         self._keep = None
         self.splat_count = s.n
 
+    def load_ply(self, path, threads: int = 0) -> L.PlyInfo:
+        """Parse a PLY file natively and upload it (gs3d_load_ply)."""
+        info = L.PlyInfo()
+        self._check(self._lib.gs3d_load_ply(self._h, str(path).encode(), threads, C.byref(info)))
+        self.splat_count = int(info.splat_count)
+        return info
+
     # -- renderer API ---------------------------------------------------------
     def set_tile_size(self, x: int, y: int) -> tuple[int, int]:
         out = np.zeros(2, np.uint32)

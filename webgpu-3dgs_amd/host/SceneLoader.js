@@ -185,12 +185,20 @@ function parseGaussianSplatPly(input) {
 
 // SceneLoader: parse off the caller's thread, one worker per load
 // (the reference restarts its worker on every load, SceneLoader.ts:103-106).
+// With `native`, the addon's multi-threaded C++ parser (parsePly) does the
+// work instead and gives the same arrays and error messages.
 class SceneLoader {
-  constructor({ useWorker = true } = {}) {
+  constructor({ useWorker = true, native = false, addon = null } = {}) {
     this.useWorker = useWorker;
+    this.native = native;
+    this.addon = addon;
   }
 
   loadArrayBuffer(buffer) {
+    if (this.native) {
+      const addon = this.addon || require('./GaussianSplatRenderer').loadAddon();
+      return Promise.resolve().then(() => addon.parsePly(buffer));
+    }
     if (!this.useWorker) return Promise.resolve().then(() => parseGaussianSplatPly(buffer));
     const { Worker } = require('worker_threads');
     return new Promise((resolve, reject) => {
@@ -207,6 +215,10 @@ class SceneLoader {
   }
 
   loadFile(file) {
+    if (this.native) {
+      const addon = this.addon || require('./GaussianSplatRenderer').loadAddon();
+      return Promise.resolve().then(() => addon.parsePly(String(file)));
+    }
     return fs.promises.readFile(file).then((b) => this.loadArrayBuffer(b));
   }
 }
