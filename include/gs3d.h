@@ -229,6 +229,30 @@ int gs3d_buffer_destroy(gs3d_ctx* ctx, const char* name);
  * `count` gets the total number (Profiler.getBuffersSortedBySize). */
 int gs3d_buffer_list(gs3d_ctx* ctx, char* names, uint64_t* sizes, size_t n_max, size_t* count);
 
+/* ---- batched views ------------------------------------------------------------
+ * Many views of the resident splats in one call (no reference counterpart:
+ * the reference renders one view per frame). Each view gets the result a
+ * gs3d_rebin of that camera and viewport would give. Views may differ in
+ * resolution; the tile size, digit width and raster variant are the
+ * context's. The launches are captured into one hipGraph on first use and
+ * replayed while the view shapes stay the same (cameras may change freely);
+ * GS3D_BATCH_EAGER launches them directly instead. The call returns when
+ * every view is done. */
+#define GS3D_MAX_BATCH_VIEWS 1024u
+#define GS3D_BATCH_EAGER 1u
+
+typedef struct gs3d_view {
+    gs3d_camera camera;      /* 288-B camera uniform (CameraUniforms)      */
+    uint32_t    width, height;
+} gs3d_view;
+
+int gs3d_rebin_batch(gs3d_ctx* ctx, const gs3d_view* views, uint32_t n_views, uint32_t flags);
+/* K (keys) of every view of the last batch. */
+int gs3d_batch_ref_counts(gs3d_ctx* ctx, uint64_t* out, uint32_t n);
+/* Per-view results of the last batch: GS3D_BUF_IMAGE_RGBA16F / _RGBA32F,
+ * GS3D_BUF_TILE_OFFSETS, GS3D_BUF_PROJECTED_SPLATS, GS3D_BUF_REF_COUNTER. */
+int gs3d_batch_read(gs3d_ctx* ctx, uint32_t view, int id, void* dst, size_t bytes, size_t* written);
+
 /* ---- display: debug heat map and composite ---------------------------------- */
 
 /* setDebugMode (GaussianSplatRenderer.ts:319-323): 0 = normal, 1 = the

@@ -1,0 +1,132 @@
+"""Batched views (gs3d_rebin_batch): every view of a batch must equal a
+single-view rebin of that camera and viewport bit for bit (the single-view
+path is itself checked against the oracle in test_gpu_parity.py), whether the
+batch runs from a captured graph (replayed with new cameras) or eagerly.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gs3d import _lib as L
+from gs3d import camera as cam_mod
+from gs3d import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def views_for(seed, count, shapes=((320, 180), (640, 360), (200, 120), (480, 272))):
+    vs = S.orbit_views(seed, count)
+    out = []
+    for i, (eye, _, _) in enumerate(vs):
+        w, h = shapes[i % len(shapes)]
+        out.append((cam_mod.view_camera(eye, w, h), w, h))
+    return out
+
+
+def single_view(renderer, cam, w, h):
+    renderer.rebin(cam, w, h)
+    return {
+        "k": renderer.ref_count(),
+        "image": renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16),
+        "tile_offsets": renderer.read(L.BUF_TILE_OFFSETS, np.uint32),
+        "projected": renderer.read(L.BUF_PROJECTED_SPLATS, np.uint32),
+    }
+
+
+def check_batch(renderer, views, eager=False):
+    renderer.rebin_batch(views, eager=eager)
+    ks = renderer.batch_ref_counts()
+    got = []
+    for i in range(len(views)):
+        got.append({"k": int(ks[i]),
+                    "image": renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16),
+                    "tile_offsets": renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32),
+                    "projected": renderer.batch_read(i, L.BUF_PROJECTED_SPLATS, np.uint32)})
+        assert int(renderer.batch_read(i, L.BUF_REF_COUNTER, np.uint32)[0]) == got[-1]["k"]
+    for i, (cam, w, h) in enumerate(views):
+        ref = single_view(renderer, cam, w, h)
+        for key in ("k", "tile_offsets", "projected", "image"):
+            assert np.array_equal(np.asarray(got[i][key]), np.asarray(ref[key])), (i, key)
+    return got
+
+
+@pytest.fixture(scope="module")
+def scene_sh3(renderer):
+    s = S.activate(S.clustered(31, 40_000, 12, sh_degree=3))
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    return s
+
+
+def test_batch_views_equal_single_view_rebins(renderer, scene_sh3):
+    say("batch: 9 views of mixed resolution, graph")
+    views = views_for(7, 9)
+    got = check_batch(renderer, views)
+    # one view straight against the oracle as well
+    cam, w, h = views[1]
+    ref = O.pipeline_f32(cam, scene_sh3, w, h, 16)
+    assert got[1]["k"] == ref["k"]
+    assert np.array_equal(got[1]["tile_offsets"], ref["tile_offsets"])
+
+
+def test_batch_graph_replays_with_new_cameras(renderer, scene_sh3):
+    """Same view shapes, new cameras: the captured graph is replayed; every
+    replay's lookback epochs must differ from the previous replay's."""
+    shapes = ((320, 180), (640, 360), (200, 120), (480, 272), (320, 180))
+    for rep in range(3):
+        say(f"batch: graph replay {rep}")
+        check_batch(renderer, views_for(100 + rep, 5, shapes))
+
+
+def test_batch_eager_equals_graph(renderer, scene_sh3):
+    say("batch: eager vs graph")
+    views = views_for(11, 6)
+    a = check_batch(renderer, views, eager=True)
+    renderer.rebin_batch(views)
+    for i in range(len(views)):
+        assert np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), a[i]["image"])
+
+
+def test_batch_single_view_and_tile_sizes(renderer, scene_sh3):
+    for tile in (8, 32):
+        say(f"batch: tile {tile}")
+        renderer.set_tile_size(tile, tile)
+        try:
+            check_batch(renderer, views_for(5, 3))
+        finally:
+            renderer.set_tile_size(16, 16)
+    say("batch: one view")
+    check_batch(renderer, views_for(6, 1))
+
+
+def test_batch_capacity_regrow(renderer):
+    """Large splats make K exceed the initial 8N key budget of a branch: the
+    batch grows its buffers, captures again and still equals single views."""
+    say("batch: capacity regrow")
+    raw = S.uniform_cube(12, 3000, log_scale_mean=-1.2, log_scale_std=0.2)
+    s = S.activate(raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    views = [(cam_mod.view_camera((0.0, 0.0, 3.0), 640, 360), 640, 360),
+             (cam_mod.view_camera((0.5, 0.2, 2.5), 480, 270), 480, 270)]
+    got = check_batch(renderer, views)
+    assert max(g["k"] for g in got) > 8 * s.n
+
+
+def test_batch_errors(renderer, scene_sh3):
+    lib = L.lib()
+    assert lib.gs3d_rebin_batch(renderer._h, None, 0, 0) == -1
+    bad = np.zeros(1, dtype=L.VIEW_DTYPE)
+    assert lib.gs3d_rebin_batch(renderer._h, L.ptr(bad), 1, 0) == -1  # zero viewport
+    renderer.rebin_batch(views_for(1, 2))
+    size = __import__("ctypes").c_size_t()
+    assert lib.gs3d_batch_read(renderer._h, 5, L.BUF_IMAGE_RGBA16F, None, 0, size) == -1
+    assert lib.gs3d_batch_read(renderer._h, 0, L.BUF_SORT_KEYS, None, 0, size) == -6

@@ -32,6 +32,16 @@ constexpr uint64_t kFlagPrefix = 2ull;
 __host__ __device__ inline uint64_t status_word(uint32_t epoch, uint64_t flag, uint32_t value) {
     return ((uint64_t)epoch << 34) | (flag << 32) | (uint64_t)value;
 }
+// A kernel replayed from a captured graph cannot take a fresh epoch as an
+// argument. It adds a device-resident base instead, which the graph's first
+// node raises by kEpochStride on every replay; the argument is then a distinct
+// offset in [1, kEpochStride) per use of a status array within one replay, so
+// no status word left by the previous replay can match.
+constexpr uint32_t kEpochMask = (1u << 30) - 1u;
+constexpr uint32_t kEpochStride = 1024;
+__device__ __forceinline__ uint32_t resolve_epoch(uint32_t epoch, const uint32_t* epoch_base) {
+    return epoch_base ? ((epoch + *epoch_base) & kEpochMask) : epoch;
+}
 
 // Scan: 1024 threads x 8 counts per thread.
 constexpr int kScanThreads = 1024;
@@ -67,6 +77,35 @@ struct PreprocessArgs {
     uint32_t tiles_x, tiles_y;
     float inv_tile_x, inv_tile_y;
 };
+
+// One view of a batch, as the multi-view preprocess sees it (device memory).
+struct ViewParams {
+    gs3d_camera cam;
+    uint32_t tiles_x, tiles_y;
+    float inv_tile_x, inv_tile_y;
+    gs3d_projected_splat* out;
+    uint32_t* depth_keys;
+    uint2* rects;
+    uint32_t* counters;  // counters[0..counter_words) = 0 except [count_slot] = n
+};
+
+// Views one workgroup of the multi-view preprocess projects each splat into.
+constexpr uint32_t kViewsPerBlock = 8;
+
+struct MultiPreprocessArgs {
+    const ViewParams* views;  // device array of n_views
+    uint32_t n_views;
+    const float4* pos;
+    const float4* scale;
+    const float4* rot;
+    const float4* color;
+    const float* sh;
+    uint32_t n;
+    int sh_degree;
+    uint32_t counter_words;
+    uint32_t count_slot;
+};
+hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s);
 
 struct EmitArgs {
     const gs3d_projected_splat* projected;
@@ -104,6 +143,7 @@ struct ScanRectsArgs {
     uint32_t n;
     uint32_t tiles_x;
     uint32_t epoch;
+    const uint32_t* epoch_base;  // null, or the replay base added to epoch
 };
 hipError_t launch_scan_rects(const ScanRectsArgs& a, hipStream_t s);
 
@@ -155,7 +195,10 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
                              uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
-                             const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s);
+                             const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
+                             const uint32_t* epoch_base = nullptr);
+// epoch_base[0] += kEpochStride (the first node of a captured batch graph).
+hipError_t launch_bump_epoch(uint32_t* epoch_base, hipStream_t s);
 // tile_offsets[0..tile_count] from n = *n_dev sorted keys (n > n_max: all 0).
 // If k_host is set (host-mapped memory), *k_host = *n_dev.
 hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,

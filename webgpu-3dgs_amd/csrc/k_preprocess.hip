@@ -12,6 +12,8 @@
 // SH, coefficient-major SoA so every load is coalesced) in, 64 + 4 B out.
 // FMA contraction is off in this file so the float results equal the CPU
 // restatement operation for operation (same association as the WGSL).
+#include <algorithm>
+
 #include "gs3d_internal.h"
 
 #pragma clang fp contract(off)
@@ -64,39 +66,63 @@ __device__ __forceinline__ void sh_basis(float x, float y, float z, float* b) {
     }
 }
 
-template <int DEG>
-__device__ __forceinline__ void sh_color(const PreprocessArgs& a, uint32_t i, float4 p, float* rgb) {
-    float dx = p.x - a.cam.position[0];
-    float dy = p.y - a.cam.position[1];
-    float dz = p.z - a.cam.position[2];
+// Colour of one splat seen from camera position `eye`: 0.5 + sum_k b_k c_k per
+// channel, clamped to [0, 1]. src.coef(k, ch) gives SH coefficient k of channel ch.
+template <int DEG, class Src>
+__device__ __forceinline__ void sh_color(const float* eye, float4 p, const Src& src, float* rgb) {
+    float dx = p.x - eye[0];
+    float dy = p.y - eye[1];
+    float dz = p.z - eye[2];
     float len = sqrtf(dx * dx + dy * dy + dz * dz);
     float inv = 1.0f / len;
     float b[16];
     sh_basis<DEG>(dx * inv, dy * inv, dz * inv, b);
     constexpr int kCoef = (DEG + 1) * (DEG + 1);
-    const size_t n = a.n;
 #pragma unroll
     for (int ch = 0; ch < 3; ch++) {
-        float acc = b[0] * a.sh[(size_t)ch * n + i];
+        float acc = b[0] * src.coef(0, ch);
 #pragma unroll
-        for (int k = 1; k < kCoef; k++) acc = acc + b[k] * a.sh[(size_t)(k * 3 + ch) * n + i];
+        for (int k = 1; k < kCoef; k++) acc = acc + b[k] * src.coef(k, ch);
         float v = acc + 0.5f;
         rgb[ch] = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
     }
 }
 
-template <int DEG>
-__global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    // the frame's counters (tickets, histograms, K) start from zero; nothing
-    // reads them before this kernel ends
-    if (a.counters && i < a.counter_words) a.counters[i] = (i == a.count_slot) ? a.n : 0u;
-    if (i >= a.n) return;
-    float4* out = reinterpret_cast<float4*>(a.out + i);
+// Splat attributes read from global memory when first needed (culled splats
+// never load their rotation, scale, colour or SH).
+struct GlobalSplat {
+    const PreprocessArgs& a;
+    uint32_t i;
+    __device__ float4 rot() const { return a.rot[i]; }
+    __device__ float4 scale() const { return a.scale[i]; }
+    __device__ float4 color() const { return a.color[i]; }
+    __device__ float coef(int k, int ch) const { return a.sh[(size_t)(k * 3 + ch) * a.n + i]; }
+};
 
-    const float4 p = a.pos[i];
-    const float* V = a.cam.view;
-    const float* P = a.cam.projection;
+// Splat attributes already in registers (one splat projected into several views).
+template <int DEG>
+struct RegSplat {
+    float4 q, s, c;
+    float sh[3][(DEG + 1) * (DEG + 1)];
+    __device__ float4 rot() const { return q; }
+    __device__ float4 scale() const { return s; }
+    __device__ float4 color() const { return c; }
+    __device__ float coef(int k, int ch) const { return sh[ch][k]; }
+};
+
+struct ViewGeometry {
+    const gs3d_camera& cam;
+    uint32_t tiles_x, tiles_y;
+    float inv_tile_x, inv_tile_y;
+};
+
+// preprocess_splats.wgsl:75-184 for one splat in one view. Returns the tile
+// count (0 when culled; then o0..o3 are zero).
+template <int DEG, class Src>
+__device__ __forceinline__ uint32_t project_splat(const ViewGeometry& g, float4 p, const Src& src,
+                                                  float4& o0, float4& o1, float4& o2, float4& o3) {
+    const float* V = g.cam.view;
+    const float* P = g.cam.projection;
     float vp[4], cp[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) vp[r] = V[r] * p.x + V[4 + r] * p.y + V[8 + r] * p.z + V[12 + r] * 1.0f;
@@ -108,15 +134,15 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     float ndc0 = cp[0] / cp[3], ndc1 = cp[1] / cp[3], ndc2 = cp[2] / cp[3];
     ok = ok && !(ndc0 < -1.0f || ndc0 > 1.0f || ndc1 < -1.0f || ndc1 > 1.0f || ndc2 < 0.0f ||
                  ndc2 > 1.0f);
-    const float W = a.cam.viewport[0], H = a.cam.viewport[1];
+    const float W = g.cam.viewport[0], H = g.cam.viewport[1];
     uint32_t count = 0;
-    float4 o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0, o3 = o0;
+    o0 = make_float4(0.f, 0.f, 0.f, 0.f), o1 = o0, o2 = o0, o3 = o0;
     if (ok) {
         const float sx = (ndc0 * 0.5f + 0.5f) * W;
         const float sy = (1.0f - (ndc1 * 0.5f + 0.5f)) * H;
 
-        const float4 q = a.rot[i];
-        const float4 s = a.scale[i];
+        const float4 q = src.rot();
+        const float4 s = src.scale();
         const float qw = q.x, qx = q.y, qy = q.z, qz = q.w;
         const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
         const float xy = qx * qy, xz = qx * qz, yz = qy * qz;
@@ -180,19 +206,19 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
             float maxx = sx + rad, maxy = sy + rad;
             maxx = maxx > W - 1.0f ? W - 1.0f : maxx;
             maxy = maxy > H - 1.0f ? H - 1.0f : maxy;
-            uint32_t t0x = (uint32_t)floorf(minx * a.inv_tile_x);
-            uint32_t t0y = (uint32_t)floorf(miny * a.inv_tile_y);
-            uint32_t t1x = (uint32_t)floorf(maxx * a.inv_tile_x);
-            uint32_t t1y = (uint32_t)floorf(maxy * a.inv_tile_y);
-            t0x = min(t0x, a.tiles_x - 1u);
-            t0y = min(t0y, a.tiles_y - 1u);
-            t1x = min(t1x, a.tiles_x - 1u);
-            t1y = min(t1y, a.tiles_y - 1u);
+            uint32_t t0x = (uint32_t)floorf(minx * g.inv_tile_x);
+            uint32_t t0y = (uint32_t)floorf(miny * g.inv_tile_y);
+            uint32_t t1x = (uint32_t)floorf(maxx * g.inv_tile_x);
+            uint32_t t1y = (uint32_t)floorf(maxy * g.inv_tile_y);
+            t0x = min(t0x, g.tiles_x - 1u);
+            t0y = min(t0y, g.tiles_y - 1u);
+            t1x = min(t1x, g.tiles_x - 1u);
+            t1y = min(t1y, g.tiles_y - 1u);
             if (t0x <= t1x && t0y <= t1y) {
                 count = (t1x - t0x + 1u) * (t1y - t0y + 1u);
-                const float4 col = a.color[i];
+                const float4 col = src.color();
                 float rgb[3] = {col.x, col.y, col.z};
-                if (DEG > 0) sh_color<DEG>(a, i, p, rgb);
+                if (DEG > 0) sh_color<DEG>(g.cam.position, p, src, rgb);
                 o0 = make_float4(sx, sy, vp[2], rad);
                 o1 = make_float4(cc * idet, -cb * idet, ca * idet, col.w);
                 o2 = make_float4(rgb[0], rgb[1], rgb[2], __uint_as_float(1u));
@@ -201,16 +227,69 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
             }
         }
     }
+    return count;
+}
+
+__device__ __forceinline__ void store_projected(gs3d_projected_splat* out_base, uint32_t i, uint32_t count,
+                                                const float4& o0, const float4& o1, const float4& o2,
+                                                const float4& o3, uint32_t* depth_keys, uint2* rects) {
+    float4* out = reinterpret_cast<float4*>(out_base + i);
     out[0] = o0;
     out[1] = o1;
     out[2] = o2;
     out[3] = o3;
-    a.counts[i] = count;
-    if (a.depth_keys) a.depth_keys[i] = count ? __float_as_uint(-o0.z) : 0xFFFFFFFFu;
+    if (depth_keys) depth_keys[i] = count ? __float_as_uint(-o0.z) : 0xFFFFFFFFu;
     // tile coordinates are < 2048 (viewport <= 16384 px, tiles >= 8 px)
-    if (a.rects)
-        a.rects[i] = make_uint2(__float_as_uint(o3.x) | (__float_as_uint(o3.y) << 16),
-                                __float_as_uint(o3.z) | (__float_as_uint(o3.w) << 16));
+    if (rects)
+        rects[i] = make_uint2(__float_as_uint(o3.x) | (__float_as_uint(o3.y) << 16),
+                              __float_as_uint(o3.z) | (__float_as_uint(o3.w) << 16));
+}
+
+template <int DEG>
+__global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    // the frame's counters (tickets, histograms, K) start from zero; nothing
+    // reads them before this kernel ends
+    if (a.counters && i < a.counter_words) a.counters[i] = (i == a.count_slot) ? a.n : 0u;
+    if (i >= a.n) return;
+    const float4 p = a.pos[i];
+    const ViewGeometry g{a.cam, a.tiles_x, a.tiles_y, a.inv_tile_x, a.inv_tile_y};
+    float4 o0, o1, o2, o3;
+    const uint32_t count = project_splat<DEG>(g, p, GlobalSplat{a, i}, o0, o1, o2, o3);
+    store_projected(a.out, i, count, o0, o1, o2, o3, a.depth_keys, a.rects);
+    a.counts[i] = count;
+}
+
+// One splat into the views [view0, view0 + kViewsPerBlock) of a batch: the
+// splat's attributes and SH coefficients are read once, then projected and
+// coloured per view with the same arithmetic as k_preprocess. Each view's
+// counters are zeroed (N stored) by the first threads of the grid's x range.
+template <int DEG>
+__global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t v0 = blockIdx.y * kViewsPerBlock;
+    const uint32_t v1 = min(v0 + kViewsPerBlock, a.n_views);
+    if (i < a.counter_words)
+        for (uint32_t v = v0; v < v1; v++) a.views[v].counters[i] = (i == a.count_slot) ? a.n : 0u;
+    if (i >= a.n) return;
+    const float4 p = a.pos[i];
+    RegSplat<DEG> r;
+    r.q = a.rot[i];
+    r.s = a.scale[i];
+    r.c = a.color[i];
+    if (DEG > 0) {
+#pragma unroll
+        for (int k = 0; k < (DEG + 1) * (DEG + 1); k++)
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) r.sh[ch][k] = a.sh[(size_t)(k * 3 + ch) * a.n + i];
+    }
+    for (uint32_t v = v0; v < v1; v++) {
+        const ViewParams& vp = a.views[v];
+        const ViewGeometry g{vp.cam, vp.tiles_x, vp.tiles_y, vp.inv_tile_x, vp.inv_tile_y};
+        float4 o0, o1, o2, o3;
+        const uint32_t count = project_splat<DEG>(g, p, r, o0, o1, o2, o3);
+        store_projected(vp.out, i, count, o0, o1, o2, o3, vp.depth_keys, vp.rects);
+    }
 }
 
 // f_dc (N x 3) + rest (N x 3R, channel-major per splat) -> coefficient-major SoA.
@@ -237,6 +316,20 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
         case 1: hipLaunchKernelGGL(k_preprocess<1>, dim3(grid), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL(k_preprocess<2>, dim3(grid), dim3(256), 0, s, a); break;
         case 3: hipLaunchKernelGGL(k_preprocess<3>, dim3(grid), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s) {
+    const uint32_t threads = std::max(a.n, a.counter_words);
+    if (threads == 0 || a.n_views == 0) return hipSuccess;
+    const dim3 grid((threads + 255u) / 256u, (a.n_views + kViewsPerBlock - 1) / kViewsPerBlock);
+    switch (a.sh_degree) {
+        case 0: hipLaunchKernelGGL(k_preprocess_views<0>, grid, dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_preprocess_views<1>, grid, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_preprocess_views<2>, grid, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_preprocess_views<3>, grid, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -142,6 +142,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restric
 // records in order instead of gathering splats: recs[p] = (first key, first
 // tile id, width in tiles, depth key); recs[n].x = total = ref_counter[0].
 __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
+    const uint32_t epoch = resolve_epoch(a.epoch, a.epoch_base);
     __shared__ uint32_t s_part;
     __shared__ uint32_t s_wave_sum[kScanThreads / kWave];
     __shared__ uint32_t s_prefix;
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
         // Publish, then look back 64 predecessors at a time.
         if (lane == 0) {
             store_status(&a.status[part],
-                         status_word(a.epoch, part == 0 ? kFlagPrefix : kFlagAggregate, tile_total));
+                         status_word(epoch, part == 0 ? kFlagPrefix : kFlagAggregate, tile_total));
         }
         uint32_t excl = 0;
         int64_t p = (int64_t)part - 1;
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
             if (q >= 0) {
                 do {
                     w = load_status(&a.status[q]);
-                    flag = ((uint32_t)(w >> 34) == a.epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
+                    flag = ((uint32_t)(w >> 34) == epoch) ? (uint32_t)((w >> 32) & 3u) : 0u;
                 } while (flag == 0);
             }
             // lanes with q < 0 contribute nothing and act as a prefix stop.
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
             p -= kWave;
         }
         if (lane == 0) {
-            if (part != 0) store_status(&a.status[part], status_word(a.epoch, kFlagPrefix, excl + tile_total));
+            if (part != 0) store_status(&a.status[part], status_word(epoch, kFlagPrefix, excl + tile_total));
             s_prefix = excl;
             if (part == parts - 1) {
                 a.recs[n] = make_uint4(excl + tile_total, 0u, 0u, 0u);

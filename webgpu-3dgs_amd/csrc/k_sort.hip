@@ -7,7 +7,8 @@
 // stable counting sort on one digit, so the final order is (key, input index).
 //
 // Here a pass is ONE kernel (a "onesweep" pass):
-//   1. a ticket gives the workgroup its 4096-key tile in launch order;
+//   1. a ticket gives the workgroup its tile (16,384 64-bit or 8,192 32-bit
+//      keys) in launch order;
 //   2. each wavefront ranks its keys (1024 of 64 bits or 512 of 32 bits) with
 //      a wave-level multisplit: per digit bit a 64-lane ballot narrows the
 //      mask of lanes holding the same digit; popcount below the lane = stable
@@ -74,8 +75,9 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
     const uint32_t* __restrict__ hist_pass, uint64_t* status, uint32_t* ticket, uint32_t n_max,
-    uint32_t shift, uint32_t epoch, const uint32_t* __restrict__ aux_in,
-    uint32_t* __restrict__ aux_out) {
+    uint32_t shift, uint32_t epoch_arg, const uint32_t* __restrict__ aux_in,
+    uint32_t* __restrict__ aux_out, const uint32_t* __restrict__ epoch_base) {
+    const uint32_t epoch = resolve_epoch(epoch_arg, epoch_base);
     constexpr int RADIX = 1 << BITS;
     constexpr int WAVES = THREADS / kWave;
     constexpr int TILE = THREADS * ITEMS;
@@ -269,20 +271,22 @@ template <int BITS, typename KeyT, bool IOTA, bool AUX>
 static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                         const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass,
                         uint64_t* status, uint32_t* ticket, uint32_t shift, uint32_t epoch,
-                        const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s) {
+                        const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
+                        const uint32_t* epoch_base) {
     constexpr int kThreads = sizeof(KeyT) == 8 ? kSort64Threads : kSort32Threads;
     constexpr int kItems = sizeof(KeyT) == 8 ? kSort64Items : kSort32Items;
     const uint32_t grid = (n_max + kThreads * kItems - 1) / (kThreads * kItems);
     hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA, kThreads, kItems, AUX>), dim3(grid), dim3(kThreads),
                        0, s, static_cast<const KeyT*>(kin), vin, static_cast<KeyT*>(kout), vout, n_dev,
-                       hist_pass, status, ticket, n_max, shift, epoch, aux_in, aux_out);
+                       hist_pass, status, ticket, n_max, shift, epoch, aux_in, aux_out, epoch_base);
 }
 
 hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                              uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
-                             const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s) {
+                             const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
+                             const uint32_t* epoch_base) {
     if (n_max == 0) return hipSuccess;
     const bool aux = aux_in != nullptr && aux_out != nullptr;
     // the gather rides on 32-bit key passes whose values are read from vin
@@ -291,7 +295,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
                     (iota_values ? 1 : 0) + (aux ? 2 : 0);
 #define GS3D_PASS(B, K, I, A)                                                                      \
     launch_pass<B, K, I, A>(kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, \
-                            aux_in, aux_out, s)
+                            aux_in, aux_out, s, epoch_base)
     switch (key) {
         case 110: GS3D_PASS(8, uint64_t, false, false); break;
         case 111: GS3D_PASS(8, uint64_t, true, false); break;
@@ -306,6 +310,17 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
         default: return hipErrorInvalidValue;
     }
 #undef GS3D_PASS
+    return hipGetLastError();
+}
+
+namespace {
+__global__ void k_bump_epoch(uint32_t* epoch_base) {
+    if (threadIdx.x == 0) epoch_base[0] = (epoch_base[0] + kEpochStride) & kEpochMask;
+}
+}  // namespace
+
+hipError_t launch_bump_epoch(uint32_t* epoch_base, hipStream_t s) {
+    hipLaunchKernelGGL(k_bump_epoch, dim3(1), dim3(64), 0, s, epoch_base);
     return hipGetLastError();
 }
 

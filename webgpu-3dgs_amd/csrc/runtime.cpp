@@ -99,6 +99,8 @@ struct gs3d_ctx {
     bool last_profiled = false;
     double stage_sum_us[GS3D_STAGE__COUNT] = {};
     uint64_t stage_frames = 0;
+
+    struct BatchState* batch = nullptr;  // gs3d_rebin_batch state, created on first use
 };
 
 namespace {
@@ -483,6 +485,409 @@ int settle(gs3d_ctx* c) {
 
 }  // namespace
 
+// ---- batched views ------------------------------------------------------------
+//
+// gs3d_rebin_batch renders many views of the resident splats. One launch of
+// k_preprocess_views projects every splat into every view (each splat's
+// attributes and SH coefficients are read once per 8 views). Then each view
+// runs the frame's chain (depth LSD, scan, emit, tile LSD, identify,
+// rasterize) on one of kBatchBranches streams: views are dealt to branches
+// longest-first by pixel count, so the latency-bound tails of one view's
+// kernels overlap with another view's work. Per-view outputs (projected
+// splats, tile ranges, image, K) persist; sort scratch is per branch. The
+// whole sequence is captured once into a hipGraph and replayed; cameras live
+// in device memory, so a new set of cameras replays the same graph.
+
+constexpr int kBatchBranches = 4;
+
+struct BatchView {
+    uint32_t W = 0, H = 0, tiles_x = 0, tiles_y = 0;
+    size_t img_off = 0, to_off = 0;  // offsets into the batch image / tile-range arrays (elements)
+    int branch = 0;
+};
+
+struct BatchState {
+    std::vector<BatchView> views;
+    // what the captured graph was built for
+    std::vector<uint64_t> shape;
+    uint32_t n = 0, tile = 0, bits = 0, ppl = 0;
+    int deg = -1;
+    uint64_t cap[kBatchBranches] = {};
+    bool captured = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t streams[kBatchBranches] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kBatchBranches] = {};
+    uint32_t* pinned_k = nullptr;  // host-mapped, one K per view (written by identify)
+    uint32_t pinned_words = 0;
+    std::vector<uint64_t> K;
+    bool have = false;
+    uint64_t replays = 0;
+};
+
+void drop_graph(BatchState* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->captured = false;
+}
+
+void destroy_batch(gs3d_ctx* c) {
+    BatchState* b = c->batch;
+    if (!b) return;
+    for (int k = 0; k < kBatchBranches; k++) {
+        if (b->streams[k]) (void)hipStreamSynchronize(b->streams[k]);
+    }
+    drop_graph(b);
+    for (int k = 0; k < kBatchBranches; k++) {
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+        if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->pinned_k) (void)hipHostFree(b->pinned_k);
+    delete b;
+    c->batch = nullptr;
+}
+
+int batch_init(gs3d_ctx* c) {
+    if (c->batch) return GS3D_OK;
+    BatchState* b = new BatchState();
+    c->batch = b;
+    for (int k = 0; k < kBatchBranches; k++) {
+        HIP_TRY(c, hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking));
+        HIP_TRY(c, hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+    }
+    HIP_TRY(c, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    return GS3D_OK;
+}
+
+std::string branch_name(int k, const char* what) { return "batch_b" + std::to_string(k) + "_" + what; }
+
+// Passes of one view: shifts of the depth digits, then the tile digits below
+// bit_length(T - 1) (zero digits of every key are skipped, as in a frame).
+uint32_t batch_plan(uint32_t T, uint32_t bits, uint32_t* shifts) {
+    const uint32_t tile_bits = T > 1 ? bit_length(T - 1) : 0;
+    uint32_t p = 0;
+    for (uint32_t sft = 0; sft < 32u + tile_bits; sft += bits) shifts[p++] = sft;
+    return p;
+}
+
+// Allocate every batch buffer for the current views and capacities. Returns
+// true in *moved when any block was (re)allocated (a captured graph then
+// holds stale addresses).
+int batch_buffers(gs3d_ctx* c, bool* moved) {
+    BatchState* b = c->batch;
+    const uint32_t n = c->n;
+    const size_t V = b->views.size();
+    size_t img = 0, to = 0;
+    for (const auto& v : b->views) {
+        img = std::max(img, v.img_off + (size_t)v.W * v.H * 4);
+        to = std::max(to, v.to_off + (size_t)v.tiles_x * v.tiles_y + 1);
+    }
+    const uint32_t radix = 1u << c->digit_bits;
+    std::vector<std::pair<std::string, size_t>> want = {
+        {"batch_views", V * sizeof(ViewParams)},
+        {"batch_projected", V * (size_t)n * sizeof(gs3d_projected_splat)},
+        {"batch_depth_keys", V * (size_t)n * 4},
+        {"batch_rects", V * (size_t)n * 8},
+        {"batch_counters", V * kCtrTotalWords * 4},
+        {"batch_tile_offsets", to * 4},
+        {"batch_images", img * 2},
+        {"batch_epoch", 4},
+    };
+    for (int k = 0; k < kBatchBranches; k++) {
+        const uint64_t cap = b->cap[k];
+        want.push_back({branch_name(k, "depth_keys_b"), (size_t)n * 4});
+        want.push_back({branch_name(k, "order_a"), (size_t)n * 4});
+        want.push_back({branch_name(k, "order_b"), (size_t)n * 4});
+        want.push_back({branch_name(k, "records"), ((size_t)n + 1) * 16});
+        want.push_back({branch_name(k, "hist_partial"),
+                        (size_t)std::max(emit_grid(n), key_hist_rows(n)) * kMaxPasses * radix * 4});
+        want.push_back({branch_name(k, "keys_a"), cap * 8});
+        want.push_back({branch_name(k, "keys_b"), cap * 8});
+        want.push_back({branch_name(k, "values_a"), cap * 4});
+        want.push_back({branch_name(k, "values_b"), cap * 4});
+    }
+    *moved = false;
+    for (auto& w : want) {
+        const void* before = c->bufs.count(w.first) ? c->bufs[w.first].ptr : nullptr;
+        void* p;
+        GS3D_TRY(ensure(c, w.first.c_str(), w.second, &p));
+        if (p != before) {
+            *moved = true;
+            if (w.first == "batch_epoch") HIP_TRY(c, launch_fill_u32((uint32_t*)p, 0u, 1, c->stream));
+        }
+    }
+    for (int k = 0; k < kBatchBranches; k++) {
+        uint64_t* st;
+        const void* before[2] = {c->bufs.count(branch_name(k, "scan_status")) ? c->bufs[branch_name(k, "scan_status")].ptr : nullptr,
+                                 c->bufs.count(branch_name(k, "radix_status")) ? c->bufs[branch_name(k, "radix_status")].ptr : nullptr};
+        GS3D_TRY(ensure_status(c, branch_name(k, "scan_status").c_str(), (size_t)n / kScanTile + 2, &st));
+        if ((void*)st != before[0]) *moved = true;
+        GS3D_TRY(ensure_status(c, branch_name(k, "radix_status").c_str(),
+                               (size_t)radix_status_tiles((uint32_t)std::max<uint64_t>(b->cap[k], n)) * radix, &st));
+        if ((void*)st != before[1]) *moved = true;
+    }
+    if (b->pinned_words < V) {
+        if (b->pinned_k) HIP_TRY(c, hipHostFree(b->pinned_k));
+        b->pinned_k = nullptr;
+        HIP_TRY(c, hipHostMalloc((void**)&b->pinned_k, V * 4, hipHostMallocMapped));
+        b->pinned_words = (uint32_t)V;
+        *moved = true;
+    }
+    return GS3D_OK;
+}
+
+// LSD passes of one view on its branch (the frame's run_passes with the
+// batch's status block and replay epochs).
+int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, uint32_t* vb,
+                 uint32_t key_bytes, bool first_iota, const uint32_t* n_dev, uint64_t n_max,
+                 const uint32_t* shifts, uint32_t passes, const uint32_t* hist, uint32_t* tickets,
+                 uint64_t* status, const uint32_t* epoch_base, uint32_t* epoch_ctr, bool* in_b) {
+    const uint32_t radix = 1u << c->digit_bits;
+    for (uint32_t p = 0; p < passes; p++) {
+        const bool even = (p % 2) == 0;
+        HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va, key_bytes,
+                                     first_iota && p == 0, n_dev, (uint32_t)n_max, hist + p * radix, status,
+                                     tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr, nullptr, nullptr, s,
+                                     epoch_base));
+    }
+    *in_b = (passes % 2) == 1;
+    return GS3D_OK;
+}
+
+// Every launch of one batch, on c->stream and the branch streams.
+int batch_enqueue(gs3d_ctx* c) {
+    BatchState* b = c->batch;
+    const uint32_t n = c->n;
+    const uint32_t V = (uint32_t)b->views.size();
+    const uint32_t bits = c->digit_bits, radix = 1u << bits;
+    ViewParams* vparams = buf<ViewParams>(c, "batch_views");
+    auto* projected = buf<gs3d_projected_splat>(c, "batch_projected");
+    uint32_t* dk = buf<uint32_t>(c, "batch_depth_keys");
+    uint2* rects = buf<uint2>(c, "batch_rects");
+    uint32_t* counters = buf<uint32_t>(c, "batch_counters");
+    uint32_t* tile_offsets = buf<uint32_t>(c, "batch_tile_offsets");
+    uint16_t* images = buf<uint16_t>(c, "batch_images");
+    uint32_t* epoch_base = buf<uint32_t>(c, "batch_epoch");
+
+    HIP_TRY(c, launch_bump_epoch(epoch_base, c->stream));
+    MultiPreprocessArgs ma{};
+    ma.views = vparams;
+    ma.n_views = V;
+    ma.pos = buf<float4>(c, "splat_positions");
+    ma.scale = buf<float4>(c, "splat_scales");
+    ma.rot = buf<float4>(c, "splat_rotations");
+    ma.color = buf<float4>(c, "splat_colors");
+    ma.sh = c->sh_degree > 0 ? buf<float>(c, "splat_sh") : nullptr;
+    ma.n = n;
+    ma.sh_degree = c->sh_degree;
+    ma.counter_words = (uint32_t)kCtrTotalWords;
+    ma.count_slot = kCtrSplatCount;
+    HIP_TRY(c, launch_preprocess_views(ma, c->stream));
+    HIP_TRY(c, hipEventRecord(b->fork, c->stream));
+    for (int k = 0; k < kBatchBranches; k++) HIP_TRY(c, hipStreamWaitEvent(b->streams[k], b->fork, 0));
+
+    uint32_t epoch_ctr[kBatchBranches] = {};
+    for (uint32_t vi = 0; vi < V; vi++) {
+        const BatchView& v = b->views[vi];
+        const int k = v.branch;
+        hipStream_t s = b->streams[k];
+        const uint64_t cap = b->cap[k];
+        const uint32_t T = v.tiles_x * v.tiles_y;
+        uint32_t shifts[kMaxPasses];
+        const uint32_t passes = batch_plan(T, bits, shifts);
+        const uint32_t depth_passes = 32u / bits, tile_passes = passes - depth_passes;
+        uint32_t* ctr = counters + (size_t)vi * kCtrTotalWords;
+        uint32_t* hist_depth = ctr + kCtrHist;
+        uint32_t* hist_tile = ctr + kCtrHist + depth_passes * radix;
+        uint32_t* hist_partial = buf<uint32_t>(c, branch_name(k, "hist_partial").c_str());
+        uint32_t* dka = dk + (size_t)vi * n;
+        uint32_t* dkb = buf<uint32_t>(c, branch_name(k, "depth_keys_b").c_str());
+        uint32_t* ida = buf<uint32_t>(c, branch_name(k, "order_a").c_str());
+        uint32_t* idb = buf<uint32_t>(c, branch_name(k, "order_b").c_str());
+        uint4* recs = buf<uint4>(c, branch_name(k, "records").c_str());
+        uint64_t* ka = buf<uint64_t>(c, branch_name(k, "keys_a").c_str());
+        uint64_t* kb = buf<uint64_t>(c, branch_name(k, "keys_b").c_str());
+        uint32_t* va = buf<uint32_t>(c, branch_name(k, "values_a").c_str());
+        uint32_t* vb = buf<uint32_t>(c, branch_name(k, "values_b").c_str());
+        uint64_t* scan_status = buf<uint64_t>(c, branch_name(k, "scan_status").c_str());
+        uint64_t* radix_status = buf<uint64_t>(c, branch_name(k, "radix_status").c_str());
+        const gs3d_projected_splat* proj = projected + (size_t)vi * n;
+
+        HIP_TRY(c, launch_key_histogram(dka, 4, ctr + kCtrSplatCount, n, hist_depth, hist_partial, bits,
+                                        depth_passes, 0, s));
+        bool order_in_b = false;
+        GS3D_TRY(batch_passes(c, s, dka, ida, dkb, idb, 4, true, ctr + kCtrSplatCount, n, shifts, depth_passes,
+                              hist_depth, ctr + kCtrSortTicket, radix_status, epoch_base, &epoch_ctr[k],
+                              &order_in_b));
+        ScanRectsArgs sa{};
+        sa.order = order_in_b ? idb : ida;
+        sa.depth_keys = order_in_b ? dkb : dka;
+        sa.rects = rects + (size_t)vi * n;
+        sa.recs = recs;
+        sa.ref_counter = ctr + kCtrRefCounter;
+        sa.status = scan_status;
+        sa.ticket = ctr + kCtrScanTicket;
+        sa.n = n;
+        sa.tiles_x = v.tiles_x;
+        sa.epoch = ++epoch_ctr[k];
+        sa.epoch_base = epoch_base;
+        HIP_TRY(c, launch_scan_rects(sa, s));
+        EmitArgs ea{};
+        ea.projected = proj;
+        ea.order = sa.order;
+        ea.recs = recs;
+        ea.keys = ka;
+        ea.values = va;
+        ea.hist = tile_passes ? hist_tile : nullptr;
+        ea.hist_partial = tile_passes ? hist_partial : nullptr;
+        ea.n = n;
+        ea.tiles_x = v.tiles_x;
+        ea.capacity = cap;
+        ea.n_passes = tile_passes;
+        ea.digit_bits = bits;
+        ea.hist_shift0 = 32;
+        HIP_TRY(c, launch_emit(ea, s));
+        bool in_b = false;
+        GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, 8, false, ctr + kCtrRefCounter, cap, shifts + depth_passes,
+                              tile_passes, hist_tile, ctr + kCtrSortTicket + depth_passes, radix_status,
+                              epoch_base, &epoch_ctr[k], &in_b));
+        uint32_t* to = tile_offsets + v.to_off;
+        HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
+                                         b->pinned_k + vi, s));
+        HIP_TRY(c, launch_rasterize(proj, in_b ? vb : va, to, c->tile_x, c->tile_y, v.tiles_x, v.tiles_y, v.W,
+                                    v.H, c->raster_ppl, images + v.img_off, s));
+    }
+    for (int k = 0; k < kBatchBranches; k++) {
+        if (epoch_ctr[k] >= kEpochStride)
+            return fail(c, GS3D_E_INVALID_ARG, "too many views on one branch for the replay epochs");
+        HIP_TRY(c, hipEventRecord(b->join[k], b->streams[k]));
+        HIP_TRY(c, hipStreamWaitEvent(c->stream, b->join[k], 0));
+    }
+    return GS3D_OK;
+}
+
+int batch_capture(gs3d_ctx* c) {
+    BatchState* b = c->batch;
+    drop_graph(b);
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_enqueue(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != GS3D_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIP_TRY(c, e);
+    b->graph = g;
+    HIP_TRY(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->captured = true;
+    return GS3D_OK;
+}
+
+int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t flags) {
+    GS3D_TRY(batch_init(c));
+    BatchState* b = c->batch;
+    // layout: images and tile ranges back to back; branches by pixel count, longest first
+    b->views.assign(n_views, BatchView{});
+    std::vector<uint64_t> shape;
+    size_t img = 0, to = 0;
+    for (uint32_t i = 0; i < n_views; i++) {
+        BatchView& v = b->views[i];
+        v.W = views[i].width;
+        v.H = views[i].height;
+        v.tiles_x = (v.W + c->tile_x - 1) / c->tile_x;
+        v.tiles_y = (v.H + c->tile_y - 1) / c->tile_y;
+        v.img_off = img;
+        v.to_off = to;
+        img += (size_t)v.W * v.H * 4;
+        img = (img + 63) & ~size_t(63);  // 128-B aligned image starts
+        to += (size_t)v.tiles_x * v.tiles_y + 1;
+        shape.push_back(((uint64_t)v.W << 32) | v.H);
+    }
+    std::vector<uint32_t> order(n_views);
+    for (uint32_t i = 0; i < n_views; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return (uint64_t)b->views[x].W * b->views[x].H > (uint64_t)b->views[y].W * b->views[y].H;
+    });
+    uint64_t load[kBatchBranches] = {};
+    for (uint32_t i : order) {
+        int best = 0;
+        for (int k = 1; k < kBatchBranches; k++)
+            if (load[k] < load[best]) best = k;
+        b->views[i].branch = best;
+        load[best] += (uint64_t)b->views[i].W * b->views[i].H;
+    }
+    const bool same = b->captured && shape == b->shape && b->n == c->n && b->deg == c->sh_degree &&
+                      b->tile == c->tile_x && b->bits == c->digit_bits && b->ppl == c->raster_ppl;
+    if (!same) {
+        drop_graph(b);
+        for (int k = 0; k < kBatchBranches; k++)
+            if (b->cap[k] == 0) b->cap[k] = std::max<uint64_t>(8ull * c->n, 1u << 16);
+    }
+    b->shape = shape;
+    b->n = c->n;
+    b->deg = c->sh_degree;
+    b->tile = c->tile_x;
+    b->bits = c->digit_bits;
+    b->ppl = c->raster_ppl;
+    b->have = false;
+    const bool eager = (flags & GS3D_BATCH_EAGER) != 0;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        bool moved = false;
+        GS3D_TRY(batch_buffers(c, &moved));
+        if (moved) drop_graph(b);
+        // cameras and per-view buffer addresses
+        std::vector<ViewParams> vp(n_views);
+        auto* projected = buf<gs3d_projected_splat>(c, "batch_projected");
+        uint32_t* dk = buf<uint32_t>(c, "batch_depth_keys");
+        uint2* rects = buf<uint2>(c, "batch_rects");
+        uint32_t* counters = buf<uint32_t>(c, "batch_counters");
+        for (uint32_t i = 0; i < n_views; i++) {
+            vp[i].cam = views[i].camera;
+            vp[i].tiles_x = b->views[i].tiles_x;
+            vp[i].tiles_y = b->views[i].tiles_y;
+            vp[i].inv_tile_x = (float)(1.0 / (double)c->tile_x);
+            vp[i].inv_tile_y = (float)(1.0 / (double)c->tile_y);
+            vp[i].out = projected + (size_t)i * c->n;
+            vp[i].depth_keys = dk + (size_t)i * c->n;
+            vp[i].rects = rects + (size_t)i * c->n;
+            vp[i].counters = counters + (size_t)i * kCtrTotalWords;
+        }
+        HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_views"), vp.data(), vp.size() * sizeof(ViewParams),
+                                  hipMemcpyHostToDevice, c->stream));
+        if (eager) {
+            GS3D_TRY(batch_enqueue(c));
+        } else {
+            if (!b->captured) GS3D_TRY(batch_capture(c));
+            HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
+            b->replays++;
+        }
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        b->K.assign(n_views, 0);
+        bool grow = false;
+        uint64_t need[kBatchBranches] = {};
+        for (uint32_t i = 0; i < n_views; i++) {
+            b->K[i] = b->pinned_k[i];
+            const int k = b->views[i].branch;
+            need[k] = std::max(need[k], b->K[i]);
+            if (b->K[i] > b->cap[k]) grow = true;
+        }
+        if (!grow) {
+            b->have = true;
+            return GS3D_OK;
+        }
+        for (int k = 0; k < kBatchBranches; k++)
+            if (need[k] > b->cap[k]) b->cap[k] = need[k] + need[k] / 4 + kSortTile;
+        drop_graph(b);
+    }
+    return fail(c, GS3D_E_STATE, "batch key capacity did not settle");
+}
+
 // Used by ply.cpp, which has no view of the context's layout.
 __attribute__((visibility("hidden"))) int gs3d_internal_fail(gs3d_ctx* c, int code, const char* msg) {
     return fail(c, code, "%s", msg);
@@ -543,6 +948,7 @@ int gs3d_destroy(gs3d_ctx* c) {
         for (int s = 0; s < GS3D_STAGE__COUNT; s++)
             for (int w = 0; w < 2; w++)
                 if (c->ring[r][s][w]) (void)hipEventDestroy(c->ring[r][s][w]);
+    destroy_batch(c);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1159,6 +1565,63 @@ int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uin
                                 c->stream));
     HIP_TRY(c, launch_f16_to_f32(dimg, dimg32, (size_t)width * height * 4, c->stream));
     HIP_TRY(c, hipMemcpyAsync(image, dimg32, (size_t)width * height * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t flags) {
+    if (!c || !views || n_views == 0) return GS3D_E_INVALID_ARG;
+    if (n_views > GS3D_MAX_BATCH_VIEWS) return fail(c, GS3D_E_INVALID_ARG, "%u views (at most %u)", n_views, GS3D_MAX_BATCH_VIEWS);
+    for (uint32_t i = 0; i < n_views; i++)
+        if (views[i].width == 0 || views[i].height == 0 || views[i].width > 16384 || views[i].height > 16384)
+            return fail(c, GS3D_E_INVALID_ARG, "view %u: viewport %ux%u", i, views[i].width, views[i].height);
+    if (c->n == 0 && !c->bufs.count("splat_positions")) return fail(c, GS3D_E_STATE, "no splats uploaded");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    return rebin_batch(c, views, n_views, flags);
+}
+
+int gs3d_batch_ref_counts(gs3d_ctx* c, uint64_t* out, uint32_t n) {
+    if (!c || !out) return GS3D_E_INVALID_ARG;
+    if (!c->batch || !c->batch->have) return fail(c, GS3D_E_STATE, "no batch yet");
+    if (n < c->batch->K.size()) return fail(c, GS3D_E_TOO_SMALL, "need %zu counts", c->batch->K.size());
+    for (size_t i = 0; i < c->batch->K.size(); i++) out[i] = c->batch->K[i];
+    return GS3D_OK;
+}
+
+int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes, size_t* written) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    BatchState* b = c->batch;
+    if (!b || !b->have) return fail(c, GS3D_E_STATE, "no batch yet");
+    if (view >= b->views.size()) return fail(c, GS3D_E_INVALID_ARG, "view %u of %zu", view, b->views.size());
+    GS3D_TRY(set_device(c));
+    const BatchView& v = b->views[view];
+    const size_t px = (size_t)v.W * v.H;
+    const void* src = nullptr;
+    size_t size = 0;
+    bool widen = false;
+    switch (id) {
+        case GS3D_BUF_IMAGE_RGBA16F: src = buf<uint16_t>(c, "batch_images") + v.img_off; size = px * 8; break;
+        case GS3D_BUF_IMAGE_RGBA32F:
+            src = buf<uint16_t>(c, "batch_images") + v.img_off; size = px * 16; widen = true; break;
+        case GS3D_BUF_TILE_OFFSETS:
+            src = buf<uint32_t>(c, "batch_tile_offsets") + v.to_off; size = ((size_t)v.tiles_x * v.tiles_y + 1) * 4; break;
+        case GS3D_BUF_PROJECTED_SPLATS:
+            src = buf<gs3d_projected_splat>(c, "batch_projected") + (size_t)view * c->n; size = (size_t)c->n * 64; break;
+        case GS3D_BUF_REF_COUNTER:
+            src = buf<uint32_t>(c, "batch_counters") + (size_t)view * kCtrTotalWords + kCtrRefCounter; size = 4; break;
+        default: return fail(c, GS3D_E_UNKNOWN_NAME, "buffer %d is not kept per batch view", id);
+    }
+    if (written) *written = size;
+    if (!dst) return GS3D_OK;
+    if (bytes < size) return fail(c, GS3D_E_TOO_SMALL, "buffer %d needs %zu bytes, got %zu", id, size, bytes);
+    if (widen) {
+        float* tmp;
+        GS3D_TRY(ensure_t(c, "readback_f32", px * 4, &tmp));
+        HIP_TRY(c, launch_f16_to_f32(static_cast<const uint16_t*>(src), tmp, px * 4, c->stream));
+        src = tmp;
+    }
+    HIP_TRY(c, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return GS3D_OK;
 }

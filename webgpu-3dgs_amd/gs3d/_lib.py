@@ -36,6 +36,11 @@ FORMAT_BGRA8UNORM = 1
 
 PLY_ASCII, PLY_BINARY_LE, PLY_BINARY_BE = 0, 1, 2
 
+BATCH_EAGER = 1
+MAX_BATCH_VIEWS = 1024
+# gs3d_view: 72-float camera uniform + width + height
+VIEW_DTYPE = [("camera", "<f4", (72,)), ("width", "<u4"), ("height", "<u4")]
+
 REBIN_PROFILE = 1
 REBIN_KEEP_UNSORTED = 2
 
@@ -89,6 +94,9 @@ _PROTOS = {
     "gs3d_get_debug_mode": (I32, [P, C.POINTER(U32), C.POINTER(U32)]),
     "gs3d_composite": (I32, [P, I32, P, P, SZ]),
     "gs3d_stage_composite": (I32, [P, P, P, U32, U32, U32, U32, U32, U32, I32, P, P, P]),
+    "gs3d_rebin_batch": (I32, [P, P, U32, U32]),
+    "gs3d_batch_ref_counts": (I32, [P, P, U32]),
+    "gs3d_batch_read": (I32, [P, U32, I32, P, SZ, C.POINTER(SZ)]),
     "gs3d_ply_probe": (I32, [P, SZ, P, C.c_char_p, SZ]),
     "gs3d_ply_parse": (I32, [P, SZ, P, P, P, P, P, P, I32, P, C.c_char_p, SZ]),
     "gs3d_load_ply": (I32, [P, C.c_char_p, I32, P]),

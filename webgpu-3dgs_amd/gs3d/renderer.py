@@ -138,6 +138,34 @@ class GaussianSplatRenderer:
             return None
         return out
 
+    # -- batched views ----------------------------------------------------------
+    def rebin_batch(self, views, eager: bool = False) -> None:
+        """Render many views: `views` is a list of (camera uniform (72 f32), width, height)."""
+        arr = np.zeros(len(views), dtype=L.VIEW_DTYPE)
+        for i, (cam, w, h) in enumerate(views):
+            cam = np.asarray(cam, np.float32).reshape(-1)
+            if cam.size != 72:
+                raise ValueError("camera uniform must hold 72 floats")
+            arr[i]["camera"] = cam
+            arr[i]["width"] = w
+            arr[i]["height"] = h
+        self._check(self._lib.gs3d_rebin_batch(self._h, L.ptr(arr), len(views),
+                                               L.BATCH_EAGER if eager else 0))
+        self._batch_views = [(int(w), int(h)) for _, w, h in views]
+
+    def batch_ref_counts(self) -> np.ndarray:
+        out = np.zeros(len(self._batch_views), np.uint64)
+        self._check(self._lib.gs3d_batch_ref_counts(self._h, L.ptr(out), out.size))
+        return out
+
+    def batch_read(self, view: int, buffer_id: int, dtype) -> np.ndarray:
+        size = C.c_size_t()
+        self._check(self._lib.gs3d_batch_read(self._h, view, buffer_id, None, 0, C.byref(size)))
+        out = np.empty(size.value // np.dtype(dtype).itemsize, dtype)
+        self._check(self._lib.gs3d_batch_read(self._h, view, buffer_id, L.ptr(out), out.nbytes,
+                                              C.byref(size)))
+        return out
+
     # -- display: heat map + composite -----------------------------------------
     def set_debug_mode(self, mode: int, debug_ref: int = 256) -> None:
         """setDebugMode (GaussianSplatRenderer.ts:319-323); debug_ref = count drawn full red."""
