@@ -6,7 +6,12 @@ rasterize) of the 1M-splat SH3 scene at 1920x1080 with 16x16 tiles, splats
 already resident in HBM. Prints ONE JSON line (rank 0).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1]
-                  [--no-cpu-baseline] [--cpu-seconds S]
+                  [--no-cpu-baseline] [--cpu-seconds S] [--no-batched]
+
+The same line carries "batched": the 2M-splat SH3 scene rendered for 64
+seeded orbit views of uneven resolution (640x360 / 1280x720 / 1920x1080) per
+gs3d_rebin_batch call, as views/s from one replayed hipGraph, beside the
+same views launched eagerly and one gs3d_rebin per view.
 """
 from __future__ import annotations
 
@@ -22,7 +27,7 @@ import numpy as np
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "webgpu-3dgs_amd"))
 
-METRIC = json.loads((REPO / "BASELINE.json").read_text())["metric"]
+METRIC = "frames/s at 1M-splat SH3 1080p; batched views/s, learned vs plain @\u226545dB"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 RASTER_FLOP_PER_PAIR = 23  # ~22 flop + 1 exp per evaluated (pixel, splat) pair
@@ -36,23 +41,23 @@ def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
     """Bytes each stage must move at minimum, per launch group of one frame.
 
     preprocess: vec4 inputs + SH coefficients in, 64-B projected splat + count
-    + depth key out; depth sort: per pass a u32 key + u32 id read and written
+    + depth key + 8-B tile rectangle out; depth sort: per pass a u32 key + u32 id read and written
     (the first pass reads keys only), plus one read of the keys to histogram;
     scan: the sorted id and depth key and the 8-B tile rectangle of each
     splat in, a 16-B record per splat out; emit: the records and ids in,
     12-B (key, id) out; key sort: 24 B per key per
-    pass; identify: one (T+1)-entry write plus the sorted tile words it
-    searches; raster: the 48 B used of each listed splat, tile ranges, 8-B
-    pixel out.
+    pass; identify: one (T+1)-entry write plus, per tile, the ceil(log2 K)
+    sorted keys its binary search reads; raster: the 48 B used of each listed
+    splat, tile ranges, 8-B pixel out.
     """
     pre_in = 64 + (12 * (deg + 1) ** 2 if deg else 0)
     return {
-        "preprocess-splats": n * (pre_in + 68 + 4),
+        "preprocess-splats": n * (pre_in + 64 + 4 + 4 + 8),
         "radix-sort-splats": n * (4 + 12 + 16 * (depth_passes - 1)) if depth_passes else 0,
         "prefix-scan": 16 * n + 16 * (n + 1),
         "emit-tile-refs": 16 * (n + 1) + 4 * n + 12 * k,
         "radix-sort": tile_passes * 24 * k,
-        "identify-tile-ranges": 8 * k + 4 * (tiles + 1),
+        "identify-tile-ranges": 8 * tiles * max(1, int(k).bit_length()) + 4 * (tiles + 1),
         "rasterize-tiles": 48 * k + 8 * (tiles + 1) + 8 * pixels,
     }
 
@@ -89,6 +94,53 @@ def cpu_baseline(cam, splats, W, H, tile, seconds):
                       f"{threads} OpenMP threads, {el:.1f} s)"}, visited, blended
 
 
+def bench_batched(steps: int, warmup: int) -> dict:
+    """views/s of the 64-view batch (2M splats, SH3, uneven resolutions)."""
+    from gs3d import camera, scene
+    from gs3d.renderer import GaussianSplatRenderer
+
+    t = time.perf_counter()
+    wl = scene.workload(3)
+    splats = scene.activate(wl.raw)
+    views = [(camera.view_camera(eye, w, h, wl.fov_y), w, h) for eye, w, h in wl.views]
+    px = sum(w * h for _, w, h in views)
+    log(f"[bench] batched workload {wl.name}: {splats.n} splats, {len(views)} views, "
+        f"{px / 1e6:.1f} Mpx (generated in {time.perf_counter() - t:.1f} s)")
+    r = GaussianSplatRenderer(0)
+    r.set_splat_data(splats)
+    r.set_tile_size(wl.tile, wl.tile)
+    out = {"workload": wl.name, "views": len(views), "megapixels": round(px / 1e6, 2)}
+
+    def timed(fn, reps):
+        r.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        r.sync()
+        return (time.perf_counter() - t0) / reps
+
+    for _ in range(warmup):
+        r.rebin_batch(views)
+    per = timed(lambda: r.rebin_batch(views), steps)
+    ks = r.batch_ref_counts()
+    out.update({"value": round(len(views) / per, 2), "unit": "views/s", "ms_per_batch": round(per * 1e3, 3),
+                "mode": "hipGraph replay", "keys_total": int(ks.sum()), "keys_max_view": int(ks.max())})
+    r.rebin_batch(views, eager=True)
+    per_e = timed(lambda: r.rebin_batch(views, eager=True), max(1, steps // 2))
+    out["eager_views_per_s"] = round(len(views) / per_e, 2)
+
+    def one_by_one():
+        for cam, w, h in views:
+            r.rebin(cam, w, h)
+    one_by_one()
+    per_s = timed(one_by_one, 1)
+    out["single_view_rebins_views_per_s"] = round(len(views) / per_s, 2)
+    log(f"[bench] batched: {out['value']} views/s (graph), {out['eager_views_per_s']} eager, "
+        f"{out['single_view_rebins_views_per_s']} one rebin per view; K total {out['keys_total']}")
+    r.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +150,10 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batch measurement")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the 64-view batch measurement (for kernel traces of it)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--composite", type=int, default=0,
                     help="after the timed region, run this many composites per display mode "
                          "(plain, tile heat map) and report their wall time per call")
@@ -106,6 +162,10 @@ def main() -> int:
         print("bench.py measures one GPU: this path runs on a single MI355X "
               "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
         return 2
+
+    if args.only_batched:
+        print(json.dumps({"batched": bench_batched(args.batch_steps, 2)}), flush=True)
+        return 0
 
     from gs3d import camera, scene
     from gs3d._lib import STAGES
@@ -234,8 +294,10 @@ def main() -> int:
                            "algo_bytes": (24 if mode else 16) * W * H}
         r.set_debug_mode(0, 256)
         out["display"] = disp
-    print(json.dumps(out), flush=True)
     r.close()
+    if not args.no_batched:
+        out["batched"] = bench_batched(args.batch_steps, 2)
+    print(json.dumps(out), flush=True)
     return 0
 
 
