@@ -141,6 +141,56 @@ def bench_batched(steps: int, warmup: int) -> dict:
     return out
 
 
+def bench_learned(steps: int, warmup: int, epochs: int) -> dict:
+    """The 32-view 1080p batch of the 3M-splat SH3 scene, coloured by exact SH and by the
+    learned stage (trained here on the GPU); PSNR of the learned render against exact SH."""
+    from gs3d import camera, scene
+    from gs3d.renderer import GaussianSplatRenderer
+
+    t = time.perf_counter()
+    wl = scene.workload(4)
+    splats = scene.activate(wl.raw)
+    views = [(camera.view_camera(eye, w, h, wl.fov_y), w, h) for eye, w, h in wl.views]
+    log(f"[bench] learned workload {wl.name}: {splats.n} splats, {len(views)} views "
+        f"(generated in {time.perf_counter() - t:.1f} s)")
+    r = GaussianSplatRenderer(0)
+    r.set_splat_data(splats)
+    r.set_tile_size(wl.tile, wl.tile)
+    r.learned_init(seed=4)
+    r.sync()
+    t0 = time.perf_counter()
+    losses = r.learned_train(epochs, lr=3e-3, lr_latent=3e-2)
+    train_s = time.perf_counter() - t0
+    psnr, enabled = r.learned_validate(views, 45.0)
+    log(f"[bench] learned: {epochs} epochs in {train_s:.2f} s (loss {losses[0]:.5f} -> {losses[-1]:.5f}); "
+        f"PSNR vs exact SH {psnr:.2f} dB -> {'learned colours' if enabled else 'falls back to exact SH'}")
+
+    def rate(learned):
+        for _ in range(warmup):
+            r.rebin_batch(views, learned=learned)
+        r.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.rebin_batch(views, learned=learned)
+        r.sync()
+        return len(views) * steps / (time.perf_counter() - t0)
+
+    plain = rate(False)
+    r.learned_enable(True)  # measured even when the 45 dB gate falls back
+    learned = rate(True)
+    r.learned_enable(enabled)
+    out = {"workload": wl.name, "views": len(views), "plain_views_per_s": round(plain, 2),
+           "learned_views_per_s": round(learned, 2), "learned_over_plain": round(learned / plain, 3),
+           "psnr_db": round(psnr, 2), "psnr_gate_db": 45.0, "learned_enabled": enabled,
+           "train_epochs": epochs, "train_seconds": round(train_s, 2),
+           "train_samples_per_s": round(epochs * 256 * splats.n / train_s, 1),
+           "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1])}
+    log(f"[bench] learned batch {out['learned_views_per_s']} views/s vs exact SH {out['plain_views_per_s']} "
+        f"({out['learned_over_plain']}x)")
+    r.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +203,9 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batch measurement")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the 64-view batch measurement (for kernel traces of it)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-colour measurement")
+    ap.add_argument("--only-learned", action="store_true", help="run only the learned-colour measurement")
+    ap.add_argument("--learned-epochs", type=int, default=4)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--composite", type=int, default=0,
                     help="after the timed region, run this many composites per display mode "
@@ -165,6 +218,9 @@ def main() -> int:
 
     if args.only_batched:
         print(json.dumps({"batched": bench_batched(args.batch_steps, 2)}), flush=True)
+        return 0
+    if args.only_learned:
+        print(json.dumps({"learned": bench_learned(args.batch_steps, 2, args.learned_epochs)}), flush=True)
         return 0
 
     from gs3d import camera, scene
@@ -297,6 +353,8 @@ def main() -> int:
     r.close()
     if not args.no_batched:
         out["batched"] = bench_batched(args.batch_steps, 2)
+    if not args.no_learned:
+        out["learned"] = bench_learned(args.batch_steps, 2, args.learned_epochs)
     print(json.dumps(out), flush=True)
     return 0
 

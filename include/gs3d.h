@@ -253,6 +253,47 @@ int gs3d_batch_ref_counts(gs3d_ctx* ctx, uint64_t* out, uint32_t n);
  * GS3D_BUF_TILE_OFFSETS, GS3D_BUF_PROJECTED_SPLATS, GS3D_BUF_REF_COUNTER. */
 int gs3d_batch_read(gs3d_ctx* ctx, uint32_t view, int id, void* dst, size_t bytes, size_t* written);
 
+/* ---- learned colour -----------------------------------------------------------
+ * No reference counterpart. Each splat stores an 8-dim fp16 latent z; a shared
+ * MLP maps (z, unit view direction) -> 64 -> 64 -> 3 (ReLU, ReLU, sigmoid)
+ * with bf16 weights and fp32 accumulation on MFMA. It is trained on the GPU
+ * against the exact SH colours clamp01(0.5 + sum_k b_k(d) c_k) of the uploaded
+ * splats at 256 fixed unit directions (a Fibonacci sphere) with Adam, and
+ * replaces the SH evaluation in a batch rendered with GS3D_BATCH_LEARNED once
+ * gs3d_learned_validate has found it within the PSNR threshold of exact SH;
+ * otherwise such a batch falls back to exact SH.
+ * Parameters, packed fp32: W1[64][11], b1[64], W2[64][64], b2[64], W3[3][64],
+ * b3[3] (the input vector is z0..z7, dx, dy, dz). */
+#define GS3D_LEARNED_PARAMS 5123
+#define GS3D_LEARNED_DIRS 256
+#define GS3D_BATCH_LEARNED 2u
+
+/* Initial parameters (He-normal weights, zero biases, latents N(0, 0.1)) from `seed`. */
+int gs3d_learned_init(gs3d_ctx* ctx, uint64_t seed);
+/* `epochs` passes over splats x 256 directions in steps of 16 directions;
+ * Adam (b1 0.9, b2 0.999, eps 1e-8) with learning rates lr (weights) and
+ * lr_latent (latents). loss[e] (nullable, n_loss entries) = mean squared error
+ * of epoch e. */
+int gs3d_learned_train(gs3d_ctx* ctx, uint32_t epochs, float lr, float lr_latent,
+                       float* loss, uint32_t n_loss);
+/* Gradients of the mean squared error over splats x directions
+ * [first_dir, first_dir + 16) at the current parameters, without an update. */
+int gs3d_learned_grad(gs3d_ctx* ctx, uint32_t first_dir, float* grad_params,
+                      float* grad_latents, float* loss);
+int gs3d_learned_get(gs3d_ctx* ctx, float* params, float* latents, uint16_t* latents_f16);
+int gs3d_learned_set(gs3d_ctx* ctx, const float* params, const float* latents);
+/* The 256 training directions ([256][3]). */
+int gs3d_learned_dirs(float* dirs);
+/* Learned colours of every splat seen from `eye` (N x 3 f32). */
+int gs3d_learned_colors(gs3d_ctx* ctx, const float eye[3], float* rgb);
+/* Render the views with exact SH and with learned colours; psnr = 10 log10(1 /
+ * MSE) over the RGB of every pixel of every view. Learned colours are enabled
+ * for GS3D_BATCH_LEARNED iff psnr >= min_psnr_db. */
+int gs3d_learned_validate(gs3d_ctx* ctx, const gs3d_view* views, uint32_t n_views,
+                          float min_psnr_db, float* psnr, int* enabled);
+/* Force the learned colours on or off for GS3D_BATCH_LEARNED batches. */
+int gs3d_learned_enable(gs3d_ctx* ctx, int on);
+
 /* ---- display: debug heat map and composite ---------------------------------- */
 
 /* setDebugMode (GaussianSplatRenderer.ts:319-323): 0 = normal, 1 = the

@@ -1,0 +1,208 @@
+"""Learned colour stage: the oracle (oracle/learned_oracle.py) pinned on the
+CPU, then the HIP kernels (k_learned.hip) against it on the GPU: gradients of
+one step, inference colours, and what training gains over the DC colour the
+reference uses.
+"""
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gs3d import _lib as L
+from gs3d import camera as cam_mod
+from gs3d import scene as S
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import learned_oracle as LO  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def random_params(rng, scale=1.0):
+    W1 = rng.normal(0, math.sqrt(2 / 11) * scale, (64, 11))
+    W2 = rng.normal(0, math.sqrt(2 / 64) * scale, (64, 64))
+    W3 = rng.normal(0, math.sqrt(1 / 64) * scale, (3, 64))
+    return LO.pack(W1, rng.normal(0, 0.1, 64), W2, rng.normal(0, 0.1, 64), W3, rng.normal(0, 0.1, 3))
+
+
+# ---------------------------------------------------------------- oracle (CPU)
+def test_oracle_backward_matches_finite_differences():
+    """float64 oracle: analytic gradients of the mean squared error vs central differences."""
+    rng = np.random.default_rng(1)
+    S_ = 24
+    p = random_params(rng)
+    z = rng.normal(0, 0.5, (S_, 8))
+    d = rng.normal(0, 1, (S_, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t = rng.uniform(0, 1, (S_, 3))
+    loss, gp, gz = LO.loss_and_grads(p, z, d, t, bf16=False)
+    eps = 1e-6
+    for idx in list(rng.choice(LO.N_PARAMS, 60, replace=False)) + [703, 704 + 3, 5120, 5122]:
+        q = p.copy()
+        q[idx] += eps
+        lp = LO.loss_and_grads(q, z, d, t, bf16=False)[0]
+        q[idx] -= 2 * eps
+        lm = LO.loss_and_grads(q, z, d, t, bf16=False)[0]
+        assert abs((lp - lm) / (2 * eps) - gp[idx]) <= 1e-6 + 1e-4 * abs(gp[idx]), idx
+    for s_, k in [(0, 0), (5, 7), (23, 3)]:
+        zz = z.copy()
+        zz[s_, k] += eps
+        lp = LO.loss_and_grads(p, zz, d, t, bf16=False)[0]
+        zz[s_, k] -= 2 * eps
+        lm = LO.loss_and_grads(p, zz, d, t, bf16=False)[0]
+        assert abs((lp - lm) / (2 * eps) - gz[s_, k]) <= 1e-6 + 1e-4 * abs(gz[s_, k])
+
+
+def test_oracle_bf16_close_to_float64_and_forward_by_hand():
+    rng = np.random.default_rng(2)
+    p = random_params(rng)
+    z = rng.normal(0, 0.5, (40, 8))
+    d = rng.normal(0, 1, (40, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    y64, _ = LO.forward(p, z, d, bf16=False)
+    y16, _ = LO.forward(p, z, d, bf16=True)
+    assert np.abs(y64 - y16).max() < 2e-2
+    # one sample by explicit loops
+    W1, b1, W2, b2, W3, b3 = LO.unpack(p)
+    x = np.concatenate([z[0], d[0]])
+    h1 = [max(0.0, sum(W1[i, k] * x[k] for k in range(11)) + b1[i]) for i in range(64)]
+    h2 = [max(0.0, sum(W2[i, k] * h1[k] for k in range(64)) + b2[i]) for i in range(64)]
+    y = [1 / (1 + math.exp(-(sum(W3[c, k] * h2[k] for k in range(64)) + b3[c]))) for c in range(3)]
+    assert np.allclose(y, y64[0], atol=1e-12)
+
+
+def test_oracle_sh_basis_and_targets():
+    """The oracle's SH basis equals the C oracle's (itself pinned against scipy), and a
+    zero f_rest gives the DC colour clamp01(0.5 + C0 f_dc) in every direction."""
+    dirs = LO.fibonacci_dirs()
+    assert np.allclose(np.linalg.norm(dirs.astype(np.float64), axis=1), 1.0, atol=1e-6)
+    assert np.abs(dirs.astype(np.float64).mean(axis=0)).max() < 0.01  # evenly spread
+    for dvec in dirs[::37]:
+        want = O.sh_basis(float(dvec[0]), float(dvec[1]), float(dvec[2]), 3)
+        assert np.allclose(LO.sh_basis(dvec.astype(np.float64)), want, atol=1e-12)
+    f_dc = np.array([[0.3, -1.0, 4.0]])
+    t = LO.sh_targets(f_dc, np.zeros((1, 45)), 3, dirs[:5])
+    assert np.allclose(t, np.clip(0.5 + 0.28209479177387814 * f_dc, 0, 1)[:, None, :])
+
+
+def test_library_directions_equal_oracle():
+    from gs3d.renderer import GaussianSplatRenderer
+
+    assert np.array_equal(GaussianSplatRenderer.learned_dirs(), LO.fibonacci_dirs())
+
+
+# ---------------------------------------------------------------- GPU
+@pytest.fixture(scope="module")
+def small_scene(renderer):
+    raw = S.clustered(21, 600, 6, sh_degree=3)
+    s = S.activate(raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    renderer.learned_init(seed=3)
+    return s
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(1e-30, np.linalg.norm(b)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("first_dir", [0, 112, 240])
+def test_step_gradients_match_oracle(renderer, small_scene, first_dir):
+    say(f"learned: gradients of one step, directions {first_dir}..{first_dir + 15}")
+    p, z, _ = renderer.learned_get()
+    loss, gp, gl = renderer.learned_grad(first_dir)
+    s = small_scene
+    rl, rgp, rgl = LO.step_grads(p, z, s.f_dc, s.sh_rest, 3, first_dir)
+    say(f"  loss gpu {loss:.6g} oracle {rl:.6g}; rel err params {rel_err(gp, rgp):.3g} latents {rel_err(gl, rgl):.3g}")
+    assert abs(loss - rl) <= 1e-3 * rl + 1e-7
+    assert rel_err(gp, rgp) < 2e-2
+    assert rel_err(gl, rgl) < 2e-2
+    # per block (W1, b1, W2, b2, W3, b3), so no block hides behind a larger one
+    for a, b in [(0, 704), (704, 768), (768, 4864), (4864, 4928), (4928, 5120), (5120, 5123)]:
+        assert rel_err(gp[a:b], rgp[a:b]) < 3e-2, (a, b)
+
+
+@pytest.mark.gpu
+def test_inference_colors_match_oracle(renderer, small_scene):
+    say("learned: inference colours vs oracle forward")
+    p, z, _ = renderer.learned_get()
+    eye = np.array([3.0, 2.0, 6.0], np.float32)
+    got = renderer.learned_colors(eye)
+    pos = small_scene.positions[:, :3].astype(np.float32)
+    d = pos - eye
+    d = d / np.sqrt((d * d).sum(axis=1, keepdims=True))
+    want, _ = LO.forward(p, z, d, bf16=True)
+    assert np.abs(got - want).max() < 4e-3  # f16 output + fast exp
+
+
+@pytest.mark.gpu
+def test_training_gains_over_dc_colour(renderer):
+    """Trained on the exact SH3 colours, the learned colours come closer to them than the
+    DC colour the reference uses, per splat (mean squared error over the 256 training
+    directions) and in rendered images (PSNR against the exact SH render).
+
+    With random SH coefficients (45 per splat, N(0, 0.1)) the view-dependent part holds
+    ~0.0119 of variance per channel; an 8-dim latent can carry only a few of those 45
+    dimensions, so the gain over the DC colour is real but small."""
+    say("learned: training and PSNR against exact SH")
+    raw = S.clustered(22, 4000, 8, sh_degree=3)
+    s = S.activate(raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    renderer.learned_init(seed=5)
+    losses = renderer.learned_train(40, lr=3e-3, lr_latent=3e-2)
+    dirs = LO.fibonacci_dirs()
+    t = LO.sh_targets(s.f_dc, s.sh_rest, 3, dirs)
+    dc_loss = float(((np.clip(0.5 + 0.28209479177387814 * s.f_dc.astype(np.float64), 0, 1)[:, None, :]
+                      - t) ** 2).mean())
+    say(f"  loss epoch 0 {losses[0]:.5f} -> epoch {len(losses) - 1} {losses[-1]:.5f}; DC colour {dc_loss:.5f}")
+    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.95 * dc_loss
+    views = [(cam_mod.view_camera(eye, w, h), w, h) for eye, w, h in S.orbit_views(9, 4, fixed=(320, 180))]
+
+    def images(learned):
+        renderer.rebin_batch(views, learned=learned)
+        return [renderer.batch_read(i, L.BUF_IMAGE_RGBA32F, np.float32).reshape(-1, 4)[:, :3]
+                for i in range(len(views))]
+
+    exact = np.concatenate(images(False))
+    psnr, enabled = renderer.learned_validate(views, 0.0)
+    assert enabled
+    learned = np.concatenate(images(True))
+    mse_l = float(((learned - exact) ** 2).mean())
+    assert abs(10 * math.log10(1 / mse_l) - psnr) < 0.05  # the library's PSNR is the same quantity
+    dc = S.Splats(s.positions, s.colors, s.scales, s.rotations, s.f_dc, None, 0, s.bounds)
+    renderer.set_splat_data(dc)
+    dc_img = np.concatenate(images(False))
+    psnr_dc = 10 * math.log10(1 / float(((dc_img - exact) ** 2).mean()))
+    say(f"  PSNR vs exact SH3: learned {psnr:.2f} dB, DC colour {psnr_dc:.2f} dB")
+    assert psnr > psnr_dc + 0.2
+    renderer.set_splat_data(s)
+
+
+@pytest.mark.gpu
+def test_fallback_to_exact_sh_below_threshold(renderer, small_scene):
+    """Below the PSNR threshold a learned batch renders exact SH; at or above it, learned colours."""
+    say("learned: fallback")
+    renderer.set_splat_data(small_scene)
+    renderer.learned_init(seed=3)
+    views = [(cam_mod.view_camera(eye, w, h), w, h) for eye, w, h in S.orbit_views(4, 2, fixed=(200, 120))]
+    renderer.rebin_batch(views)
+    exact = [renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16) for i in range(2)]
+    psnr, enabled = renderer.learned_validate(views, 200.0)
+    assert not enabled and psnr < 200.0
+    renderer.rebin_batch(views, learned=True)
+    for i in range(2):
+        assert np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), exact[i])
+    psnr2, enabled = renderer.learned_validate(views, -100.0)
+    assert enabled and abs(psnr2 - psnr) < 1e-3
+    renderer.rebin_batch(views, learned=True)
+    differs = any(not np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), exact[i])
+                  for i in range(2))
+    assert differs

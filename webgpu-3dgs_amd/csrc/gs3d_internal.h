@@ -95,6 +95,7 @@ constexpr uint32_t kViewsPerBlock = 8;
 struct MultiPreprocessArgs {
     const ViewParams* views;  // device array of n_views
     uint32_t n_views;
+    const uint2* view_colors;  // null, or learned colours [n_views][n] (f16 r, g, b, 1) used instead of SH
     const float4* pos;
     const float4* scale;
     const float4* rot;
@@ -212,6 +213,40 @@ hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_
                             uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
                             uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
+
+// ---- learned colour ---------------------------------------------------------------
+struct LearnedColorArgs {
+    const float* params;     // GS3D_LEARNED_PARAMS fp32
+    const uint16_t* latents; // [n][8] f16
+    const float4* pos;
+    const float* eyes;       // [n_views][4] camera positions
+    uint2* colors;           // [n_views][n] f16 (r, g, b, 1)
+    uint32_t n, n_views;
+};
+hipError_t launch_learned_colors(const LearnedColorArgs& a, hipStream_t s);
+
+struct LearnedTrainArgs {
+    const float* params;       // fp32 masters
+    const float* latents;      // [n][8] fp32 masters (rounded through f16 on input)
+    const float* sh;           // SoA SH coefficients of the splats
+    int sh_degree;
+    const float* dirs;         // [256][3] unit directions
+    const float* basis;        // [256][16] SH basis at each direction (zero-padded)
+    uint32_t first_dir;        // this step's directions: first_dir .. first_dir + 15
+    uint32_t n;
+    float grad_scale;          // d(loss)/d(sum of squared errors)
+    float* grad_partial;       // [waves][GS3D_LEARNED_PARAMS]
+    float* grad_latents;       // [n][8]
+    float* loss_partial;       // [waves] sums of squared errors
+};
+uint32_t learned_train_waves(uint32_t n);
+hipError_t launch_learned_train(const LearnedTrainArgs& a, uint32_t waves, hipStream_t s);
+hipError_t launch_learned_reduce(const float* partial, uint32_t rows, float* grad, const float* loss_partial,
+                                 float* loss, hipStream_t s);
+hipError_t launch_adam(float* x, float* m, float* v, const float* grad, size_t n, float lr, float b1, float b2,
+                       float eps, uint32_t step, uint16_t* f16_copy, hipStream_t s);
+// sum over pixels of the squared RGB differences of two f16 RGBA images (accumulated into *out, f64)
+hipError_t launch_sq_diff_sum(const uint16_t* a, const uint16_t* b, size_t pixels, double* out, hipStream_t s);
 
 // Composite of the f16 target over an unorm8 framebuffer, through the
 // splats-per-tile heat map when debug is set (then also written to

@@ -12,6 +12,8 @@
 // SH, coefficient-major SoA so every load is coalesced) in, 64 + 4 B out.
 // FMA contraction is off in this file so the float results equal the CPU
 // restatement operation for operation (same association as the WGSL).
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 
 #include "gs3d_internal.h"
@@ -288,6 +290,12 @@ __global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a)
         const ViewGeometry g{vp.cam, vp.tiles_x, vp.tiles_y, vp.inv_tile_x, vp.inv_tile_y};
         float4 o0, o1, o2, o3;
         const uint32_t count = project_splat<DEG>(g, p, r, o0, o1, o2, o3);
+        if (a.view_colors && count) {
+            const uint2 c = a.view_colors[(size_t)v * a.n + i];
+            o2.x = __half2float(__ushort_as_half((unsigned short)(c.x & 0xFFFFu)));
+            o2.y = __half2float(__ushort_as_half((unsigned short)(c.x >> 16)));
+            o2.z = __half2float(__ushort_as_half((unsigned short)(c.y & 0xFFFFu)));
+        }
         store_projected(vp.out, i, count, o0, o1, o2, o3, vp.depth_keys, vp.rects);
     }
 }
@@ -325,7 +333,9 @@ hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s) 
     const uint32_t threads = std::max(a.n, a.counter_words);
     if (threads == 0 || a.n_views == 0) return hipSuccess;
     const dim3 grid((threads + 255u) / 256u, (a.n_views + kViewsPerBlock - 1) / kViewsPerBlock);
-    switch (a.sh_degree) {
+    // learned colours replace the SH evaluation (the DC colour path is the
+    // cheapest projection; its colour is then overwritten)
+    switch (a.view_colors ? 0 : a.sh_degree) {
         case 0: hipLaunchKernelGGL(k_preprocess_views<0>, grid, dim3(256), 0, s, a); break;
         case 1: hipLaunchKernelGGL(k_preprocess_views<1>, grid, dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL(k_preprocess_views<2>, grid, dim3(256), 0, s, a); break;

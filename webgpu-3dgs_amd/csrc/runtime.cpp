@@ -101,6 +101,13 @@ struct gs3d_ctx {
     uint64_t stage_frames = 0;
 
     struct BatchState* batch = nullptr;  // gs3d_rebin_batch state, created on first use
+
+    // learned colour
+    bool learned_init = false;
+    bool learned_enabled = false;
+    bool learned_force = false;  // gs3d_learned_validate's learned render
+    uint32_t learned_n = 0;      // splats the latents belong to
+    uint32_t learned_step = 0;   // Adam steps taken
 };
 
 namespace {
@@ -189,6 +196,31 @@ template <class T>
 T* buf(gs3d_ctx* c, const char* name) {
     auto it = c->bufs.find(name);
     return it == c->bufs.end() ? nullptr : static_cast<T*>(it->second.ptr);
+}
+
+float half_to_float_host(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31, exp = (h >> 10) & 0x1Fu, man = h & 0x3FFu;
+    uint32_t bits;
+    if (exp == 0) {
+        if (man == 0) {
+            bits = sign;
+        } else {  // subnormal: renormalise
+            int e = -1;
+            uint32_t m = man;
+            do {
+                e++;
+                m <<= 1;
+            } while (!(m & 0x400u));
+            bits = sign | ((uint32_t)(127 - 15 - e) << 23) | ((m & 0x3FFu) << 13);
+        }
+    } else if (exp == 31) {
+        bits = sign | 0x7F800000u | (man << 13);
+    } else {
+        bits = sign | ((exp + 112u) << 23) | (man << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
 }
 
 uint32_t bit_length(uint32_t v) {
@@ -524,6 +556,7 @@ struct BatchState {
     std::vector<uint64_t> K;
     bool have = false;
     uint64_t replays = 0;
+    bool learned = false;  // the captured graph computes learned colours
 };
 
 void drop_graph(BatchState* b) {
@@ -597,6 +630,10 @@ int batch_buffers(gs3d_ctx* c, bool* moved) {
         {"batch_images", img * 2},
         {"batch_epoch", 4},
     };
+    if (b->learned) {
+        want.push_back({"batch_view_colors", V * (size_t)n * 8});
+        want.push_back({"batch_eyes", V * 16});
+    }
     for (int k = 0; k < kBatchBranches; k++) {
         const uint64_t cap = b->cap[k];
         want.push_back({branch_name(k, "depth_keys_b"), (size_t)n * 4});
@@ -677,6 +714,18 @@ int batch_enqueue(gs3d_ctx* c) {
     MultiPreprocessArgs ma{};
     ma.views = vparams;
     ma.n_views = V;
+    if (b->learned) {
+        LearnedColorArgs la{};
+        la.params = buf<float>(c, "learned_params");
+        la.latents = buf<uint16_t>(c, "learned_latents_f16");
+        la.pos = buf<float4>(c, "splat_positions");
+        la.eyes = buf<float>(c, "batch_eyes");
+        la.colors = buf<uint2>(c, "batch_view_colors");
+        la.n = n;
+        la.n_views = V;
+        HIP_TRY(c, launch_learned_colors(la, c->stream));
+        ma.view_colors = la.colors;
+    }
     ma.pos = buf<float4>(c, "splat_positions");
     ma.scale = buf<float4>(c, "splat_scales");
     ma.rot = buf<float4>(c, "splat_rotations");
@@ -821,8 +870,11 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
         b->views[i].branch = best;
         load[best] += (uint64_t)b->views[i].W * b->views[i].H;
     }
+    const bool learned = (flags & GS3D_BATCH_LEARNED) != 0 && c->learned_init && c->learned_n == c->n &&
+                         (c->learned_enabled || c->learned_force);
     const bool same = b->captured && shape == b->shape && b->n == c->n && b->deg == c->sh_degree &&
-                      b->tile == c->tile_x && b->bits == c->digit_bits && b->ppl == c->raster_ppl;
+                      b->tile == c->tile_x && b->bits == c->digit_bits && b->ppl == c->raster_ppl &&
+                      b->learned == learned;
     if (!same) {
         drop_graph(b);
         for (int k = 0; k < kBatchBranches; k++)
@@ -834,6 +886,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
     b->tile = c->tile_x;
     b->bits = c->digit_bits;
     b->ppl = c->raster_ppl;
+    b->learned = learned;
     b->have = false;
     const bool eager = (flags & GS3D_BATCH_EAGER) != 0;
     for (int attempt = 0; attempt < 3; attempt++) {
@@ -860,6 +913,14 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
         }
         HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_views"), vp.data(), vp.size() * sizeof(ViewParams),
                                   hipMemcpyHostToDevice, c->stream));
+        std::vector<float> eyes;
+        if (learned) {
+            eyes.resize((size_t)n_views * 4);
+            for (uint32_t i = 0; i < n_views; i++)
+                for (int k = 0; k < 4; k++) eyes[(size_t)i * 4 + k] = views[i].camera.position[k];
+            HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_eyes"), eyes.data(), eyes.size() * 4,
+                                      hipMemcpyHostToDevice, c->stream));
+        }
         if (eager) {
             GS3D_TRY(batch_enqueue(c));
         } else {
@@ -886,6 +947,106 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
         drop_graph(b);
     }
     return fail(c, GS3D_E_STATE, "batch key capacity did not settle");
+}
+
+// ---- learned colour ------------------------------------------------------------
+
+constexpr int kLearnedParams = GS3D_LEARNED_PARAMS;
+
+// The 256 training directions: a Fibonacci sphere (z evenly spaced, azimuth by
+// the golden angle), computed in f64 and stored as f32.
+void learned_directions(float* dirs) {
+    const double golden = 3.14159265358979323846 * (3.0 - std::sqrt(5.0));
+    for (int k = 0; k < GS3D_LEARNED_DIRS; k++) {
+        const double z = 1.0 - (2.0 * k + 1.0) / GS3D_LEARNED_DIRS;
+        const double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+        const double phi = golden * k;
+        dirs[3 * k] = (float)(r * std::cos(phi));
+        dirs[3 * k + 1] = (float)(r * std::sin(phi));
+        dirs[3 * k + 2] = (float)z;
+    }
+}
+
+// Real SH basis of degree 3 at a unit direction (the constants of the 3DGS
+// formulation, as in the preprocess kernel), in f64.
+void sh_basis_f64(double x, double y, double z, double* b) {
+    const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    b[0] = 0.28209479177387814;
+    b[1] = -0.4886025119029199 * y;
+    b[2] = 0.4886025119029199 * z;
+    b[3] = -0.4886025119029199 * x;
+    b[4] = 1.0925484305920792 * xy;
+    b[5] = -1.0925484305920792 * yz;
+    b[6] = 0.31539156525252005 * (2.0 * zz - xx - yy);
+    b[7] = -1.0925484305920792 * xz;
+    b[8] = 0.5462742152960396 * (xx - yy);
+    b[9] = -0.5900435899266435 * y * (3.0 * xx - yy);
+    b[10] = 2.890611442640554 * xy * z;
+    b[11] = -0.4570457994644658 * y * (4.0 * zz - xx - yy);
+    b[12] = 0.3731763325901154 * z * (2.0 * zz - 3.0 * xx - 3.0 * yy);
+    b[13] = -0.4570457994644658 * x * (4.0 * zz - xx - yy);
+    b[14] = 1.445305721320277 * z * (xx - yy);
+    b[15] = -0.5900435899266435 * x * (xx - 3.0 * yy);
+}
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double normal01(uint64_t& s) {
+    const double u1 = ((splitmix(s) >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+    const double u2 = (splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+int learned_require(gs3d_ctx* c) {
+    if (!c->learned_init) return fail(c, GS3D_E_STATE, "learned colour not initialised (gs3d_learned_init)");
+    if (c->learned_n != c->n) return fail(c, GS3D_E_STATE, "splats changed since gs3d_learned_init");
+    return GS3D_OK;
+}
+
+int learned_buffers(gs3d_ctx* c) {
+    const size_t n8 = (size_t)c->n * 8;
+    const uint32_t waves = learned_train_waves(c->n);
+    const std::pair<const char*, size_t> want[] = {
+        {"learned_params", kLearnedParams * 4}, {"learned_pm", kLearnedParams * 4}, {"learned_pv", kLearnedParams * 4},
+        {"learned_grad", kLearnedParams * 4},   {"learned_partial", (size_t)waves * kLearnedParams * 4},
+        {"learned_loss_partial", waves * 4u},   {"learned_loss", 16 * 4},
+        {"learned_latents", n8 * 4},            {"learned_lm", n8 * 4},  {"learned_lv", n8 * 4},
+        {"learned_lgrad", n8 * 4},              {"learned_latents_f16", n8 * 2},
+        {"learned_dirs", GS3D_LEARNED_DIRS * 3 * 4}, {"learned_basis", GS3D_LEARNED_DIRS * 16 * 4},
+    };
+    for (const auto& w : want) {
+        void* p;
+        GS3D_TRY(ensure(c, w.first, w.second, &p));
+    }
+    return GS3D_OK;
+}
+
+// One forward/backward over all splats x directions [first, first + 16):
+// gradients into learned_grad / learned_lgrad, the sum of squared errors into
+// learned_loss[slot].
+int learned_step_grads(gs3d_ctx* c, uint32_t first_dir, uint32_t slot) {
+    LearnedTrainArgs ta{};
+    ta.params = buf<float>(c, "learned_params");
+    ta.latents = buf<float>(c, "learned_latents");
+    ta.sh = c->sh_degree > 0 ? buf<float>(c, "splat_sh") : nullptr;
+    ta.sh_degree = c->sh_degree;
+    ta.dirs = buf<float>(c, "learned_dirs");
+    ta.basis = buf<float>(c, "learned_basis");
+    ta.first_dir = first_dir;
+    ta.n = c->n;
+    ta.grad_scale = 1.0f / (3.0f * 16.0f * (float)std::max(1u, c->n));
+    ta.grad_partial = buf<float>(c, "learned_partial");
+    ta.grad_latents = buf<float>(c, "learned_lgrad");
+    ta.loss_partial = buf<float>(c, "learned_loss_partial");
+    const uint32_t waves = learned_train_waves(c->n);
+    HIP_TRY(c, launch_learned_train(ta, waves, c->stream));
+    HIP_TRY(c, launch_learned_reduce(ta.grad_partial, waves, buf<float>(c, "learned_grad"), ta.loss_partial,
+                                     buf<float>(c, "learned_loss") + slot, c->stream));
+    return GS3D_OK;
 }
 
 // Used by ply.cpp, which has no view of the context's layout.
@@ -1623,6 +1784,204 @@ int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes,
     }
     HIP_TRY(c, hipMemcpyAsync(dst, src, size, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_learned_dirs(float* dirs) {
+    if (!dirs) return GS3D_E_INVALID_ARG;
+    learned_directions(dirs);
+    return GS3D_OK;
+}
+
+int gs3d_learned_init(gs3d_ctx* c, uint64_t seed) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (!c->bufs.count("splat_positions")) return fail(c, GS3D_E_STATE, "no splats uploaded");
+    if (c->sh_degree == 0) return fail(c, GS3D_E_STATE, "learned colour needs SH coefficients (degree >= 1)");
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    c->learned_n = c->n;
+    GS3D_TRY(learned_buffers(c));
+    uint64_t st = seed * 0x2545F4914F6CDD1Dull + 1;
+    std::vector<float> p(kLearnedParams, 0.0f);
+    const double s1 = std::sqrt(2.0 / 11.0), s2 = std::sqrt(2.0 / 64.0), s3 = std::sqrt(1.0 / 64.0);
+    for (int i = 0; i < 64 * 11; i++) p[i] = (float)(s1 * normal01(st));
+    for (int i = 0; i < 64 * 64; i++) p[64 * 11 + 64 + i] = (float)(s2 * normal01(st));
+    for (int i = 0; i < 3 * 64; i++) p[64 * 11 + 64 + 64 * 64 + 64 + i] = (float)(s3 * normal01(st));
+    std::vector<float> z((size_t)c->n * 8);
+    for (auto& v : z) v = (float)(0.1 * normal01(st));
+    std::vector<float> dirs(GS3D_LEARNED_DIRS * 3), basis(GS3D_LEARNED_DIRS * 16);
+    learned_directions(dirs.data());
+    for (int k = 0; k < GS3D_LEARNED_DIRS; k++) {
+        double b[16];
+        sh_basis_f64(dirs[3 * k], dirs[3 * k + 1], dirs[3 * k + 2], b);
+        for (int i = 0; i < 16; i++) basis[16 * k + i] = (float)b[i];
+    }
+    HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "learned_params"), p.data(), p.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "learned_latents"), z.data(), z.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "learned_dirs"), dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "learned_basis"), basis.data(), basis.size() * 4, hipMemcpyHostToDevice, c->stream));
+    for (const char* name : {"learned_pm", "learned_pv"})
+        HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, name), 0u, kLearnedParams, c->stream));
+    for (const char* name : {"learned_lm", "learned_lv"})
+        HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, name), 0u, (size_t)c->n * 8, c->stream));
+    // f16 copy of the latents: an Adam step with zero learning rate writes it
+    HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, "learned_lgrad"), 0u, (size_t)c->n * 8, c->stream));
+    HIP_TRY(c, launch_adam(buf<float>(c, "learned_latents"), buf<float>(c, "learned_lm"), buf<float>(c, "learned_lv"),
+                           buf<float>(c, "learned_lgrad"), (size_t)c->n * 8, 0.0f, 0.9f, 0.999f, 1e-8f, 1,
+                           buf<uint16_t>(c, "learned_latents_f16"), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->learned_init = true;
+    c->learned_enabled = false;
+    c->learned_step = 0;
+    return GS3D_OK;
+}
+
+int gs3d_learned_train(gs3d_ctx* c, uint32_t epochs, float lr, float lr_latent, float* loss, uint32_t n_loss) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    const size_t n8 = (size_t)c->n * 8;
+    for (uint32_t e = 0; e < epochs; e++) {
+        for (uint32_t s = 0; s < GS3D_LEARNED_DIRS / 16; s++) {
+            GS3D_TRY(learned_step_grads(c, 16 * s, s));
+            c->learned_step++;
+            HIP_TRY(c, launch_adam(buf<float>(c, "learned_params"), buf<float>(c, "learned_pm"),
+                                   buf<float>(c, "learned_pv"), buf<float>(c, "learned_grad"), kLearnedParams, lr,
+                                   0.9f, 0.999f, 1e-8f, c->learned_step, nullptr, c->stream));
+            HIP_TRY(c, launch_adam(buf<float>(c, "learned_latents"), buf<float>(c, "learned_lm"),
+                                   buf<float>(c, "learned_lv"), buf<float>(c, "learned_lgrad"), n8, lr_latent, 0.9f,
+                                   0.999f, 1e-8f, c->learned_step, buf<uint16_t>(c, "learned_latents_f16"), c->stream));
+        }
+        if (loss && e < n_loss) {
+            float l[16];
+            HIP_TRY(c, hipMemcpyAsync(l, buf<float>(c, "learned_loss"), sizeof l, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            double t = 0;
+            for (float v : l) t += v;
+            loss[e] = (float)(t / (3.0 * GS3D_LEARNED_DIRS * std::max(1u, c->n)));
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->learned_enabled = false;  // parameters changed: validate again
+    return GS3D_OK;
+}
+
+int gs3d_learned_grad(gs3d_ctx* c, uint32_t first_dir, float* grad_params, float* grad_latents, float* loss) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (first_dir % 16 || first_dir + 16 > GS3D_LEARNED_DIRS) return fail(c, GS3D_E_INVALID_ARG, "first_dir %u", first_dir);
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    GS3D_TRY(learned_step_grads(c, first_dir, 0));
+    float l = 0.f;
+    if (grad_params)
+        HIP_TRY(c, hipMemcpyAsync(grad_params, buf<float>(c, "learned_grad"), kLearnedParams * 4, hipMemcpyDeviceToHost, c->stream));
+    if (grad_latents)
+        HIP_TRY(c, hipMemcpyAsync(grad_latents, buf<float>(c, "learned_lgrad"), (size_t)c->n * 32, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(&l, buf<float>(c, "learned_loss"), 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l / (3.0f * 16.0f * (float)std::max(1u, c->n));
+    return GS3D_OK;
+}
+
+int gs3d_learned_get(gs3d_ctx* c, float* params, float* latents, uint16_t* latents_f16) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    if (params) HIP_TRY(c, hipMemcpyAsync(params, buf<float>(c, "learned_params"), kLearnedParams * 4, hipMemcpyDeviceToHost, c->stream));
+    if (latents) HIP_TRY(c, hipMemcpyAsync(latents, buf<float>(c, "learned_latents"), (size_t)c->n * 32, hipMemcpyDeviceToHost, c->stream));
+    if (latents_f16) HIP_TRY(c, hipMemcpyAsync(latents_f16, buf<uint16_t>(c, "learned_latents_f16"), (size_t)c->n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_learned_set(gs3d_ctx* c, const float* params, const float* latents) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    if (params) HIP_TRY(c, hipMemcpyAsync(buf<float>(c, "learned_params"), params, kLearnedParams * 4, hipMemcpyHostToDevice, c->stream));
+    if (latents) {
+        HIP_TRY(c, hipMemcpyAsync(buf<float>(c, "learned_latents"), latents, (size_t)c->n * 32, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, "learned_lgrad"), 0u, (size_t)c->n * 8, c->stream));
+        HIP_TRY(c, launch_adam(buf<float>(c, "learned_latents"), buf<float>(c, "learned_lm"), buf<float>(c, "learned_lv"),
+                               buf<float>(c, "learned_lgrad"), (size_t)c->n * 8, 0.0f, 0.9f, 0.999f, 1e-8f, 1,
+                               buf<uint16_t>(c, "learned_latents_f16"), c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->learned_enabled = false;
+    return GS3D_OK;
+}
+
+int gs3d_learned_colors(gs3d_ctx* c, const float eye[3], float* rgb) {
+    if (!c || !eye || !rgb) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    float* eyes;
+    uint2* colors;
+    GS3D_TRY(ensure_t(c, "learned_probe_eye", 4, &eyes));
+    GS3D_TRY(ensure_t(c, "learned_probe_colors", c->n, &colors));
+    const float e4[4] = {eye[0], eye[1], eye[2], 1.0f};
+    HIP_TRY(c, hipMemcpyAsync(eyes, e4, 16, hipMemcpyHostToDevice, c->stream));
+    LearnedColorArgs la{};
+    la.params = buf<float>(c, "learned_params");
+    la.latents = buf<uint16_t>(c, "learned_latents_f16");
+    la.pos = buf<float4>(c, "splat_positions");
+    la.eyes = eyes;
+    la.colors = colors;
+    la.n = c->n;
+    la.n_views = 1;
+    HIP_TRY(c, launch_learned_colors(la, c->stream));
+    std::vector<uint16_t> h((size_t)c->n * 4);
+    HIP_TRY(c, hipMemcpyAsync(h.data(), colors, h.size() * 2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->n; i++)
+        for (int k = 0; k < 3; k++) rgb[i * 3 + k] = half_to_float_host(h[i * 4 + k]);
+    return GS3D_OK;
+}
+
+int gs3d_learned_enable(gs3d_ctx* c, int on) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (on) GS3D_TRY(learned_require(c));
+    c->learned_enabled = on != 0;
+    return GS3D_OK;
+}
+
+int gs3d_learned_validate(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, float min_psnr_db, float* psnr,
+                          int* enabled) {
+    if (!c || !views || n_views == 0) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(gs3d_rebin_batch(c, views, n_views, 0));
+    BatchState* b = c->batch;
+    size_t img_elems = 0;
+    for (const auto& v : b->views) img_elems = std::max(img_elems, v.img_off + (size_t)v.W * v.H * 4);
+    uint16_t* ref;
+    GS3D_TRY(ensure_t(c, "learned_ref_images", img_elems, &ref));
+    HIP_TRY(c, hipMemcpyAsync(ref, buf<uint16_t>(c, "batch_images"), img_elems * 2, hipMemcpyDeviceToDevice, c->stream));
+    c->learned_force = true;
+    const int rc = gs3d_rebin_batch(c, views, n_views, GS3D_BATCH_LEARNED);
+    c->learned_force = false;
+    GS3D_TRY(rc);
+    double* acc;
+    GS3D_TRY(ensure_t(c, "learned_sq_sum", 1, &acc));
+    HIP_TRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(acc), 0u, 2, c->stream));
+    size_t px = 0;
+    for (const auto& v : b->views) {
+        HIP_TRY(c, launch_sq_diff_sum(ref + v.img_off, buf<uint16_t>(c, "batch_images") + v.img_off,
+                                      (size_t)v.W * v.H, acc, c->stream));
+        px += (size_t)v.W * v.H;
+    }
+    double sq = 0;
+    HIP_TRY(c, hipMemcpyAsync(&sq, acc, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const double mse = sq / (3.0 * (double)std::max<size_t>(1, px));
+    const double p = mse > 0 ? 10.0 * std::log10(1.0 / mse) : 200.0;
+    if (psnr) *psnr = (float)p;
+    c->learned_enabled = p >= (double)min_psnr_db;
+    if (enabled) *enabled = c->learned_enabled ? 1 : 0;
     return GS3D_OK;
 }
 

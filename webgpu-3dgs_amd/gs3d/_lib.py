@@ -37,6 +37,9 @@ FORMAT_BGRA8UNORM = 1
 PLY_ASCII, PLY_BINARY_LE, PLY_BINARY_BE = 0, 1, 2
 
 BATCH_EAGER = 1
+BATCH_LEARNED = 2
+LEARNED_PARAMS = 5123
+LEARNED_DIRS = 256
 MAX_BATCH_VIEWS = 1024
 # gs3d_view: 72-float camera uniform + width + height
 VIEW_DTYPE = [("camera", "<f4", (72,)), ("width", "<u4"), ("height", "<u4")]
@@ -97,6 +100,15 @@ _PROTOS = {
     "gs3d_rebin_batch": (I32, [P, P, U32, U32]),
     "gs3d_batch_ref_counts": (I32, [P, P, U32]),
     "gs3d_batch_read": (I32, [P, U32, I32, P, SZ, C.POINTER(SZ)]),
+    "gs3d_learned_init": (I32, [P, C.c_uint64]),
+    "gs3d_learned_train": (I32, [P, U32, F32, F32, P, U32]),
+    "gs3d_learned_grad": (I32, [P, U32, P, P, C.POINTER(F32)]),
+    "gs3d_learned_get": (I32, [P, P, P, P]),
+    "gs3d_learned_set": (I32, [P, P, P]),
+    "gs3d_learned_dirs": (I32, [P]),
+    "gs3d_learned_colors": (I32, [P, P, P]),
+    "gs3d_learned_validate": (I32, [P, P, U32, F32, C.POINTER(F32), C.POINTER(I32)]),
+    "gs3d_learned_enable": (I32, [P, I32]),
     "gs3d_ply_probe": (I32, [P, SZ, P, C.c_char_p, SZ]),
     "gs3d_ply_parse": (I32, [P, SZ, P, P, P, P, P, P, I32, P, C.c_char_p, SZ]),
     "gs3d_load_ply": (I32, [P, C.c_char_p, I32, P]),

@@ -139,8 +139,8 @@ class GaussianSplatRenderer:
         return out
 
     # -- batched views ----------------------------------------------------------
-    def rebin_batch(self, views, eager: bool = False) -> None:
-        """Render many views: `views` is a list of (camera uniform (72 f32), width, height)."""
+    @staticmethod
+    def _views(views) -> np.ndarray:
         arr = np.zeros(len(views), dtype=L.VIEW_DTYPE)
         for i, (cam, w, h) in enumerate(views):
             cam = np.asarray(cam, np.float32).reshape(-1)
@@ -149,8 +149,21 @@ class GaussianSplatRenderer:
             arr[i]["camera"] = cam
             arr[i]["width"] = w
             arr[i]["height"] = h
-        self._check(self._lib.gs3d_rebin_batch(self._h, L.ptr(arr), len(views),
-                                               L.BATCH_EAGER if eager else 0))
+        return arr
+
+    def rebin_batch(self, views, eager: bool = False, learned: bool = False) -> None:
+        """Render many views: `views` is a list of (camera uniform (72 f32), width, height).
+        learned: colour with the learned stage when it is enabled (else exact SH)."""
+        arr = np.zeros(len(views), dtype=L.VIEW_DTYPE)
+        for i, (cam, w, h) in enumerate(views):
+            cam = np.asarray(cam, np.float32).reshape(-1)
+            if cam.size != 72:
+                raise ValueError("camera uniform must hold 72 floats")
+            arr[i]["camera"] = cam
+            arr[i]["width"] = w
+            arr[i]["height"] = h
+        flags = (L.BATCH_EAGER if eager else 0) | (L.BATCH_LEARNED if learned else 0)
+        self._check(self._lib.gs3d_rebin_batch(self._h, L.ptr(arr), len(views), flags))
         self._batch_views = [(int(w), int(h)) for _, w, h in views]
 
     def batch_ref_counts(self) -> np.ndarray:
@@ -165,6 +178,59 @@ class GaussianSplatRenderer:
         self._check(self._lib.gs3d_batch_read(self._h, view, buffer_id, L.ptr(out), out.nbytes,
                                               C.byref(size)))
         return out
+
+    # -- learned colour -------------------------------------------------------------
+    def learned_init(self, seed: int = 0) -> None:
+        self._check(self._lib.gs3d_learned_init(self._h, seed))
+
+    def learned_train(self, epochs: int, lr: float = 1e-3, lr_latent: float = 1e-2) -> np.ndarray:
+        """Train; returns the mean squared error of each epoch."""
+        loss = np.zeros(max(1, epochs), np.float32)
+        self._check(self._lib.gs3d_learned_train(self._h, epochs, lr, lr_latent, L.ptr(loss), loss.size))
+        return loss[:epochs]
+
+    def learned_grad(self, first_dir: int):
+        gp = np.zeros(L.LEARNED_PARAMS, np.float32)
+        gl = np.zeros((self.splat_count, 8), np.float32)
+        loss = C.c_float()
+        self._check(self._lib.gs3d_learned_grad(self._h, first_dir, L.ptr(gp), L.ptr(gl), C.byref(loss)))
+        return float(loss.value), gp, gl
+
+    def learned_get(self):
+        p = np.zeros(L.LEARNED_PARAMS, np.float32)
+        z = np.zeros((self.splat_count, 8), np.float32)
+        z16 = np.zeros((self.splat_count, 8), np.uint16)
+        self._check(self._lib.gs3d_learned_get(self._h, L.ptr(p), L.ptr(z), L.ptr(z16)))
+        return p, z, z16
+
+    def learned_set(self, params=None, latents=None) -> None:
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        z = None if latents is None else np.ascontiguousarray(latents, np.float32)
+        self._check(self._lib.gs3d_learned_set(self._h, L.ptr(p) if p is not None else None,
+                                               L.ptr(z) if z is not None else None))
+
+    @staticmethod
+    def learned_dirs() -> np.ndarray:
+        d = np.zeros((L.LEARNED_DIRS, 3), np.float32)
+        L.lib().gs3d_learned_dirs(L.ptr(d))
+        return d
+
+    def learned_colors(self, eye) -> np.ndarray:
+        e = np.asarray(eye, np.float32)
+        out = np.zeros((self.splat_count, 3), np.float32)
+        self._check(self._lib.gs3d_learned_colors(self._h, L.ptr(e), L.ptr(out)))
+        return out
+
+    def learned_validate(self, views, min_psnr_db: float = 45.0):
+        arr = self._views(views)
+        psnr, en = C.c_float(), C.c_int()
+        self._check(self._lib.gs3d_learned_validate(self._h, L.ptr(arr), len(views), min_psnr_db,
+                                                    C.byref(psnr), C.byref(en)))
+        self._batch_views = [(int(w), int(h)) for _, w, h in views]
+        return float(psnr.value), bool(en.value)
+
+    def learned_enable(self, on: bool) -> None:
+        self._check(self._lib.gs3d_learned_enable(self._h, 1 if on else 0))
 
     # -- display: heat map + composite -----------------------------------------
     def set_debug_mode(self, mode: int, debug_ref: int = 256) -> None:
