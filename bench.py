@@ -94,8 +94,9 @@ def cpu_baseline(cam, splats, W, H, tile, seconds):
                       f"{threads} OpenMP threads, {el:.1f} s)"}, visited, blended
 
 
-def bench_batched(steps: int, warmup: int) -> dict:
-    """views/s of the 64-view batch (2M splats, SH3, uneven resolutions)."""
+def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
+    """views/s of the 64-view batch (2M splats, SH3, uneven resolutions) at 16x16 tiles,
+    1 pixel per lane and 8-bit digits; then the autotuner's pick and its views/s."""
     from gs3d import camera, scene
     from gs3d.renderer import GaussianSplatRenderer
 
@@ -135,6 +136,19 @@ def bench_batched(steps: int, warmup: int) -> dict:
     one_by_one()
     per_s = timed(one_by_one, 1)
     out["single_view_rebins_views_per_s"] = round(len(views) / per_s, 2)
+    if tune:
+        from gs3d import autotune
+
+        best, results = autotune.tune_batch(r, views, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8, 4),
+                                            warmup=1, reps=3)
+        out["autotune"] = {"best": {"tile": best.tile, "pixels_per_lane": best.pixels_per_lane,
+                                    "radix_bits": best.radix_bits, "views_per_s": round(best.views_per_s, 2)},
+                           "candidates": [[c.tile, c.pixels_per_lane, c.radix_bits, round(c.views_per_s, 1)]
+                                          for c in results]}
+        per_t = timed(lambda: r.rebin_batch(views), steps)
+        out["tuned_views_per_s"] = round(len(views) / per_t, 2)
+        log(f"[bench] batch autotune: best tile {best.tile}, ppl {best.pixels_per_lane}, "
+            f"{best.radix_bits}-bit digits: {out['tuned_views_per_s']} views/s")
     log(f"[bench] batched: {out['value']} views/s (graph), {out['eager_views_per_s']} eager, "
         f"{out['single_view_rebins_views_per_s']} one rebin per view; K total {out['keys_total']}")
     r.close()

@@ -130,3 +130,25 @@ def test_batch_errors(renderer, scene_sh3):
     size = __import__("ctypes").c_size_t()
     assert lib.gs3d_batch_read(renderer._h, 5, L.BUF_IMAGE_RGBA16F, None, 0, size) == -1
     assert lib.gs3d_batch_read(renderer._h, 0, L.BUF_SORT_KEYS, None, 0, size) == -6
+
+
+def test_batch_autotuner_picks_the_fastest_candidate(renderer, scene_sh3):
+    """tune_batch times every (tile, pixels per lane, digit width) candidate on whole
+    batches, applies the fastest, and the batch still equals single views there."""
+    from gs3d import autotune
+
+    say("batch: autotuner")
+    renderer.set_splat_data(scene_sh3)
+    views = views_for(3, 4)
+    try:
+        best, results = autotune.tune_batch(renderer, views, tiles=(8, 16), ppls=(1, 2), radix_bits=(8, 4),
+                                            warmup=1, reps=2)
+        assert len(results) == 6  # 8x8 tiles take only 1 pixel per lane
+        assert best.views_per_s == max(r.views_per_s for r in results)
+        assert renderer.get_tile_size() == (best.tile, best.tile)
+        assert renderer.get_kernel_variant("radix-sort", "RADIX_BITS") == best.radix_bits
+        check_batch(renderer, views)
+    finally:
+        renderer.set_tile_size(16, 16)
+        renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 1)
+        renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)

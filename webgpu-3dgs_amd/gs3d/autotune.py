@@ -2,16 +2,19 @@
 
 Knobs (each a compiled kernel variant): binning/raster tile size {8, 16, 32},
 rasterizer pixels per lane {1, 2, 4} (a wavefront covers 64*ppl pixels of one
-tile) and radix digit width {4, 8}. A candidate is timed as the mean GPU time of
-whole rebins (hipEvent pairs inside the library) over the given views after a
-warm-up; the fastest wins. Tile size changes the binning and so the image;
-callers that must keep a given tile size pass `tiles=(t,)`.
+tile) and radix digit width {4, 8}. `tune` times a candidate as the mean GPU
+time of whole single-view rebins (hipEvent pairs inside the library) over the
+given views after a warm-up; `tune_batch` times whole batched calls (one
+replayed hipGraph over all the views, wall clock). The fastest wins. Tile size
+changes the binning and so the image; callers that must keep a given tile size
+pass `tiles=(t,)`.
 """
 from __future__ import annotations
 
 import itertools
 import json
-from dataclasses import asdict, dataclass
+import time
+from dataclasses import asdict, dataclass, field
 from pathlib import Path
 
 from . import camera as cam_mod
@@ -56,6 +59,37 @@ def tune(renderer, views, *, tiles=(8, 16, 32), ppls=(1, 2, 4), radix_bits=(8, 4
         results.append(Choice(t, p, b, stages["rebin-total"], stages))
     best = min(results, key=lambda c: c.frame_us)
     apply(renderer, best)
+    return best, results
+
+
+@dataclass
+class BatchChoice:
+    tile: int
+    pixels_per_lane: int
+    radix_bits: int
+    views_per_s: float
+    ms_per_batch: float
+    extra: dict = field(default_factory=dict)
+
+
+def tune_batch(renderer, cams, *, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8, 4), warmup=2,
+               reps=4, learned=False) -> tuple[BatchChoice, list[BatchChoice]]:
+    """Time every candidate on gs3d_rebin_batch over `cams` [(camera uniform, W, H), ...]
+    (a re-capture per candidate, then `reps` timed replays); returns (best, all)."""
+    results = []
+    for t, p, b in candidates(tiles, ppls, radix_bits):
+        apply(renderer, Choice(t, p, b, 0.0, {}))
+        for _ in range(warmup):
+            renderer.rebin_batch(cams, learned=learned)
+        renderer.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            renderer.rebin_batch(cams, learned=learned)
+        renderer.sync()
+        per = (time.perf_counter() - t0) / reps
+        results.append(BatchChoice(t, p, b, len(cams) / per, per * 1e3))
+    best = max(results, key=lambda c: c.views_per_s)
+    apply(renderer, Choice(best.tile, best.pixels_per_lane, best.radix_bits, 0.0, {}))
     return best, results
 
 
