@@ -212,6 +212,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--key-tile", type=int, default=0, help="keys per workgroup of the 64-bit sort passes "
+                    "(0 = chosen by key count, 8192, 16384)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batch measurement")
@@ -252,6 +254,7 @@ def main() -> int:
     r = GaussianSplatRenderer(0)
     r.set_splat_data(splats)
     r.set_tile_size(args.tile, args.tile)
+    r.set_kernel_variant("radix-sort", "KEY_TILE", args.key_tile)
     for _ in range(args.warmup):
         r.rebin(cu, W, H)
     r.sync()

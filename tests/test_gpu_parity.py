@@ -368,3 +368,18 @@ def test_composite_after_rebin(renderer):
     # resident framebuffer: blending the same frame onto the previous result again
     again = renderer.composite(W, H, clear=None)
     assert np.array_equal(again, O.composite(src, W, H, framebuffer=got, bgra=False))
+
+
+@pytest.mark.parametrize("key_tile", [8192, 16384])
+@pytest.mark.parametrize("n", [1, 8191, 8193, 100_003, 3_400_000])
+def test_sort_key_tile_shapes_bit_exact(renderer, n, key_tile):
+    """Both tile shapes of the 64-bit key pass give the oracle's stable order."""
+    say(f"sort key_tile={key_tile} n={n}")
+    keys, vals = make_keys(n, n + key_tile)
+    renderer.set_kernel_variant("radix-sort", "KEY_TILE", key_tile)
+    try:
+        ko, vo = renderer.stage_sort(keys, vals, 8)
+    finally:
+        renderer.set_kernel_variant("radix-sort", "KEY_TILE", 0)
+    order = np.lexsort((np.arange(n), keys))
+    assert np.array_equal(ko, keys[order]) and np.array_equal(vo, vals[order])

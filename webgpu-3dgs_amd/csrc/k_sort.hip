@@ -76,7 +76,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
     const uint32_t* __restrict__ hist_pass, uint64_t* status, uint32_t* ticket, uint32_t n_max,
     uint32_t shift, uint32_t epoch_arg, const uint32_t* __restrict__ aux_in,
-    uint32_t* __restrict__ aux_out, const uint32_t* __restrict__ epoch_base) {
+    uint32_t* __restrict__ aux_out, const uint32_t* __restrict__ epoch_base, uint32_t match_bits) {
     const uint32_t epoch = resolve_epoch(epoch_arg, epoch_base);
     constexpr int RADIX = 1 << BITS;
     constexpr int WAVES = THREADS / kWave;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         const size_t idx = wave_base + (size_t)it * kWave + lane;
         const bool valid = idx < n;
         const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
-        const uint64_t peers = match_digit<BITS>(d, valid);
+        const uint64_t peers = match_digit<BITS>(d, valid, match_bits);
         uint32_t r = 0;
         if (valid) {
             const uint64_t below = peers & lt_mask;
@@ -263,22 +263,46 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
 
 uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / kSortTile + 1; }
 
-// Tile shapes (threads x keys per thread) for 64-bit and 32-bit keys.
+// Tile shapes (threads x keys per thread). 64-bit keys: 16,384-key tiles
+// (145 KB of LDS, one workgroup per CU) keep the lookback chain short while
+// the keys are few; 8,192-key tiles (73 KB, two workgroups per CU, so one
+// loads while the other scatters) when there are many. 32-bit keys: 8,192.
 constexpr int kSort64Threads = 1024, kSort64Items = 16;
+constexpr int kSort64SmallThreads = 512, kSort64SmallItems = 16;
 constexpr int kSort32Threads = 1024, kSort32Items = 8;
 
-template <int BITS, typename KeyT, bool IOTA, bool AUX>
+template <int BITS, typename KeyT, bool IOTA, bool AUX, int THREADS, int ITEMS>
 static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                         const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass,
                         uint64_t* status, uint32_t* ticket, uint32_t shift, uint32_t epoch,
                         const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
-                        const uint32_t* epoch_base) {
-    constexpr int kThreads = sizeof(KeyT) == 8 ? kSort64Threads : kSort32Threads;
-    constexpr int kItems = sizeof(KeyT) == 8 ? kSort64Items : kSort32Items;
-    const uint32_t grid = (n_max + kThreads * kItems - 1) / (kThreads * kItems);
-    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA, kThreads, kItems, AUX>), dim3(grid), dim3(kThreads),
+                        const uint32_t* epoch_base, uint32_t match_bits) {
+    const uint32_t grid = (n_max + THREADS * ITEMS - 1) / (THREADS * ITEMS);
+    hipLaunchKernelGGL((k_radix_pass<BITS, KeyT, IOTA, THREADS, ITEMS, AUX>), dim3(grid), dim3(THREADS),
                        0, s, static_cast<const KeyT*>(kin), vin, static_cast<KeyT*>(kout), vout, n_dev,
-                       hist_pass, status, ticket, n_max, shift, epoch, aux_in, aux_out, epoch_base);
+                       hist_pass, status, ticket, n_max, shift, epoch, aux_in, aux_out, epoch_base, match_bits);
+}
+
+template <int BITS, typename KeyT, bool IOTA, bool AUX>
+static void launch_shape(bool small_tiles, const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
+                         const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
+                         uint32_t* ticket, uint32_t shift, uint32_t epoch, const uint32_t* aux_in,
+                         uint32_t* aux_out, hipStream_t s, const uint32_t* epoch_base, uint32_t match_bits) {
+    if constexpr (sizeof(KeyT) == 8) {
+        if (small_tiles)
+            launch_pass<BITS, KeyT, IOTA, AUX, kSort64SmallThreads, kSort64SmallItems>(
+                kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
+                epoch_base, match_bits);
+        else
+            launch_pass<BITS, KeyT, IOTA, AUX, kSort64Threads, kSort64Items>(
+                kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
+                epoch_base, match_bits);
+    } else {
+        (void)small_tiles;
+        launch_pass<BITS, KeyT, IOTA, AUX, kSort32Threads, kSort32Items>(
+            kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
+            epoch_base, match_bits);
+    }
 }
 
 hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
@@ -286,16 +310,19 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
                              const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
-                             const uint32_t* epoch_base) {
+                             const uint32_t* epoch_base, uint32_t key_tile, uint32_t match_bits) {
     if (n_max == 0) return hipSuccess;
+    if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
+    if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
+    const bool small_tiles = key_tile == 8192 || (key_tile == 0 && n_max >= kSortSmallTileFrom);
     const bool aux = aux_in != nullptr && aux_out != nullptr;
     // the gather rides on 32-bit key passes whose values are read from vin
     if (aux && (key_bytes != 4 || iota_values)) return hipErrorInvalidValue;
     const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
                     (iota_values ? 1 : 0) + (aux ? 2 : 0);
 #define GS3D_PASS(B, K, I, A)                                                                      \
-    launch_pass<B, K, I, A>(kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, \
-                            aux_in, aux_out, s, epoch_base)
+    launch_shape<B, K, I, A>(small_tiles, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, \
+                             epoch, aux_in, aux_out, s, epoch_base, match_bits)
     switch (key) {
         case 110: GS3D_PASS(8, uint64_t, false, false); break;
         case 111: GS3D_PASS(8, uint64_t, true, false); break;

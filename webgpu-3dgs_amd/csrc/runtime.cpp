@@ -67,6 +67,7 @@ struct gs3d_ctx {
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
     uint32_t raster_ppl = 1;
+    uint32_t sort_key_tile = 0;  // 64-bit key passes: 0 = by key count, or 8192 / 16384
     uint64_t key_capacity = 0;
     uint32_t n_passes = 0;
     uint32_t shifts[kMaxPasses] = {};
@@ -282,7 +283,7 @@ int set_device(gs3d_ctx* c) {
 int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint32_t key_bytes,
                bool first_iota, const uint32_t* n_dev, uint64_t n_max, const uint32_t* shifts,
                uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b,
-               const uint32_t* aux_in = nullptr, uint32_t* aux_out = nullptr) {
+               const uint32_t* aux_in = nullptr, uint32_t* aux_out = nullptr, uint32_t key_bits = 64) {
     uint64_t* radix_status = nullptr;
     const uint32_t radix = 1u << c->digit_bits;
     GS3D_TRY(ensure_status(c, "radix_status",
@@ -293,7 +294,8 @@ int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint
                                      key_bytes, first_iota && p == 0, n_dev, (uint32_t)n_max,
                                      hist + p * radix, radix_status, tickets + p, shifts[p],
                                      c->digit_bits, next_epoch(c), p + 1 == passes ? aux_in : nullptr,
-                                     p + 1 == passes ? aux_out : nullptr, c->stream));
+                                     p + 1 == passes ? aux_out : nullptr, c->stream, nullptr, c->sort_key_tile,
+                                     pass_match_bits(shifts[p], c->digit_bits, key_bits)));
     }
     *in_b = (passes % 2) == 1;
     return GS3D_OK;
@@ -410,7 +412,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     bool order_in_b = false;
     // pass 0 takes the positions as values (ida is not read); odd passes write ida
     GS3D_TRY(run_passes(c, dka, ida, dkb, idb, 4, true, counters + kCtrSplatCount, n, c->shifts,
-                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b));
+                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b, nullptr, nullptr, 32));
     const uint32_t* order = order_in_b ? idb : ida;
     const uint32_t* depth_sorted = order_in_b ? dkb : dka;
     record(c, GS3D_STAGE_DEPTH_SORT, 1);
@@ -453,7 +455,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     record(c, GS3D_STAGE_SORT, 0);
     GS3D_TRY(run_passes(c, ka, va, kb, vb, 8, false, ref_counter, cap, c->shifts + depth_passes,
                         tile_passes, hist_tile, counters + kCtrSortTicket + depth_passes,
-                        &c->result_in_b));
+                        &c->result_in_b, nullptr, nullptr, 32u + (T > 1 ? bit_length(T - 1) : 0u)));
     record(c, GS3D_STAGE_SORT, 1);
     const uint64_t* sorted_keys = c->result_in_b ? kb : ka;
     const uint32_t* sorted_vals = c->result_in_b ? vb : va;
@@ -682,14 +684,16 @@ int batch_buffers(gs3d_ctx* c, bool* moved) {
 int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, uint32_t* vb,
                  uint32_t key_bytes, bool first_iota, const uint32_t* n_dev, uint64_t n_max,
                  const uint32_t* shifts, uint32_t passes, const uint32_t* hist, uint32_t* tickets,
-                 uint64_t* status, const uint32_t* epoch_base, uint32_t* epoch_ctr, bool* in_b) {
+                 uint64_t* status, const uint32_t* epoch_base, uint32_t* epoch_ctr, bool* in_b,
+                 uint32_t key_bits) {
     const uint32_t radix = 1u << c->digit_bits;
     for (uint32_t p = 0; p < passes; p++) {
         const bool even = (p % 2) == 0;
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va, key_bytes,
                                      first_iota && p == 0, n_dev, (uint32_t)n_max, hist + p * radix, status,
                                      tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr, nullptr, nullptr, s,
-                                     epoch_base));
+                                     epoch_base, c->sort_key_tile,
+                                     pass_match_bits(shifts[p], c->digit_bits, key_bits)));
     }
     *in_b = (passes % 2) == 1;
     return GS3D_OK;
@@ -771,7 +775,7 @@ int batch_enqueue(gs3d_ctx* c) {
         bool order_in_b = false;
         GS3D_TRY(batch_passes(c, s, dka, ida, dkb, idb, 4, true, ctr + kCtrSplatCount, n, shifts, depth_passes,
                               hist_depth, ctr + kCtrSortTicket, radix_status, epoch_base, &epoch_ctr[k],
-                              &order_in_b));
+                              &order_in_b, 32));
         ScanRectsArgs sa{};
         sa.order = order_in_b ? idb : ida;
         sa.depth_keys = order_in_b ? dkb : dka;
@@ -803,7 +807,7 @@ int batch_enqueue(gs3d_ctx* c) {
         bool in_b = false;
         GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, 8, false, ctr + kCtrRefCounter, cap, shifts + depth_passes,
                               tile_passes, hist_tile, ctr + kCtrSortTicket + depth_passes, radix_status,
-                              epoch_base, &epoch_ctr[k], &in_b));
+                              epoch_base, &epoch_ctr[k], &in_b, 32u + (T > 1 ? bit_length(T - 1) : 0u)));
         uint32_t* to = tile_offsets + v.to_off;
         HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
                                          b->pinned_k + vi, s));
@@ -1205,6 +1209,13 @@ int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
         c->digit_bits = (uint32_t)value;
         return GS3D_OK;
     }
+    if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "KEY_TILE")) {
+        if (value != 0 && value != 8192 && value != 16384)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "no radix-sort variant with %d-key tiles (0, 8192 or 16384)", value);
+        c->sort_key_tile = (uint32_t)value;
+        if (c->batch) drop_graph(c->batch);
+        return GS3D_OK;
+    }
     return fail(c, GS3D_E_UNKNOWN_NAME, "no kernel variant %s.%s", kernel, param);
 }
 
@@ -1220,6 +1231,10 @@ int gs3d_get_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
     }
     if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "RADIX_BITS")) {
         *value = (int)c->digit_bits;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "KEY_TILE")) {
+        *value = (int)c->sort_key_tile;
         return GS3D_OK;
     }
     return fail(c, GS3D_E_UNKNOWN_NAME, "no kernel variant %s.%s", kernel, param);

@@ -167,7 +167,7 @@ const KERNEL_VARIANTS = {
   'preprocess-splats': { constants: {} },
   'prefix-scan': { constants: {} },
   'emit-tile-refs': { constants: {} },
-  'radix-sort': { constants: { RADIX_BITS: [4, 8] } },
+  'radix-sort': { constants: { RADIX_BITS: [4, 8], KEY_TILE: [0, 8192, 16384] } },
   'identify-tile-ranges': { constants: {} },
   'rasterize-tiles': {
     constants: { TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4] },
@@ -176,7 +176,7 @@ const KERNEL_VARIANTS = {
 
 // Constants the library selects by gs3d_set_kernel_variant; the tile constants
 // follow gs3d_set_tile_size instead.
-const LIBRARY_CONSTANTS = new Set(['RADIX_BITS', 'PIXELS_PER_LANE']);
+const LIBRARY_CONSTANTS = new Set(['RADIX_BITS', 'KEY_TILE', 'PIXELS_PER_LANE']);
 
 class PipelineManager {
   // With an addon and context, creating or updating a pipeline also selects
