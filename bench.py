@@ -222,6 +222,9 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-colour measurement")
     ap.add_argument("--only-learned", action="store_true", help="run only the learned-colour measurement")
     ap.add_argument("--learned-epochs", type=int, default=4)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device 5 times (the reference reading for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--composite", type=int, default=0,
                     help="after the timed region, run this many composites per display mode "
@@ -232,6 +235,16 @@ def main() -> int:
               "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {a.numel() * 4} bytes x5")
+        return 0
     if args.only_batched:
         print(json.dumps({"batched": bench_batched(args.batch_steps, 2)}), flush=True)
         return 0
