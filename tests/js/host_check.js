@@ -41,6 +41,36 @@ async function plyTime(req) {
   return { ok: true, n: splats.splatCount, seconds };
 }
 
+// GPU: renderBatch of several views vs render() of each view; then a short
+// learned-colour train + validate through the addon.
+async function batch(req) {
+  const scene = new host.Scene();
+  const { splats, bounds } = await new host.SceneLoader({ native: true }).loadFile(req.path);
+  scene.splats = splats;
+  scene.bounds = bounds;
+  scene.tiles = [16, 16];
+  const r = new host.GaussianSplatRenderer(scene);
+  try {
+    r.init();
+    const frames = req.views.map((v) => ({
+      width: v.width, height: v.height,
+      camera: new host.Camera(v.eye, [0, 0, 0], [0, 1, 0], Math.PI / 3, v.width / v.height, 0.1, 10000),
+    }));
+    const ks = Array.from(r.renderBatch(frames));
+    const batchImages = frames.map((_, i) => b64(r.readBatchImage(i, false)));
+    const single = [];
+    for (const f of frames) {
+      r.render({ ...f, cameraVersion: single.length + 1000 });
+      single.push({ image: b64(r.readImage(false)), k: r.refCount() });
+    }
+    const losses = Array.from(r.trainLearnedColor({ epochs: 2 }));
+    const validation = r.validateLearnedColor(frames, 45);
+    return { ok: true, ks, batchImages, single, losses, validation };
+  } finally {
+    r.destroy();
+  }
+}
+
 function camera(req) {
   const out = [];
   for (const c of req.cameras) {
@@ -126,7 +156,7 @@ function addonExports() {
   return { ok: true, names: Object.keys(addon).sort(), abiVersion: addon.abiVersion };
 }
 
-const HANDLERS = { ply, plyTime, camera, workgroups, pipelines, render, addonExports };
+const HANDLERS = { ply, plyTime, batch, camera, workgroups, pipelines, render, addonExports };
 
 (async () => {
   const req = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));

@@ -250,6 +250,40 @@ def test_full_size_1m_sh3_properties(renderer):
     assert np.array_equal(renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16), img1)
 
 
+def test_full_size_6m_sh3_2160p_properties(renderer):
+    """The largest single-GPU workload: 6M SH3 splats at 3840x2160, ~119M keys. The key
+    buffers start at 8N = 48M entries, so the frame regrows them and runs again; then the
+    same size-independent properties as at 1M, and determinism."""
+    say("full-size clusters-6M-sh3-2160p")
+    wl = sc.workload(2)
+    s = sc.activate(wl.raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    eye, W, H = wl.views[0]
+    cu = cam_mod.view_camera(eye, W, H)
+    renderer.rebin(cu, W, H)
+    k = renderer.ref_count()
+    say(f"  K={k}")
+    assert k > 8 * s.n  # the regrow path ran
+    counts = renderer.read(L.BUF_SPLAT_REF_COUNTS, np.uint32)
+    keys = renderer.read(L.BUF_SORT_KEYS, np.uint64)
+    vals = renderer.read(L.BUF_SORT_VALUES, np.uint32)
+    toff = renderer.read(L.BUF_TILE_OFFSETS, np.uint32)
+    assert k == int(counts.sum(dtype=np.uint64)) == keys.size
+    assert np.all(keys[1:] >= keys[:-1])
+    same = keys[1:] == keys[:-1]
+    assert np.all(vals[1:][same] > vals[:-1][same])
+    del same
+    assert np.array_equal(np.bincount(vals, minlength=s.n), counts)
+    T = 240 * 135
+    assert np.array_equal(toff, O.identify(keys, T))
+    img1 = renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16)
+    renderer.rebin(cu, W, H)
+    assert renderer.ref_count() == k
+    assert np.array_equal(renderer.read(L.BUF_SORT_VALUES, np.uint32), vals)
+    assert np.array_equal(renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16), img1)
+
+
 def test_errors_through_abi(renderer):
     import gs3d.renderer as R
 

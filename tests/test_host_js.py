@@ -371,7 +371,8 @@ def test_addon_exports(tmp_path):
             "stageTimesAvg", "profilerReset", "stageLabel", "bufferCreate", "bufferResize", "bufferSize",
             "bufferWrite", "bufferRead", "bufferDestroy", "bufferList", "setKernelVariant",
             "getKernelVariant", "setProfileMask", "sortPasses", "setDebugMode", "composite",
-            "parsePly", "loadPly", "abiVersion"}
+            "parsePly", "loadPly", "rebinBatch", "batchRefCounts", "batchRead", "learnedInit",
+            "learnedTrain", "learnedValidate", "learnedEnable", "abiVersion"}
     assert set(r["names"]) == want
     assert r["abiVersion"] >= 1
 
@@ -408,3 +409,22 @@ def test_node_render_matches_oracle(tmp_path, tile, ppl):
     fb = np.frombuffer(base64.b64decode(r["framebuffer"]), np.uint8).reshape(H, W, 4)
     heat = O.tile_overlay(img, ref["tile_offsets"], tile, W, H, r["debugRef"])
     np.testing.assert_array_equal(fb, O.composite(heat, W, H, clear=(0.12, 0.12, 0.13, 1.0)))
+
+
+@pytest.mark.gpu
+def test_node_render_batch_equals_single_views(tmp_path):
+    """GaussianSplatRenderer.renderBatch (one hipGraph over all views) gives each view the
+    image and key count of render() alone; the learned-colour calls run through the addon."""
+    raw = S.clustered(13, 20_000, 6, sh_degree=3)
+    p = tmp_path / "scene.ply"
+    write_ply(p, raw, "binary_little_endian")
+    views = [{"eye": [3.0, 2.0, 4.0], "width": 320, "height": 180},
+             {"eye": [-4.0, 1.0, 3.0], "width": 256, "height": 256},
+             {"eye": [0.5, -3.0, 4.5], "width": 200, "height": 120}]
+    r = run_node({"cmd": "batch", "path": str(p), "views": views}, tmp_path, timeout=300)
+    assert r["ok"]
+    for i, sv in enumerate(r["single"]):
+        assert r["ks"][i] == sv["k"]
+        assert r["batchImages"][i] == sv["image"]
+    assert len(r["losses"]) == 2 and r["losses"][1] < r["losses"][0]
+    assert r["validation"]["enabled"] == (r["validation"]["psnr"] >= 45)

@@ -690,6 +690,162 @@ static napi_value js_load_ply(napi_env env, napi_callback_info info) {
     return r;
 }
 
+/* views: Array<{camera: Float32Array(72), width, height}> -> malloc'd gs3d_view[] */
+static gs3d_view* get_views(napi_env env, napi_value arr, uint32_t* count) {
+    bool is_array = false;
+    if (napi_is_array(env, arr, &is_array) != napi_ok || !is_array) return NULL;
+    uint32_t n = 0;
+    napi_get_array_length(env, arr, &n);
+    if (n == 0) return NULL;
+    gs3d_view* v = (gs3d_view*)calloc(n, sizeof(gs3d_view));
+    if (!v) return NULL;
+    for (uint32_t i = 0; i < n; i++) {
+        napi_value e, cam, w, h;
+        void* data;
+        size_t bytes;
+        if (napi_get_element(env, arr, i, &e) != napi_ok || napi_get_named_property(env, e, "camera", &cam) != napi_ok ||
+            !get_bytes(env, cam, &data, &bytes) || !data || bytes < sizeof(gs3d_camera) ||
+            napi_get_named_property(env, e, "width", &w) != napi_ok || !get_u32(env, w, &v[i].width) ||
+            napi_get_named_property(env, e, "height", &h) != napi_ok || !get_u32(env, h, &v[i].height)) {
+            free(v);
+            return NULL;
+        }
+        memcpy(&v[i].camera, data, sizeof(gs3d_camera));
+    }
+    *count = n;
+    return v;
+}
+
+/* rebinBatch(ctx, views, flags) */
+static napi_value js_rebin_batch(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t n = 0, flags = 0;
+    gs3d_view* v = get_views(env, argv[1], &n);
+    if (!v) {
+        napi_throw_type_error(env, NULL, "rebinBatch(ctx, [{camera: Float32Array(72), width, height}, ...], flags)");
+        return NULL;
+    }
+    if (argc > 2) get_u32(env, argv[2], &flags);
+    const int rc = gs3d_rebin_batch(c, v, n, flags);
+    free(v);
+    CHECK(env, c, rc, "gs3d_rebin_batch");
+    return NULL;
+}
+
+/* batchRefCounts(ctx) -> Float64Array (K per view) */
+static napi_value js_batch_ref_counts(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint64_t k[GS3D_MAX_BATCH_VIEWS];
+    uint32_t n = 0;
+    if (!get_u32(env, argv[1], &n) || n == 0 || n > GS3D_MAX_BATCH_VIEWS) {
+        napi_throw_type_error(env, NULL, "batchRefCounts(ctx, nViews)");
+        return NULL;
+    }
+    CHECK(env, c, gs3d_batch_ref_counts(c, k, n), "gs3d_batch_ref_counts");
+    double d[GS3D_MAX_BATCH_VIEWS];
+    for (uint32_t i = 0; i < n; i++) d[i] = (double)k[i];
+    return make_typed(env, napi_float64_array, d, n, 8);
+}
+
+/* batchRead(ctx, view, bufferId) -> ArrayBuffer */
+static napi_value js_batch_read(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t view = 0;
+    int32_t id = 0;
+    get_u32(env, argv[1], &view);
+    napi_get_value_int32(env, argv[2], &id);
+    size_t size = 0;
+    CHECK(env, c, gs3d_batch_read(c, view, id, NULL, 0, &size), "gs3d_batch_read");
+    void* data;
+    napi_value ab;
+    NAPI_CALL(env, napi_create_arraybuffer(env, size, &data, &ab));
+    CHECK(env, c, gs3d_batch_read(c, view, id, data, size, &size), "gs3d_batch_read");
+    return ab;
+}
+
+/* learnedInit(ctx, seed) */
+static napi_value js_learned_init(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    int64_t seed = 0;
+    if (argc > 1) napi_get_value_int64(env, argv[1], &seed);
+    CHECK(env, c, gs3d_learned_init(c, (uint64_t)seed), "gs3d_learned_init");
+    return NULL;
+}
+
+/* learnedTrain(ctx, epochs, lr, lrLatent) -> Float32Array (mean squared error per epoch) */
+static napi_value js_learned_train(napi_env env, napi_callback_info info) {
+    ARGS(4);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t epochs = 0;
+    double lr = 1e-3, lrz = 1e-2;
+    if (!get_u32(env, argv[1], &epochs) || epochs == 0 || epochs > 100000) {
+        napi_throw_type_error(env, NULL, "learnedTrain(ctx, epochs, lr, lrLatent)");
+        return NULL;
+    }
+    if (argc > 2) napi_get_value_double(env, argv[2], &lr);
+    if (argc > 3) napi_get_value_double(env, argv[3], &lrz);
+    float* loss = (float*)calloc(epochs, sizeof(float));
+    if (!loss) {
+        napi_throw_error(env, NULL, "learnedTrain: out of memory");
+        return NULL;
+    }
+    const int rc = gs3d_learned_train(c, epochs, (float)lr, (float)lrz, loss, epochs);
+    if (rc != GS3D_OK) {
+        free(loss);
+        return throw_rc(env, c, rc, "gs3d_learned_train");
+    }
+    napi_value out = make_typed(env, napi_float32_array, loss, epochs, 4);
+    free(loss);
+    return out;
+}
+
+/* learnedValidate(ctx, views, minPsnrDb) -> {psnr, enabled} */
+static napi_value js_learned_validate(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    uint32_t n = 0;
+    gs3d_view* v = get_views(env, argv[1], &n);
+    double gate = 45.0;
+    if (!v) {
+        napi_throw_type_error(env, NULL, "learnedValidate(ctx, views, minPsnrDb)");
+        return NULL;
+    }
+    if (argc > 2) napi_get_value_double(env, argv[2], &gate);
+    float psnr = 0.f;
+    int enabled = 0;
+    const int rc = gs3d_learned_validate(c, v, n, (float)gate, &psnr, &enabled);
+    free(v);
+    CHECK(env, c, rc, "gs3d_learned_validate");
+    napi_value r, p, e;
+    napi_create_object(env, &r);
+    napi_create_double(env, psnr, &p);
+    napi_get_boolean(env, enabled != 0, &e);
+    napi_set_named_property(env, r, "psnr", p);
+    napi_set_named_property(env, r, "enabled", e);
+    return r;
+}
+
+/* learnedEnable(ctx, on) */
+static napi_value js_learned_enable(napi_env env, napi_callback_info info) {
+    ARGS(2);
+    gs3d_ctx* c = get_ctx(env, argv[0]);
+    if (!c) return NULL;
+    bool on = false;
+    napi_get_value_bool(env, argv[1], &on);
+    CHECK(env, c, gs3d_learned_enable(c, on ? 1 : 0), "gs3d_learned_enable");
+    return NULL;
+}
+
 static napi_value init(napi_env env, napi_value exports) {
     napi_property_descriptor d[] = {
 #define FN(js, c) {js, NULL, c, NULL, NULL, NULL, napi_enumerable, NULL}
@@ -725,6 +881,13 @@ static napi_value init(napi_env env, napi_value exports) {
         FN("composite", js_composite),
         FN("parsePly", js_parse_ply),
         FN("loadPly", js_load_ply),
+        FN("rebinBatch", js_rebin_batch),
+        FN("batchRefCounts", js_batch_ref_counts),
+        FN("batchRead", js_batch_read),
+        FN("learnedInit", js_learned_init),
+        FN("learnedTrain", js_learned_train),
+        FN("learnedValidate", js_learned_validate),
+        FN("learnedEnable", js_learned_enable),
 #undef FN
     };
     NAPI_CALL(env, napi_define_properties(env, exports, sizeof d / sizeof d[0], d));

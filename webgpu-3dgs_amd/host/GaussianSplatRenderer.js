@@ -10,6 +10,8 @@
 const { WorkgroupManager, linear1D, tiled1D, tile2D, BufferManager, PipelineManager } = require('./managers');
 
 const REBIN_PROFILE = 1;
+const BATCH_EAGER = 1;
+const BATCH_LEARNED = 2;
 const BUF = {
   PROJECTED_SPLATS: 0, SPLAT_REF_COUNTS: 1, SPLAT_REF_OFFSETS: 2, REF_COUNTER: 3, SORT_KEYS: 4,
   SORT_VALUES: 5, TILE_OFFSETS: 6, IMAGE_RGBA16F: 7, IMAGE_RGBA32F: 8,
@@ -158,6 +160,37 @@ class GaussianSplatRenderer {
     const flags = this.profiler ? REBIN_PROFILE : 0;
     this.addon.rebin(this.ctx, camera.getUniformData(width, height), width, height, flags);
     this.rebinCount++;
+  }
+
+  // Many views in one call (no reference counterpart): frames = [{width, height, camera}],
+  // rendered by one replayed hipGraph; learned: colour with the learned stage when it
+  // passed validation (exact SH otherwise). Returns the K of each view.
+  renderBatch(frames, { learned = false, eager = false } = {}) {
+    const splats = this.scene.splats;
+    if (!splats) return null;
+    if (this.uploadedSplats !== splats) this.setSplatData(splats);
+    const views = frames.map((f) => ({ camera: f.camera.getUniformData(f.width, f.height), width: f.width, height: f.height }));
+    this.addon.rebinBatch(this.ctx, views, (eager ? BATCH_EAGER : 0) | (learned ? BATCH_LEARNED : 0));
+    this.batchViews = frames.length;
+    return this.addon.batchRefCounts(this.ctx, frames.length);
+  }
+
+  readBatchImage(view, asFloat32 = true) {
+    const ab = this.addon.batchRead(this.ctx, view, asFloat32 ? BUF.IMAGE_RGBA32F : BUF.IMAGE_RGBA16F);
+    return asFloat32 ? new Float32Array(ab) : new Uint16Array(ab);
+  }
+
+  // Learned colour: train on the uploaded splats' exact SH colours, then check the
+  // learned render against exact SH on `frames`; enabled iff PSNR >= minPsnrDb.
+  trainLearnedColor({ epochs = 8, lr = 3e-3, lrLatent = 3e-2, seed = 0 } = {}) {
+    if (this.uploadedSplats !== this.scene.splats) this.setSplatData(this.scene.splats);
+    this.addon.learnedInit(this.ctx, seed);
+    return this.addon.learnedTrain(this.ctx, epochs, lr, lrLatent);
+  }
+
+  validateLearnedColor(frames, minPsnrDb = 45) {
+    const views = frames.map((f) => ({ camera: f.camera.getUniformData(f.width, f.height), width: f.width, height: f.height }));
+    return this.addon.learnedValidate(this.ctx, views, minPsnrDb);
   }
 
   getTileCount() {
