@@ -257,8 +257,10 @@ struct LearnedTrainArgs {
 };
 uint32_t learned_train_waves(uint32_t n);
 hipError_t launch_learned_train(const LearnedTrainArgs& a, uint32_t waves, hipStream_t s);
+// scratch: learned_reduce_scratch_floats() floats
+size_t learned_reduce_scratch_floats();
 hipError_t launch_learned_reduce(const float* partial, uint32_t rows, float* grad, const float* loss_partial,
-                                 float* loss, hipStream_t s);
+                                 float* loss, float* scratch, hipStream_t s);
 hipError_t launch_adam(float* x, float* m, float* v, const float* grad, size_t n, float lr, float b1, float b2,
                        float eps, uint32_t step, uint16_t* f16_copy, hipStream_t s);
 // sum over pixels of the squared RGB differences of two f16 RGBA images (accumulated into *out, f64)

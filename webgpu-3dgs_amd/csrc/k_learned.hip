@@ -274,22 +274,35 @@ __global__ __launch_bounds__(kTrainWaves * 64) void k_learned_train(LearnedTrain
     for (int k = 0; k < 16; k++) basis[k] = a.basis[16 * dj + k];
     const int ncoef = (a.sh_degree + 1) * (a.sh_degree + 1);
 
+    // The next splat's SH coefficients and latent are loaded while the current
+    // one is processed (one wavefront per SIMD leaves no other wave to hide
+    // the load latency).
+    float coef[16][3], zc[8];
+    auto load_splat = [&](uint32_t i, float (*c)[3], float* z) {
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) c[k][ch] = k < ncoef ? a.sh[(size_t)(k * 3 + ch) * a.n + i] : 0.f;
+        const float4* zp = reinterpret_cast<const float4*>(a.latents + (size_t)i * 8);
+        const float4 z0 = zp[0], z1 = zp[1];
+        z[0] = z0.x, z[1] = z0.y, z[2] = z0.z, z[3] = z0.w, z[4] = z1.x, z[5] = z1.y, z[6] = z1.z, z[7] = z1.w;
+    };
+    if (wave < a.n) load_splat(__builtin_amdgcn_readfirstlane(wave), coef, zc);
     for (uint32_t iv = wave; iv < a.n; iv += n_waves) {
         const uint32_t i = __builtin_amdgcn_readfirstlane(iv);  // one splat per wavefront
+        float tgt[3], z8[8];
         // target: exact SH colour clamp01(0.5 + sum_k b_k c_k) for direction j
-        float tgt[3];
 #pragma unroll
         for (int ch = 0; ch < 3; ch++) {
             float acc = 0.f;
 #pragma unroll
-            for (int k = 0; k < 16; k++)
-                if (k < ncoef) acc += basis[k] * a.sh[(size_t)(k * 3 + ch) * a.n + i];
+            for (int k = 0; k < 16; k++) acc += basis[k] * coef[k][ch];
             const float v = acc + 0.5f;
             tgt[ch] = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
         }
-        float z8[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) z8[k] = f16_bits_to_f32(f32_to_f16_bits(a.latents[(size_t)i * 8 + k]));
+        for (int k = 0; k < 8; k++) z8[k] = f16_bits_to_f32(f32_to_f16_bits(zc[k]));
+        if (iv + n_waves < a.n) load_splat(__builtin_amdgcn_readfirstlane(iv + n_waves), coef, zc);
         const bf16x4 x = input_tile(z8, dx, dy, dz);
         Acts act;
         forward(f, x, act);
@@ -424,18 +437,37 @@ __global__ __launch_bounds__(kTrainWaves * 64) void k_learned_train(LearnedTrain
     if (lane == 0) a.loss_partial[wave] = l;
 }
 
-// grad[p] = sum over rows of partial[row][p] (p < cols); loss likewise.
-__global__ __launch_bounds__(256) void k_learned_reduce(const float* partial, uint32_t rows, float* grad,
-                                                       const float* loss_partial, float* loss) {
+// Two-level sum over the per-wavefront rows: k_learned_reduce_rows folds row
+// group y (rows y, y + kReduceGroups, ...) into fold[y][p]; k_learned_reduce
+// sums the kReduceGroups folds. Both are coalesced over p.
+constexpr uint32_t kReduceGroups = 64;
+
+__global__ __launch_bounds__(256) void k_learned_reduce_rows(const float* partial, uint32_t rows, float* fold,
+                                                            const float* loss_partial, float* loss_fold) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x, y = blockIdx.y;
+    if (p < GS3D_LEARNED_PARAMS) {
+        float s = 0.f;
+        for (uint32_t r = y; r < rows; r += kReduceGroups) s += partial[(size_t)r * GS3D_LEARNED_PARAMS + p];
+        fold[(size_t)y * GS3D_LEARNED_PARAMS + p] = s;
+    }
+    if (p == 0) {
+        float s = 0.f;
+        for (uint32_t r = y; r < rows; r += kReduceGroups) s += loss_partial[r];
+        loss_fold[y] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_learned_reduce(const float* fold, float* grad, const float* loss_fold,
+                                                       float* loss) {
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
     if (p < GS3D_LEARNED_PARAMS) {
         float s = 0.f;
-        for (uint32_t r = 0; r < rows; r++) s += partial[(size_t)r * GS3D_LEARNED_PARAMS + p];
+        for (uint32_t y = 0; y < kReduceGroups; y++) s += fold[(size_t)y * GS3D_LEARNED_PARAMS + p];
         grad[p] = s;
     }
     if (p == 0 && loss) {
         float s = 0.f;
-        for (uint32_t r = 0; r < rows; r++) s += loss_partial[r];
+        for (uint32_t y = 0; y < kReduceGroups; y++) s += loss_fold[y];
         *loss = s;
     }
 }
@@ -510,10 +542,16 @@ hipError_t launch_learned_train(const LearnedTrainArgs& a, uint32_t waves, hipSt
     return hipGetLastError();
 }
 
+size_t learned_reduce_scratch_floats() { return (size_t)kReduceGroups * (GS3D_LEARNED_PARAMS + 1); }
+
 hipError_t launch_learned_reduce(const float* partial, uint32_t rows, float* grad, const float* loss_partial,
-                                 float* loss, hipStream_t s) {
-    hipLaunchKernelGGL(k_learned_reduce, dim3((GS3D_LEARNED_PARAMS + 255) / 256), dim3(256), 0, s, partial, rows,
-                       grad, loss_partial, loss);
+                                 float* loss, float* scratch, hipStream_t s) {
+    float* fold = scratch;
+    float* loss_fold = scratch + (size_t)kReduceGroups * GS3D_LEARNED_PARAMS;
+    const uint32_t blocks = (GS3D_LEARNED_PARAMS + 255) / 256;
+    hipLaunchKernelGGL(k_learned_reduce_rows, dim3(blocks, kReduceGroups), dim3(256), 0, s, partial, rows, fold,
+                       loss_partial, loss_fold);
+    hipLaunchKernelGGL(k_learned_reduce, dim3(blocks), dim3(256), 0, s, fold, grad, loss_fold, loss);
     return hipGetLastError();
 }
 

@@ -1021,6 +1021,7 @@ int learned_buffers(gs3d_ctx* c) {
         {"learned_latents", n8 * 4},            {"learned_lm", n8 * 4},  {"learned_lv", n8 * 4},
         {"learned_lgrad", n8 * 4},              {"learned_latents_f16", n8 * 2},
         {"learned_dirs", GS3D_LEARNED_DIRS * 3 * 4}, {"learned_basis", GS3D_LEARNED_DIRS * 16 * 4},
+        {"learned_reduce_scratch", learned_reduce_scratch_floats() * 4},
     };
     for (const auto& w : want) {
         void* p;
@@ -1049,7 +1050,8 @@ int learned_step_grads(gs3d_ctx* c, uint32_t first_dir, uint32_t slot) {
     const uint32_t waves = learned_train_waves(c->n);
     HIP_TRY(c, launch_learned_train(ta, waves, c->stream));
     HIP_TRY(c, launch_learned_reduce(ta.grad_partial, waves, buf<float>(c, "learned_grad"), ta.loss_partial,
-                                     buf<float>(c, "learned_loss") + slot, c->stream));
+                                     buf<float>(c, "learned_loss") + slot, buf<float>(c, "learned_reduce_scratch"),
+                                     c->stream));
     return GS3D_OK;
 }
 
