@@ -214,6 +214,8 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--key-tile", type=int, default=0, help="keys per workgroup of the 64-bit sort passes "
                     "(0 = chosen by key count, 8192, 16384)")
+    ap.add_argument("--depth-key-tile", type=int, default=0, help="keys per workgroup of the 32-bit depth "
+                    "sort passes (0 = 8192, 4096)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batch measurement")
@@ -221,7 +223,7 @@ def main() -> int:
                     help="run only the 64-view batch measurement (for kernel traces of it)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-colour measurement")
     ap.add_argument("--only-learned", action="store_true", help="run only the learned-colour measurement")
-    ap.add_argument("--learned-epochs", type=int, default=4)
+    ap.add_argument("--learned-epochs", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the reference reading for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -268,6 +270,7 @@ def main() -> int:
     r.set_splat_data(splats)
     r.set_tile_size(args.tile, args.tile)
     r.set_kernel_variant("radix-sort", "KEY_TILE", args.key_tile)
+    r.set_kernel_variant("radix-sort-splats", "KEY_TILE", args.depth_key_tile)
     for _ in range(args.warmup):
         r.rebin(cu, W, H)
     r.sync()

@@ -166,6 +166,8 @@ int gs3d_get_tile_size(const gs3d_ctx* ctx, uint32_t out_xy[2]);
  *   "radix-sort"      "RADIX_BITS"       4, 8
  *   "radix-sort"      "KEY_TILE"         0 (by key count), 8192, 16384: keys per
  *                                        workgroup of the 64-bit key passes
+ *   "radix-sort-splats" "KEY_TILE"       0 (8192), 4096: keys per workgroup of
+ *                                        the 32-bit depth passes
  * get_kernel_variant also reads "rasterize-tiles" "TILE_X". Anything else:
  * GS3D_E_UNKNOWN_NAME. A tile-size change resets an impossible PIXELS_PER_LANE
  * to 1. */

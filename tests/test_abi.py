@@ -41,6 +41,13 @@ def test_null_arguments_rejected_without_device():
     assert lib.gs3d_set_debug_mode(None, 1, 256) == -1
     assert lib.gs3d_composite(None, 0, None, None, 0) == -1
     assert lib.gs3d_stage_composite(None, None, None, 16, 16, 1, 1, 0, 256, 0, None, None, None) == -1
+    assert lib.gs3d_rebin_batch(None, None, 1, 0) == -1
+    assert lib.gs3d_batch_read(None, 0, 7, None, 0, None) == -1
+    assert lib.gs3d_learned_init(None, 0) == -1
+    assert lib.gs3d_learned_train(None, 1, 1e-3, 1e-2, None, 0) == -1
+    assert lib.gs3d_learned_validate(None, None, 1, 45.0, None, None) == -1
+    assert lib.gs3d_load_ply(None, b"x.ply", 0, None) == -1
+    assert lib.gs3d_ply_parse(None, 0, None, None, None, None, None, None, 0, None, None, 0) == -1
     assert lib.gs3d_last_error(None) == b"null context"
 
 

@@ -152,3 +152,16 @@ def test_batch_autotuner_picks_the_fastest_candidate(renderer, scene_sh3):
         renderer.set_tile_size(16, 16)
         renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 1)
         renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)
+
+
+def test_batch_all_culled_views(renderer, scene_sh3):
+    """Views that see no splat (cameras facing away from the scene) next to one that does:
+    K = 0, empty tile ranges and a cleared image, exactly as single views give them."""
+    say("batch: all-culled views")
+    renderer.set_splat_data(scene_sh3)
+    cams = [(cam_mod.view_camera((0.0, 0.0, 30.0), 160, 90, target=(0.0, 0.0, 60.0)), 160, 90),
+            (cam_mod.view_camera((5.0, 2.0, 6.0), 128, 96), 128, 96),
+            (cam_mod.view_camera((0.0, 40.0, 0.5), 96, 96, target=(0.0, 80.0, 0.5)), 96, 96)]
+    got = check_batch(renderer, cams)
+    assert got[0]["k"] == 0 and got[2]["k"] == 0 and got[1]["k"] > 0
+    assert not got[0]["image"].any() and not got[0]["tile_offsets"].any()

@@ -417,3 +417,16 @@ def test_sort_key_tile_shapes_bit_exact(renderer, n, key_tile):
         renderer.set_kernel_variant("radix-sort", "KEY_TILE", 0)
     order = np.lexsort((np.arange(n), keys))
     assert np.array_equal(ko, keys[order]) and np.array_equal(vo, vals[order])
+
+
+@pytest.mark.parametrize("depth_tile", [4096, 8192])
+def test_depth_key_tile_shapes_bit_exact(renderer, depth_tile):
+    """Both tile shapes of the 32-bit depth passes give the same frame as the oracle."""
+    say(f"pipeline depth key tile {depth_tile}")
+    renderer.set_kernel_variant("radix-sort-splats", "KEY_TILE", depth_tile)
+    try:
+        s = sc.activate(sc.clustered(43, 60_000, 16, sh_degree=1))
+        _, _, err = rebin_and_compare(renderer, s, sc.orbit_eye(7.0, 10.0, 25.0), 640, 480)
+        assert err <= TOL_IMAGE
+    finally:
+        renderer.set_kernel_variant("radix-sort-splats", "KEY_TILE", 0)

@@ -206,3 +206,21 @@ def test_fallback_to_exact_sh_below_threshold(renderer, small_scene):
     differs = any(not np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), exact[i])
                   for i in range(2))
     assert differs
+
+
+@pytest.mark.gpu
+def test_new_splats_reset_the_learned_state(renderer, small_scene):
+    """Latents belong to the splats they were trained on: uploading splats again (even of
+    the same count) drops them, and a learned batch falls back to exact SH until re-init."""
+    say("learned: reset on upload")
+    renderer.set_splat_data(small_scene)
+    renderer.learned_init(seed=3)
+    renderer.learned_enable(True)
+    renderer.set_splat_data(small_scene)
+    with pytest.raises(Exception):
+        renderer.learned_get()
+    views = [(cam_mod.view_camera(eye, w, h), w, h) for eye, w, h in S.orbit_views(4, 1, fixed=(200, 120))]
+    renderer.rebin_batch(views)
+    exact = renderer.batch_read(0, L.BUF_IMAGE_RGBA16F, np.uint16)
+    renderer.rebin_batch(views, learned=True)
+    assert np.array_equal(renderer.batch_read(0, L.BUF_IMAGE_RGBA16F, np.uint16), exact)

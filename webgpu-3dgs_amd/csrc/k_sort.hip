@@ -270,6 +270,7 @@ uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / k
 constexpr int kSort64Threads = 1024, kSort64Items = 16;
 constexpr int kSort64SmallThreads = 512, kSort64SmallItems = 16;
 constexpr int kSort32Threads = 1024, kSort32Items = 8;
+constexpr int kSort32SmallThreads = 512, kSort32SmallItems = 8;
 
 template <int BITS, typename KeyT, bool IOTA, bool AUX, int THREADS, int ITEMS>
 static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
@@ -298,10 +299,14 @@ static void launch_shape(bool small_tiles, const void* kin, const uint32_t* vin,
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
     } else {
-        (void)small_tiles;
-        launch_pass<BITS, KeyT, IOTA, AUX, kSort32Threads, kSort32Items>(
-            kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
-            epoch_base, match_bits);
+        if (small_tiles)
+            launch_pass<BITS, KeyT, IOTA, AUX, kSort32SmallThreads, kSort32SmallItems>(
+                kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
+                epoch_base, match_bits);
+        else
+            launch_pass<BITS, KeyT, IOTA, AUX, kSort32Threads, kSort32Items>(
+                kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
+                epoch_base, match_bits);
     }
 }
 
@@ -313,8 +318,17 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
                              const uint32_t* epoch_base, uint32_t key_tile, uint32_t match_bits) {
     if (n_max == 0) return hipSuccess;
     if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
-    if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
-    const bool small_tiles = key_tile == 8192 || (key_tile == 0 && n_max >= kSortSmallTileFrom);
+    // key_tile: keys per workgroup, 0 = the default shape for the key width
+    // (16,384 for 64-bit keys, 8,192 for 32-bit keys); the half-size shape is
+    // 8,192 (64-bit) or 4,096 (32-bit)
+    bool small_tiles;
+    if (key_bytes == 8) {
+        if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
+        small_tiles = key_tile == 8192 || (key_tile == 0 && n_max >= kSortSmallTileFrom);
+    } else {
+        if (key_tile != 0 && key_tile != 4096 && key_tile != 8192) return hipErrorInvalidValue;
+        small_tiles = key_tile == 4096;
+    }
     const bool aux = aux_in != nullptr && aux_out != nullptr;
     // the gather rides on 32-bit key passes whose values are read from vin
     if (aux && (key_bytes != 4 || iota_values)) return hipErrorInvalidValue;

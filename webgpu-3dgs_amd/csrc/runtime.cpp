@@ -67,7 +67,8 @@ struct gs3d_ctx {
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
     uint32_t raster_ppl = 1;
-    uint32_t sort_key_tile = 0;  // 64-bit key passes: 0 = by key count, or 8192 / 16384
+    uint32_t sort_key_tile = 0;    // 64-bit key passes: 0 = by key count, or 8192 / 16384
+    uint32_t sort_key_tile32 = 0;  // 32-bit key passes: 0 = 8192, or 4096
     uint64_t key_capacity = 0;
     uint32_t n_passes = 0;
     uint32_t shifts[kMaxPasses] = {};
@@ -294,7 +295,8 @@ int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint
                                      key_bytes, first_iota && p == 0, n_dev, (uint32_t)n_max,
                                      hist + p * radix, radix_status, tickets + p, shifts[p],
                                      c->digit_bits, next_epoch(c), p + 1 == passes ? aux_in : nullptr,
-                                     p + 1 == passes ? aux_out : nullptr, c->stream, nullptr, c->sort_key_tile,
+                                     p + 1 == passes ? aux_out : nullptr, c->stream, nullptr,
+                                     key_bytes == 8 ? c->sort_key_tile : c->sort_key_tile32,
                                      pass_match_bits(shifts[p], c->digit_bits, key_bits)));
     }
     *in_b = (passes % 2) == 1;
@@ -692,7 +694,7 @@ int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, u
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va, key_bytes,
                                      first_iota && p == 0, n_dev, (uint32_t)n_max, hist + p * radix, status,
                                      tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr, nullptr, nullptr, s,
-                                     epoch_base, c->sort_key_tile,
+                                     epoch_base, key_bytes == 8 ? c->sort_key_tile : c->sort_key_tile32,
                                      pass_match_bits(shifts[p], c->digit_bits, key_bits)));
     }
     *in_b = (passes % 2) == 1;
@@ -1171,6 +1173,9 @@ int gs3d_set_splats(gs3d_ctx* c, const float* pos4, const float* scale4, const f
     c->sh_degree = sh_degree;
     c->have_frame = false;
     c->key_capacity = 0;
+    // latents belong to the splats they were trained on
+    c->learned_init = false;
+    c->learned_enabled = false;
     return GS3D_OK;
 }
 
@@ -1218,6 +1223,13 @@ int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
         if (c->batch) drop_graph(c->batch);
         return GS3D_OK;
     }
+    if (!std::strcmp(kernel, "radix-sort-splats") && !std::strcmp(param, "KEY_TILE")) {
+        if (value != 0 && value != 4096 && value != 8192)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "no radix-sort-splats variant with %d-key tiles (0, 4096 or 8192)", value);
+        c->sort_key_tile32 = (uint32_t)value;
+        if (c->batch) drop_graph(c->batch);
+        return GS3D_OK;
+    }
     return fail(c, GS3D_E_UNKNOWN_NAME, "no kernel variant %s.%s", kernel, param);
 }
 
@@ -1237,6 +1249,10 @@ int gs3d_get_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
     }
     if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "KEY_TILE")) {
         *value = (int)c->sort_key_tile;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "radix-sort-splats") && !std::strcmp(param, "KEY_TILE")) {
+        *value = (int)c->sort_key_tile32;
         return GS3D_OK;
     }
     return fail(c, GS3D_E_UNKNOWN_NAME, "no kernel variant %s.%s", kernel, param);
