@@ -45,8 +45,8 @@ def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
     (the first pass reads keys only), plus one read of the keys to histogram;
     scan: the sorted id and depth key and the 8-B tile rectangle of each
     splat in, a 16-B record per splat out; emit: the records and ids in,
-    12-B (key, id) out; key sort: 24 B per key per
-    pass; identify: one (T+1)-entry write plus, per tile, the ceil(log2 K)
+    8-B (tile id, splat id) out; key sort: 16 B per key per pass (32-bit tile id
+    + 32-bit splat id, read and written); identify: one (T+1)-entry write plus, per tile, the ceil(log2 K)
     sorted keys its binary search reads; raster: the 48 B used of each listed
     splat, tile ranges, 8-B pixel out.
     """
@@ -55,8 +55,8 @@ def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
         "preprocess-splats": n * (pre_in + 64 + 4 + 4 + 8),
         "radix-sort-splats": n * (4 + 12 + 16 * (depth_passes - 1)) if depth_passes else 0,
         "prefix-scan": 16 * n + 16 * (n + 1),
-        "emit-tile-refs": 16 * (n + 1) + 4 * n + 12 * k,
-        "radix-sort": tile_passes * 24 * k,
+        "emit-tile-refs": 16 * (n + 1) + 4 * n + 8 * k,
+        "radix-sort": tile_passes * 16 * k,
         "identify-tile-ranges": 8 * tiles * max(1, int(k).bit_length()) + 4 * (tiles + 1),
         "rasterize-tiles": 48 * k + 8 * (tiles + 1) + 8 * pixels,
     }

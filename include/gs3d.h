@@ -164,8 +164,9 @@ int gs3d_get_tile_size(const gs3d_ctx* ctx, uint32_t out_xy[2]);
  * here a precompiled variant is selected):
  *   "rasterize-tiles" "PIXELS_PER_LANE"  1, 2, 4 (workgroup tile^2/ppl >= 64)
  *   "radix-sort"      "RADIX_BITS"       4, 8
- *   "radix-sort"      "KEY_TILE"         0 (by key count), 8192, 16384: keys per
- *                                        workgroup of the 64-bit key passes
+ *   "radix-sort"      "KEY_TILE"         0 (16384), 8192, 16384: keys per
+ *                                        workgroup of the tile-id passes (and of
+ *                                        gs3d_stage_sort's 64-bit passes)
  *   "radix-sort-splats" "KEY_TILE"       0 (8192), 4096: keys per workgroup of
  *                                        the 32-bit depth passes
  * get_kernel_variant also reads "rasterize-tiles" "TILE_X". Anything else:

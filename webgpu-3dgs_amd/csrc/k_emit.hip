@@ -12,6 +12,10 @@
 // tile, width, depth key), read coalesced; with offsets instead, the tile
 // rectangle and depth are read from the projected splats.
 //
+// With `tiles` set, only the tile id is stored (4 B instead of 8): the stream
+// is emitted in stable depth order, so sorting the tile ids alone finishes
+// the (tile, depth) order; the 64-bit keys are rebuilt only for readback.
+//
 // Fused here: the digit histograms of every radix pass that the sort will run,
 // so the sort does not re-read the keys to count them. Keys past `capacity`
 // are counted but not stored; the host sees that in ref_counter and re-runs.
@@ -92,7 +96,8 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
                 key = ((uint64_t)tile << 32) | (uint64_t)s_keylo[o];
                 const uint64_t pos = (uint64_t)block_start + w0 + jw;
                 if (pos < a.capacity) {
-                    a.keys[pos] = key;
+                    if (a.tiles) a.tiles[pos] = tile;
+                    else a.keys[pos] = key;
                     a.values[pos] = s_sid[o];
                 }
             }

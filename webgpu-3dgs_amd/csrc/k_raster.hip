@@ -27,14 +27,18 @@
 // results stay within one fp16 step of the fp32 restatement on the CPU.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "gs3d_internal.h"
 
 namespace gs3d {
 
 namespace {
 
-__global__ __launch_bounds__(256) void k_identify(const uint64_t* __restrict__ keys,
-                                                  const uint32_t* __restrict__ n_dev,
+// The tile id of sorted key i sits at word stride * i + word (64-bit keys:
+// stride 2, the high word; 32-bit tile ids: stride 1).
+__global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ kw, uint32_t stride,
+                                                  uint32_t word, const uint32_t* __restrict__ n_dev,
                                                   uint32_t* __restrict__ tile_offsets,
                                                   uint32_t tile_count, uint32_t n_max,
                                                   uint32_t* __restrict__ k_host) {
@@ -43,20 +47,31 @@ __global__ __launch_bounds__(256) void k_identify(const uint64_t* __restrict__ k
     uint32_t n = *n_dev;
     if (t == 0 && k_host) *k_host = n;  // K for the host, no copy packet needed
     if (n > n_max) n = 0;  // keys did not fit: draw nothing, the host re-runs the frame
-    const uint32_t* kw = reinterpret_cast<const uint32_t*>(keys);
     uint32_t out = 0;
     if (n > 0) {
-        const uint32_t last = kw[2ull * (n - 1) + 1];
+        const uint32_t last = kw[(size_t)stride * (n - 1) + word];
         if (t <= last + 1u) {
-            uint32_t lo = 0, hi = n;  // first index with hi-word >= t
+            uint32_t lo = 0, hi = n;  // first index whose tile id >= t
             while (lo < hi) {
                 const uint32_t mid = lo + ((hi - lo) >> 1);
-                if (kw[2ull * mid + 1] < t) lo = mid + 1; else hi = mid;
+                if (kw[(size_t)stride * mid + word] < t) lo = mid + 1; else hi = mid;
             }
             out = lo;
         }
     }
     tile_offsets[t] = out;
+}
+
+__global__ __launch_bounds__(256) void k_rebuild_keys(const uint32_t* __restrict__ tiles,
+                                                      const uint32_t* __restrict__ values,
+                                                      const gs3d_projected_splat* __restrict__ projected,
+                                                      const uint32_t* __restrict__ n_dev, uint32_t n_max,
+                                                      uint64_t* __restrict__ keys) {
+    const uint32_t n = min(*n_dev, n_max);
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const float depth = projected[values[i]].depth;
+        keys[i] = ((uint64_t)tiles[i] << 32) | (uint64_t)__float_as_uint(-depth);
+    }
 }
 
 // Conservative test of one gathered splat against a wavefront's rectangle of
@@ -269,12 +284,22 @@ __global__ void k_f16_to_f32(const uint16_t* in, float* out, size_t n) {
 
 }  // namespace
 
-hipError_t launch_identify_tiles(const uint64_t* keys, const uint32_t* n_dev, uint32_t n_max,
-                                 uint32_t* tile_offsets, uint32_t tile_count, uint32_t* k_host,
-                                 hipStream_t s) {
-    const uint32_t threads = tile_count + 1u;
-    hipLaunchKernelGGL(k_identify, dim3((threads + 255u) / 256u), dim3(256), 0, s, keys, n_dev,
-                       tile_offsets, tile_count, n_max, k_host);
+hipError_t launch_identify_tiles(const void* keys, uint32_t key_bytes, const uint32_t* n_dev, uint32_t n_max,
+                                 uint32_t* tile_offsets, uint32_t tile_count, uint32_t* k_host, hipStream_t s) {
+    const uint32_t threads = tile_count + 1;
+    const uint32_t stride = key_bytes == 8 ? 2u : 1u, word = key_bytes == 8 ? 1u : 0u;
+    hipLaunchKernelGGL(k_identify, dim3((threads + 255u) / 256u), dim3(256), 0, s,
+                       static_cast<const uint32_t*>(keys), stride, word, n_dev, tile_offsets, tile_count, n_max,
+                       k_host);
+    return hipGetLastError();
+}
+
+hipError_t launch_rebuild_keys(const uint32_t* tiles, const uint32_t* values, const gs3d_projected_splat* projected,
+                               const uint32_t* n_dev, uint32_t n_max, uint64_t* keys, hipStream_t s) {
+    if (n_max == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((n_max + 255u) / 256u, 8192u);
+    hipLaunchKernelGGL(k_rebuild_keys, dim3(blocks), dim3(256), 0, s, tiles, values, projected, n_dev, n_max,
+                       keys);
     return hipGetLastError();
 }
 

@@ -271,6 +271,7 @@ constexpr int kSort64Threads = 1024, kSort64Items = 16;
 constexpr int kSort64SmallThreads = 512, kSort64SmallItems = 16;
 constexpr int kSort32Threads = 1024, kSort32Items = 8;
 constexpr int kSort32SmallThreads = 512, kSort32SmallItems = 8;
+constexpr int kSort32LargeThreads = 1024, kSort32LargeItems = 16;
 
 template <int BITS, typename KeyT, bool IOTA, bool AUX, int THREADS, int ITEMS>
 static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
@@ -285,12 +286,13 @@ static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32
 }
 
 template <int BITS, typename KeyT, bool IOTA, bool AUX>
-static void launch_shape(bool small_tiles, const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
+static void launch_shape(int shape, const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                          const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                          uint32_t* ticket, uint32_t shift, uint32_t epoch, const uint32_t* aux_in,
                          uint32_t* aux_out, hipStream_t s, const uint32_t* epoch_base, uint32_t match_bits) {
+    // shape: -1 the half-size tile, 0 the default, 1 the double-size tile (32-bit keys only)
     if constexpr (sizeof(KeyT) == 8) {
-        if (small_tiles)
+        if (shape < 0)
             launch_pass<BITS, KeyT, IOTA, AUX, kSort64SmallThreads, kSort64SmallItems>(
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
@@ -299,8 +301,12 @@ static void launch_shape(bool small_tiles, const void* kin, const uint32_t* vin,
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
     } else {
-        if (small_tiles)
+        if (shape < 0)
             launch_pass<BITS, KeyT, IOTA, AUX, kSort32SmallThreads, kSort32SmallItems>(
+                kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
+                epoch_base, match_bits);
+        else if (shape > 0 && !AUX)
+            launch_pass<BITS, KeyT, IOTA, AUX, kSort32LargeThreads, kSort32LargeItems>(
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
         else
@@ -319,15 +325,15 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
     if (n_max == 0) return hipSuccess;
     if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
     // key_tile: keys per workgroup, 0 = the default shape for the key width
-    // (16,384 for 64-bit keys, 8,192 for 32-bit keys); the half-size shape is
-    // 8,192 (64-bit) or 4,096 (32-bit)
-    bool small_tiles;
+    // (16,384 for 64-bit keys, 8,192 for 32-bit keys); 64-bit: 8,192 or
+    // 16,384; 32-bit: 4,096, 8,192 or 16,384
+    int shape;  // -1 the half-size tile, 0 the default, 1 the double-size tile
     if (key_bytes == 8) {
         if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
-        small_tiles = key_tile == 8192 || (key_tile == 0 && n_max >= kSortSmallTileFrom);
+        shape = (key_tile == 8192 || (key_tile == 0 && n_max >= kSortSmallTileFrom)) ? -1 : 0;
     } else {
-        if (key_tile != 0 && key_tile != 4096 && key_tile != 8192) return hipErrorInvalidValue;
-        small_tiles = key_tile == 4096;
+        if (key_tile != 0 && key_tile != 4096 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
+        shape = key_tile == 4096 ? -1 : (key_tile == 16384 ? 1 : 0);
     }
     const bool aux = aux_in != nullptr && aux_out != nullptr;
     // the gather rides on 32-bit key passes whose values are read from vin
@@ -335,7 +341,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
     const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
                     (iota_values ? 1 : 0) + (aux ? 2 : 0);
 #define GS3D_PASS(B, K, I, A)                                                                      \
-    launch_shape<B, K, I, A>(small_tiles, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, \
+    launch_shape<B, K, I, A>(shape, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, \
                              epoch, aux_in, aux_out, s, epoch_base, match_bits)
     switch (key) {
         case 110: GS3D_PASS(8, uint64_t, false, false); break;

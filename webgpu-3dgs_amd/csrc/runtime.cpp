@@ -67,7 +67,7 @@ struct gs3d_ctx {
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
     uint32_t raster_ppl = 1;
-    uint32_t sort_key_tile = 0;    // 64-bit key passes: 0 = by key count, or 8192 / 16384
+    uint32_t sort_key_tile = 0;    // the frame's tile-id passes (and stage sorts of 64-bit keys): 0, 8192, 16384
     uint32_t sort_key_tile32 = 0;  // 32-bit key passes: 0 = 8192, or 4096
     uint64_t key_capacity = 0;
     uint32_t n_passes = 0;
@@ -281,10 +281,14 @@ int set_device(gs3d_ctx* c) {
 // input; ticket: one counter per pass; first_iota: pass 0 takes the
 // positions as values; aux_in/aux_out (or null): the last pass also writes
 // aux_out[p] = aux_in[value at p]. Sets *in_b when the result ends in b.
+// Keys per workgroup of the tile-id passes (32-bit keys): 16,384 unless set.
+uint32_t tile_key_tile(const gs3d_ctx* c) { return c->sort_key_tile ? c->sort_key_tile : 16384u; }
+
 int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint32_t key_bytes,
                bool first_iota, const uint32_t* n_dev, uint64_t n_max, const uint32_t* shifts,
                uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b,
-               const uint32_t* aux_in = nullptr, uint32_t* aux_out = nullptr, uint32_t key_bits = 64) {
+               const uint32_t* aux_in = nullptr, uint32_t* aux_out = nullptr, uint32_t key_bits = 64,
+               uint32_t key_tile = 0) {
     uint64_t* radix_status = nullptr;
     const uint32_t radix = 1u << c->digit_bits;
     GS3D_TRY(ensure_status(c, "radix_status",
@@ -295,8 +299,7 @@ int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint
                                      key_bytes, first_iota && p == 0, n_dev, (uint32_t)n_max,
                                      hist + p * radix, radix_status, tickets + p, shifts[p],
                                      c->digit_bits, next_epoch(c), p + 1 == passes ? aux_in : nullptr,
-                                     p + 1 == passes ? aux_out : nullptr, c->stream, nullptr,
-                                     key_bytes == 8 ? c->sort_key_tile : c->sort_key_tile32,
+                                     p + 1 == passes ? aux_out : nullptr, c->stream, nullptr, key_tile,
                                      pass_match_bits(shifts[p], c->digit_bits, key_bits)));
     }
     *in_b = (passes % 2) == 1;
@@ -414,7 +417,8 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     bool order_in_b = false;
     // pass 0 takes the positions as values (ida is not read); odd passes write ida
     GS3D_TRY(run_passes(c, dka, ida, dkb, idb, 4, true, counters + kCtrSplatCount, n, c->shifts,
-                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b, nullptr, nullptr, 32));
+                        depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b, nullptr, nullptr, 32,
+                        c->sort_key_tile32));
     const uint32_t* order = order_in_b ? idb : ida;
     const uint32_t* depth_sorted = order_in_b ? dkb : dka;
     record(c, GS3D_STAGE_DEPTH_SORT, 1);
@@ -441,7 +445,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     ea.projected = projected;
     ea.order = order;
     ea.recs = recs;
-    ea.keys = ka;
+    ea.tiles = reinterpret_cast<uint32_t*>(ka);  // tile ids; the sort_keys buffers hold them as u32
     ea.values = va;
     ea.hist = tile_passes ? hist_tile : nullptr;
     ea.hist_partial = tile_passes ? hist_partial : nullptr;
@@ -455,15 +459,18 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     record(c, GS3D_STAGE_EMIT, 1);
 
     record(c, GS3D_STAGE_SORT, 0);
-    GS3D_TRY(run_passes(c, ka, va, kb, vb, 8, false, ref_counter, cap, c->shifts + depth_passes,
-                        tile_passes, hist_tile, counters + kCtrSortTicket + depth_passes,
-                        &c->result_in_b, nullptr, nullptr, 32u + (T > 1 ? bit_length(T - 1) : 0u)));
+    // the tile digits of the keys are the digits of the 32-bit tile ids, shifted by 32
+    uint32_t tile_shifts[kMaxPasses];
+    for (uint32_t p = 0; p < tile_passes; p++) tile_shifts[p] = c->shifts[depth_passes + p] - 32u;
+    GS3D_TRY(run_passes(c, ka, va, kb, vb, 4, false, ref_counter, cap, tile_shifts, tile_passes, hist_tile,
+                        counters + kCtrSortTicket + depth_passes, &c->result_in_b, nullptr, nullptr,
+                        T > 1 ? bit_length(T - 1) : 0u, tile_key_tile(c)));
     record(c, GS3D_STAGE_SORT, 1);
-    const uint64_t* sorted_keys = c->result_in_b ? kb : ka;
+    const uint32_t* sorted_tiles = reinterpret_cast<const uint32_t*>(c->result_in_b ? kb : ka);
     const uint32_t* sorted_vals = c->result_in_b ? vb : va;
 
     record(c, GS3D_STAGE_IDENTIFY, 0);
-    HIP_TRY(c, launch_identify_tiles(sorted_keys, ref_counter, (uint32_t)cap, tile_offsets, T,
+    HIP_TRY(c, launch_identify_tiles(sorted_tiles, 4, ref_counter, (uint32_t)cap, tile_offsets, T,
                                      c->pinned, c->stream));
     record(c, GS3D_STAGE_IDENTIFY, 1);
 
@@ -687,15 +694,14 @@ int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, u
                  uint32_t key_bytes, bool first_iota, const uint32_t* n_dev, uint64_t n_max,
                  const uint32_t* shifts, uint32_t passes, const uint32_t* hist, uint32_t* tickets,
                  uint64_t* status, const uint32_t* epoch_base, uint32_t* epoch_ctr, bool* in_b,
-                 uint32_t key_bits) {
+                 uint32_t key_bits, uint32_t key_tile) {
     const uint32_t radix = 1u << c->digit_bits;
     for (uint32_t p = 0; p < passes; p++) {
         const bool even = (p % 2) == 0;
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va, key_bytes,
                                      first_iota && p == 0, n_dev, (uint32_t)n_max, hist + p * radix, status,
                                      tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr, nullptr, nullptr, s,
-                                     epoch_base, key_bytes == 8 ? c->sort_key_tile : c->sort_key_tile32,
-                                     pass_match_bits(shifts[p], c->digit_bits, key_bits)));
+                                     epoch_base, key_tile, pass_match_bits(shifts[p], c->digit_bits, key_bits)));
     }
     *in_b = (passes % 2) == 1;
     return GS3D_OK;
@@ -777,7 +783,7 @@ int batch_enqueue(gs3d_ctx* c) {
         bool order_in_b = false;
         GS3D_TRY(batch_passes(c, s, dka, ida, dkb, idb, 4, true, ctr + kCtrSplatCount, n, shifts, depth_passes,
                               hist_depth, ctr + kCtrSortTicket, radix_status, epoch_base, &epoch_ctr[k],
-                              &order_in_b, 32));
+                              &order_in_b, 32, c->sort_key_tile32));
         ScanRectsArgs sa{};
         sa.order = order_in_b ? idb : ida;
         sa.depth_keys = order_in_b ? dkb : dka;
@@ -795,7 +801,7 @@ int batch_enqueue(gs3d_ctx* c) {
         ea.projected = proj;
         ea.order = sa.order;
         ea.recs = recs;
-        ea.keys = ka;
+        ea.tiles = reinterpret_cast<uint32_t*>(ka);
         ea.values = va;
         ea.hist = tile_passes ? hist_tile : nullptr;
         ea.hist_partial = tile_passes ? hist_partial : nullptr;
@@ -807,11 +813,13 @@ int batch_enqueue(gs3d_ctx* c) {
         ea.hist_shift0 = 32;
         HIP_TRY(c, launch_emit(ea, s));
         bool in_b = false;
-        GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, 8, false, ctr + kCtrRefCounter, cap, shifts + depth_passes,
+        uint32_t tile_shifts[kMaxPasses];
+        for (uint32_t p = 0; p < tile_passes; p++) tile_shifts[p] = shifts[depth_passes + p] - 32u;
+        GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, 4, false, ctr + kCtrRefCounter, cap, tile_shifts,
                               tile_passes, hist_tile, ctr + kCtrSortTicket + depth_passes, radix_status,
-                              epoch_base, &epoch_ctr[k], &in_b, 32u + (T > 1 ? bit_length(T - 1) : 0u)));
+                              epoch_base, &epoch_ctr[k], &in_b, T > 1 ? bit_length(T - 1) : 0u, tile_key_tile(c)));
         uint32_t* to = tile_offsets + v.to_off;
-        HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
+        HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, 4, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
                                          b->pinned_k + vi, s));
         HIP_TRY(c, launch_rasterize(proj, in_b ? vb : va, to, c->tile_x, c->tile_y, v.tiles_x, v.tiles_y, v.W,
                                     v.H, c->raster_ppl, images + v.img_off, s));
@@ -1325,7 +1333,19 @@ int gs3d_read(gs3d_ctx* c, int id, void* dst, size_t bytes, size_t* written) {
             break;
         }
         case GS3D_BUF_REF_COUNTER: src = buf<uint32_t>(c, "counters") + kCtrRefCounter; size = 4; break;
-        case GS3D_BUF_SORT_KEYS: src = buf<void>(c, c->result_in_b ? "sort_keys_b" : "sort_keys_a"); size = K * 8; break;
+        case GS3D_BUF_SORT_KEYS: {
+            // the frame sorts tile ids; the (tile << 32 | depth) keys are rebuilt
+            // from each key's splat
+            uint64_t* rk;
+            GS3D_TRY(ensure_t(c, "readback_keys", std::max<uint64_t>(K, 1), &rk));
+            HIP_TRY(c, launch_rebuild_keys(buf<uint32_t>(c, c->result_in_b ? "sort_keys_b" : "sort_keys_a"),
+                                           buf<uint32_t>(c, c->result_in_b ? "sort_values_b" : "sort_values_a"),
+                                           buf<gs3d_projected_splat>(c, "projected_splats"),
+                                           buf<uint32_t>(c, "counters") + kCtrRefCounter, (uint32_t)K, rk, c->stream));
+            src = rk;
+            size = K * 8;
+            break;
+        }
         case GS3D_BUF_SORT_VALUES: src = buf<void>(c, c->result_in_b ? "sort_values_b" : "sort_values_a"); size = K * 4; break;
         case GS3D_BUF_TILE_OFFSETS: src = buf<void>(c, "tile_offsets"); size = ((size_t)c->tiles_x * c->tiles_y + 1) * 4; break;
         case GS3D_BUF_IMAGE_RGBA16F: src = buf<void>(c, "splat_target"); size = px * 8; break;
@@ -1690,7 +1710,7 @@ int gs3d_stage_sort(gs3d_ctx* c, const uint64_t* keys_in, const uint32_t* values
     c->digit_bits = digit_bits;
     bool in_b = false;
     int rc = run_passes(c, ka, va, kb, vb, 8, false, ctr + kCtrRefCounter, n, shifts, passes,
-                        ctr + kCtrHist, ctr + kCtrSortTicket, &in_b);
+                        ctr + kCtrHist, ctr + kCtrSortTicket, &in_b, nullptr, nullptr, 64, c->sort_key_tile);
     c->digit_bits = saved_bits;
     if (rc != GS3D_OK) return rc;
     HIP_TRY(c, hipMemcpyAsync(keys_out, in_b ? kb : ka, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1713,7 +1733,7 @@ int gs3d_stage_identify(gs3d_ctx* c, const uint64_t* sorted_keys, uint32_t n, ui
     host_ctr[kCtrRefCounter] = n;
     if (n) HIP_TRY(c, hipMemcpyAsync(dk, sorted_keys, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(ctr, host_ctr, sizeof host_ctr, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, launch_identify_tiles(dk, ctr + kCtrRefCounter, n, dt, tile_count, nullptr, c->stream));
+    HIP_TRY(c, launch_identify_tiles(dk, 8, ctr + kCtrRefCounter, n, dt, tile_count, nullptr, c->stream));
     HIP_TRY(c, hipMemcpyAsync(tile_offsets, dt, ((size_t)tile_count + 1) * 4, hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
