@@ -329,6 +329,21 @@ def main() -> int:
         tf = visited * RASTER_FLOP_PER_PAIR / (us * 1e-6) / 1e12
         per_stage["rasterize-tiles"].update({"pairs": visited, "TFLOP/s": round(tf, 2),
                                              "valu_frac": round(tf / FP32_PEAK_TFLOPS, 4)})
+    sort_us = stages.get("radix-sort", 0.0) + stages.get("radix-sort-splats", 0.0)
+    if sort_us > 0:
+        # The whole sort priced on the reference's layout: one 64-bit key
+        # (tile << 32 | depth) + 32-bit id per reference, sorted in
+        # ceil((32 + ceil(log2 T)) / 8) 8-bit passes of 24 B per key each, plus
+        # one 8-B histogram read (sort.wgsl). This pipeline reaches the same
+        # order with four 32-bit passes over the splats and then passes over
+        # 32-bit tile ids; the figure measures both against the bytes the
+        # reference's sort has to move.
+        ref_passes = -(-(32 + (max(1, T) - 1).bit_length()) // 8)
+        ref_bytes = 8 * k + ref_passes * 24 * k
+        gbs = ref_bytes / (sort_us * 1e-6) / 1e9
+        per_stage["sort-reference-layout"] = {"us": round(sort_us, 2), "algo_bytes": int(ref_bytes),
+                                              "passes": ref_passes, "GB/s": round(gbs, 1),
+                                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     d = per_stage[dominant]
     d["us_timed_region"] = round(dominant_us, 2)
     d["GB/s"] = round(algo[dominant] / (dominant_us * 1e-6) / 1e9, 1)

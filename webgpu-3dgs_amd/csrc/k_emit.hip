@@ -5,9 +5,9 @@
 // key = (ty*tiles_x + tx) << 32 | bits(-depth), value = id. Instead of one
 // thread looping over one splat's tiles (divergent, uncoalesced), a workgroup
 // owns 256 consecutive splats and therefore one contiguous output range
-// [offsets[i0], offsets[i0+256]); thread j of the range finds its splat by a
-// direct lookup in an LDS owner table (one byte store per key to build), so
-// every store of keys and ids is coalesced. In the frame the splats come in
+// [offsets[i0], offsets[i0+256]); thread j of the range finds its splat from
+// the splats' first slots marked in LDS (see emit_chunk), so every store of
+// keys and ids is coalesced and no thread loops over a splat's tiles. In the frame the splats come in
 // depth order with per-position records from k_scan_rects (first key, first
 // tile, width, depth key), read coalesced; with offsets instead, the tile
 // rectangle and depth are read from the projected splats.
@@ -28,23 +28,45 @@ namespace {
 constexpr int kEmitThreads = 256;
 
 // One 256-splat chunk: stage the chunk's splats in LDS, then emit its
-// contiguous key range with every store coalesced. The range is walked in
-// windows of kOwnerWindow keys: each splat first marks the window slots it owns
-// (owner[x] = local splat index), then thread j reads its owner directly.
+// contiguous key range. The range is walked in windows of kOwnerWindow keys.
+// Each splat that overlaps a window marks only its first slot there (owner
+// byte + one bit of a slot mask; the ranges are disjoint and slot 0 always
+// belongs to some splat); one wavefront then max-scans the mask words, so a
+// slot's splat is the last mark at or before it, found in O(1) whatever the
+// splats' sizes. A thread takes kSlotsPerThread consecutive slots: one lookup
+// and one division for the first, then the next slot is the next tile of the
+// same row, the first tile of its next row, or the first tile of the splat
+// that starts there (its mask bit). Two slots per thread measured fastest
+// (halves the dependent LDS reads per key; with four, the stores of a
+// wavefront stride 16 B and the kernel slows down).
 constexpr uint32_t kOwnerWindow = 4096;
+constexpr uint32_t kSlotsPerThread = 2;
+constexpr uint32_t kMarkWords = kOwnerWindow / 32;
+static_assert(kEmitThreads >= kWave + kMarkWords, "mask clearing needs its own threads");
+static_assert(kMarkWords == 2 * kWave, "the mask scan gives each lane of one wavefront two words");
+static_assert(32 % kSlotsPerThread == 0, "a thread's slots share one mask word");
 
-__device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint32_t* s_start,
-                                           uint32_t* s_tile0, uint32_t* s_width, uint32_t* s_keylo,
-                                           uint32_t* s_sid, uint8_t* s_owner, uint32_t* s_hist) {
+struct EmitShared {
+    uint32_t start[kEmitThreads];  // first slot of the splat, relative to the chunk's first key
+    uint32_t tile0[kEmitThreads];  // tile id of the splat's top-left tile
+    uint32_t width[kEmitThreads];  // rectangle width in tiles
+    uint32_t keylo[kEmitThreads];  // depth key
+    uint32_t sid[kEmitThreads];    // splat id
+    uint32_t mark[2][kMarkWords];  // slot masks, alternating; the one a window uses starts clear
+    int32_t prev[kMarkWords];      // last marked slot before each mask word (-1: none)
+    uint8_t owner[kOwnerWindow];   // at marked slots: the chunk-local splat
+    uint32_t hist[kMaxPasses * 256];
+};
+
+__device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, EmitShared& s, uint32_t& cur) {
     const int tid = threadIdx.x;
     const uint32_t i = i0 + tid;
     const uint32_t last = min(i0 + kEmitThreads, a.n);
-    const uint32_t radix = 1u << a.digit_bits;
     // first key of position p: the records' start column or the offsets
     auto first_key = [&](uint32_t p) { return a.recs ? a.recs[p].x : a.offsets[p]; };
     const uint32_t block_start = first_key(i0);
     const uint32_t block_end = first_key(last);
-    uint32_t my_start = 0, my_count = 0, sid = i;
+    uint32_t my_start = 0, my_count = 0, sid = i, width = 1;
     if (i < a.n) {
         if (a.order) sid = a.order[i];
         if (a.recs) {
@@ -52,9 +74,9 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
             const uint4 rec = a.recs[i];
             my_count = a.recs[i + 1].x - rec.x;
             my_start = rec.x - block_start;
-            s_tile0[tid] = rec.y;
-            s_width[tid] = rec.z;
-            s_keylo[tid] = rec.w;
+            s.tile0[tid] = rec.y;
+            width = rec.z;
+            s.keylo[tid] = rec.w;
         } else {
             const uint32_t off = a.offsets[i];
             my_count = a.offsets[i + 1] - off;  // > 0 exactly for valid splats
@@ -65,51 +87,101 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
                 const float4 f3 = ps[3];
                 const uint32_t t0x = __float_as_uint(f3.x), t0y = __float_as_uint(f3.y);
                 const uint32_t t1x = __float_as_uint(f3.z);
-                s_tile0[tid] = t0y * a.tiles_x + t0x;
-                s_width[tid] = t1x - t0x + 1u;
-                s_keylo[tid] = __float_as_uint(-f0.z);
+                s.tile0[tid] = t0y * a.tiles_x + t0x;
+                width = t1x - t0x + 1u;
+                s.keylo[tid] = __float_as_uint(-f0.z);
             }
         }
     }
-    s_start[tid] = my_start;
-    s_sid[tid] = sid;
+    if (my_count == 0) width = 1;
+    s.width[tid] = width;
+    s.start[tid] = my_start;
+    s.sid[tid] = sid;
 
     const uint32_t total = block_end - block_start;
     for (uint32_t w0 = 0; w0 < total; w0 += kOwnerWindow) {
         const uint32_t wn = min(kOwnerWindow, total - w0);
-        __syncthreads();  // previous window's owners consumed (and the staging above visible)
+        __syncthreads();  // previous window's slots consumed (and the staging above visible)
         const uint32_t lo = max(my_start, w0), hi = min(my_start + my_count, w0 + wn);
-        for (uint32_t x = lo; x < hi; x++) s_owner[x - w0] = (uint8_t)tid;
+        if (lo < hi) {
+            const uint32_t x = lo - w0;
+            s.owner[x] = (uint8_t)tid;
+            atomicOr(&s.mark[cur][x >> 5], 1u << (x & 31u));
+        }
         __syncthreads();
-        const uint32_t rounds = (wn + kEmitThreads - 1) / kEmitThreads;
-        for (uint32_t round = 0; round < rounds; round++) {
-            const uint32_t jw = round * kEmitThreads + tid;
-            const bool live = jw < wn;
-            uint64_t key = 0;
-            if (live) {
-                const uint32_t o = s_owner[jw];
-                const uint32_t r = w0 + jw - s_start[o];
-                const uint32_t w = s_width[o];
-                const uint32_t ry = r / w;
-                const uint32_t rx = r - ry * w;
-                const uint32_t tile = s_tile0[o] + ry * a.tiles_x + rx;
-                key = ((uint64_t)tile << 32) | (uint64_t)s_keylo[o];
-                const uint64_t pos = (uint64_t)block_start + w0 + jw;
-                if (pos < a.capacity) {
-                    if (a.tiles) a.tiles[pos] = tile;
-                    else a.keys[pos] = key;
-                    a.values[pos] = s_sid[o];
-                }
+        if (tid < kWave) {
+            // lane l: mask words 2l and 2l+1; inclusive max-scan of their last marks
+            const uint32_t ma = s.mark[cur][2 * tid], mb = s.mark[cur][2 * tid + 1];
+            const int32_t la = ma ? (int32_t)(64 * tid + 31 - __clz((int)ma)) : -1;
+            const int32_t lb = mb ? (int32_t)(64 * tid + 63 - __clz((int)mb)) : -1;
+            int32_t v = la > lb ? la : lb;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int32_t t = __shfl_up(v, d, kWave);
+                if (tid >= d) v = t > v ? t : v;
             }
-            if (a.hist_partial && live) {
-                // plain LDS atomics: a wavefront's keys come from ~18 splats in
-                // depth order, so their tile digits rarely collide
-                for (uint32_t p = 0; p < a.n_passes; p++) {
-                    const uint32_t d = (uint32_t)(key >> (a.hist_shift0 + p * a.digit_bits)) & (radix - 1u);
-                    atomicAdd(&s_hist[p * radix + d], 1u);
+            int32_t ex = __shfl_up(v, 1, kWave);
+            if (tid == 0) ex = -1;
+            s.prev[2 * tid] = ex;
+            s.prev[2 * tid + 1] = la > ex ? la : ex;
+        } else if (tid < kWave + (int)kMarkWords) {
+            s.mark[cur ^ 1u][tid - kWave] = 0u;  // the next window's mask
+        }
+        __syncthreads();
+        constexpr uint32_t SPT = kSlotsPerThread;
+        const uint32_t rounds = (wn + kEmitThreads * SPT - 1) / (kEmitThreads * SPT);
+        const uint32_t radix = 1u << a.digit_bits;
+        for (uint32_t round = 0; round < rounds; round++) {
+            const uint32_t g0 = (round * kEmitThreads + tid) * SPT;
+            if (g0 < wn) {
+                const uint32_t word = g0 >> 5;
+                const uint32_t mword = s.mark[cur][word];
+                const uint32_t m = mword & ((2u << (g0 & 31u)) - 1u);
+                const int32_t at = m ? (int32_t)((word << 5) + 31 - __clz((int)m)) : s.prev[word];
+                uint32_t o = s.owner[at];
+                const uint32_t r = w0 + g0 - s.start[o];
+                uint32_t w = s.width[o];
+                const uint32_t ry = r / w;
+                uint32_t rx = r - ry * w;
+                uint32_t tile = s.tile0[o] + ry * a.tiles_x + rx;
+                uint32_t keylo = s.keylo[o], sid = s.sid[o];
+#pragma unroll
+                for (uint32_t k = 0; k < SPT; k++) {
+                    const uint32_t slot = g0 + k;
+                    if (slot >= wn) break;
+                    if (k > 0) {
+                        if ((mword >> (slot & 31u)) & 1u) {
+                            // the next splat starts at this slot
+                            o = s.owner[slot];
+                            w = s.width[o];
+                            tile = s.tile0[o];
+                            keylo = s.keylo[o];
+                            sid = s.sid[o];
+                            rx = 0;
+                        } else if (++rx == w) {
+                            rx = 0;
+                            tile += a.tiles_x - w + 1u;
+                        } else {
+                            tile++;
+                        }
+                    }
+                    const uint64_t key = ((uint64_t)tile << 32) | (uint64_t)keylo;
+                    const uint64_t pos = (uint64_t)block_start + w0 + slot;
+                    if (pos < a.capacity) {
+                        if (a.tiles) a.tiles[pos] = tile;
+                        else a.keys[pos] = key;
+                        a.values[pos] = sid;
+                    }
+                    if (a.hist_partial) {
+                        for (uint32_t p = 0; p < a.n_passes; p++) {
+                            const uint32_t d = (uint32_t)(key >> (a.hist_shift0 + p * a.digit_bits)) & (radix - 1u);
+                            atomicAdd(&s.hist[p * radix + d], 1u);
+                        }
+                    }
                 }
             }
         }
+        cur ^= 1u;
     }
     __syncthreads();  // LDS staging is reused by the next chunk
 }
@@ -118,23 +190,18 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, uint3
 // across a workgroup's chunks and leave as one plain row per workgroup
 // (hist_partial[block][pass][digit]); k_hist_reduce folds the rows.
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
-    __shared__ uint32_t s_start[kEmitThreads];
-    __shared__ uint32_t s_tile0[kEmitThreads];
-    __shared__ uint32_t s_width[kEmitThreads];
-    __shared__ uint32_t s_keylo[kEmitThreads];
-    __shared__ uint32_t s_sid[kEmitThreads];
-    __shared__ uint8_t s_owner[kOwnerWindow];
-    __shared__ uint32_t s_hist[kMaxPasses * 256];
+    __shared__ EmitShared s;
     const uint32_t radix = 1u << a.digit_bits;
     const uint32_t hist_words = a.hist_partial ? a.n_passes * radix : 0u;
-    for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) s_hist[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) s.hist[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < 2 * kMarkWords; k += kEmitThreads) (&s.mark[0][0])[k] = 0u;
+    uint32_t cur = 0;
     const uint32_t chunks = (a.n + kEmitThreads - 1) / kEmitThreads;
-    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x)
-        emit_chunk(a, c * kEmitThreads, s_start, s_tile0, s_width, s_keylo, s_sid, s_owner, s_hist);
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) emit_chunk(a, c * kEmitThreads, s, cur);
     if (a.hist_partial) {
         __syncthreads();
         uint32_t* row = a.hist_partial + (size_t)blockIdx.x * hist_words;
-        for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) row[k] = s_hist[k];
+        for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) row[k] = s.hist[k];
     }
 }
 
