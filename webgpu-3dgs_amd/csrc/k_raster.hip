@@ -138,7 +138,7 @@ struct alignas(16) Survivor {
 // zero-opacity entries, which fail alpha >= 1/255 and blend nothing. The
 // composite loop reads 4 slots at a time with uniform-address LDS loads (a
 // broadcast) and blends branch-free: a splat that fails a rule is blended with
-// alpha 0, which leaves C and T bit-identical (T * (1 - 0) = T,
+// alpha 0, which leaves C and T bit-identical (T - T * 0 = T,
 // C + T * 0 * c = C). The all-saturated exit is checked once per 4 splats.
 template <int TW, int PPL>
 __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __restrict__ proj,
@@ -201,7 +201,9 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __res
             c0[k] = fmaf(w, s.b.w, c0[k]);
             c1[k] = fmaf(w, s.c.x, c1[k]);
             c2[k] = fmaf(w, s.c.y, c2[k]);
-            tr[k] = tr[k] * (1.0f - ae);
+            // T (1 - a) as T - T a: one instruction instead of two, the same
+            // value up to rounding; a rejected splat (a = 0) leaves T bit-identical
+            tr[k] = tr[k] - w;
         }
     };
 

@@ -264,9 +264,11 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
 uint32_t radix_status_tiles(uint32_t n_max) { return (n_max + kSortTile - 1) / kSortTile + 1; }
 
 // Tile shapes (threads x keys per thread). 64-bit keys: 16,384-key tiles
-// (145 KB of LDS, one workgroup per CU) keep the lookback chain short while
-// the keys are few; 8,192-key tiles (73 KB, two workgroups per CU, so one
-// loads while the other scatters) when there are many. 32-bit keys: 8,192.
+// (145 KB of LDS, one workgroup per CU) by default, 8,192 (73 KB, two
+// workgroups per CU) on request. 32-bit keys: 8,192 by default (the depth
+// passes over the splats), 16,384 for the tile-id passes over the references
+// (fewer tiles in the lookback chain; measured faster at every size), 4,096
+// on request.
 constexpr int kSort64Threads = 1024, kSort64Items = 16;
 constexpr int kSort64SmallThreads = 512, kSort64SmallItems = 16;
 constexpr int kSort32Threads = 1024, kSort32Items = 8;
@@ -301,18 +303,21 @@ static void launch_shape(int shape, const void* kin, const uint32_t* vin, void* 
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
     } else {
-        if (shape < 0)
+        if (shape < 0) {
             launch_pass<BITS, KeyT, IOTA, AUX, kSort32SmallThreads, kSort32SmallItems>(
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
-        else if (shape > 0 && !AUX)
-            launch_pass<BITS, KeyT, IOTA, AUX, kSort32LargeThreads, kSort32LargeItems>(
-                kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
-                epoch_base, match_bits);
-        else
+        } else if (shape > 0) {
+            // the 16,384-key shape has no gather variant (rejected by launch_radix_pass)
+            if constexpr (!AUX)
+                launch_pass<BITS, KeyT, IOTA, AUX, kSort32LargeThreads, kSort32LargeItems>(
+                    kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out,
+                    s, epoch_base, match_bits);
+        } else {
             launch_pass<BITS, KeyT, IOTA, AUX, kSort32Threads, kSort32Items>(
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
+        }
     }
 }
 
@@ -326,7 +331,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
     if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
     // key_tile: keys per workgroup, 0 = the default shape for the key width
     // (16,384 for 64-bit keys, 8,192 for 32-bit keys); 64-bit: 8,192 or
-    // 16,384; 32-bit: 4,096, 8,192 or 16,384
+    // 16,384; 32-bit: 4,096, 8,192 or 16,384 (16,384 not with the gather)
     int shape;  // -1 the half-size tile, 0 the default, 1 the double-size tile
     if (key_bytes == 8) {
         if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
@@ -337,7 +342,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
     }
     const bool aux = aux_in != nullptr && aux_out != nullptr;
     // the gather rides on 32-bit key passes whose values are read from vin
-    if (aux && (key_bytes != 4 || iota_values)) return hipErrorInvalidValue;
+    if (aux && (key_bytes != 4 || iota_values || shape > 0)) return hipErrorInvalidValue;
     const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
                     (iota_values ? 1 : 0) + (aux ? 2 : 0);
 #define GS3D_PASS(B, K, I, A)                                                                      \
