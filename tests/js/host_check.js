@@ -156,7 +156,47 @@ function addonExports() {
   return { ok: true, names: Object.keys(addon).sort(), abiVersion: addon.abiVersion };
 }
 
-const HANDLERS = { ply, plyTime, batch, camera, workgroups, pipelines, render, addonExports };
+// Wrong argument types throw a TypeError naming the call instead of becoming 0.
+function argErrors(req) {
+  const addon = host.loadAddon();
+  const out = {};
+  const attempt = (name, fn) => {
+    try {
+      fn();
+      out[name] = { threw: false };
+    } catch (e) {
+      out[name] = { threw: true, type: e.constructor.name, message: String(e.message) };
+    }
+  };
+  attempt('stageLabel', () => addon.stageLabel('preprocess'));
+  attempt('parsePly', () => addon.parsePly(new Uint8Array(8), 'four'));
+  if (req.noGpu) return { ok: true, results: out };
+  const ctx = addon.create(0);
+  try {
+    const cam = new Float32Array(72);
+    attempt('setTileSize', () => addon.setTileSize(ctx, '16', 16));
+    attempt('rebin', () => addon.rebin(ctx, cam, 320, 'tall'));
+    attempt('rebinFlags', () => addon.rebin(ctx, cam, 320, 240, 'profile'));
+    attempt('read', () => addon.read(ctx, 'image'));
+    attempt('batchRead', () => addon.batchRead(ctx, -1, 0));
+    attempt('bufferCreate', () => addon.bufferCreate(ctx, 'scratch_a', -4));
+    attempt('bufferCreateName', () => addon.bufferCreate(ctx, 17, 64));
+    attempt('bufferRead', () => addon.bufferRead(ctx, 'scratch_a', 0.5, 4));
+    attempt('bufferWrite', () => addon.bufferWrite(ctx, 'scratch_a', NaN, new Uint8Array(4)));
+    attempt('bufferResize', () => addon.bufferResize(ctx, 'scratch_a', 'big'));
+    attempt('learnedEnable', () => addon.learnedEnable(ctx, 1));
+    attempt('learnedTrain', () => addon.learnedTrain(ctx, 2, -1));
+    attempt('loadPly', () => addon.loadPly(ctx, '/nonexistent.ply', 'many'));
+    // well-formed calls still work
+    addon.bufferCreate(ctx, 'scratch_b', 64);
+    out.bufferSizeOk = addon.bufferSize(ctx, 'scratch_b');
+  } finally {
+    addon.destroy(ctx);
+  }
+  return { ok: true, results: out };
+}
+
+const HANDLERS = { ply, plyTime, batch, camera, workgroups, pipelines, render, addonExports, argErrors };
 
 (async () => {
   const req = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));

@@ -377,6 +377,28 @@ def test_addon_exports(tmp_path):
     assert r["abiVersion"] >= 1
 
 
+@pytest.mark.skipif(not ADDON.exists(), reason="addon not built")
+def test_addon_rejects_wrong_argument_types_cpu(tmp_path):
+    """Entry points that need no device reject wrong-typed arguments with a TypeError."""
+    r = run_node({"cmd": "argErrors", "noGpu": True}, tmp_path)["results"]
+    for name in ("stageLabel", "parsePly"):
+        assert r[name]["threw"] and r[name]["type"] == "TypeError", (name, r[name])
+
+
+@pytest.mark.gpu
+def test_addon_rejects_wrong_argument_types(tmp_path):
+    """A wrong-typed argument throws a TypeError naming the call; nothing is silently
+    converted to 0 (which would, e.g., read view 0 or create an empty buffer)."""
+    r = run_node({"cmd": "argErrors"}, tmp_path, timeout=300)["results"]
+    for name in ("stageLabel", "parsePly", "setTileSize", "rebin", "rebinFlags", "read", "batchRead",
+                 "bufferCreate", "bufferCreateName", "bufferRead", "bufferWrite", "bufferResize",
+                 "learnedEnable", "learnedTrain", "loadPly"):
+        assert r[name]["threw"] and r[name]["type"] == "TypeError", (name, r[name])
+    assert "batchRead(ctx, view, bufferId)" in r["batchRead"]["message"]
+    assert "buffer name" in r["bufferCreateName"]["message"]
+    assert r["bufferSizeOk"] == 64
+
+
 # ---------------------------------------------------------------- GPU render through node
 
 @pytest.mark.gpu

@@ -69,6 +69,51 @@ static int get_u32(napi_env env, napi_value v, uint32_t* out) {
     return napi_get_value_uint32(env, v, out) == napi_ok;
 }
 
+/* Argument conversions that throw a TypeError naming the call on failure
+ * (a wrong type never silently becomes 0). */
+static int arg_u32(napi_env env, napi_value v, uint32_t* out, const char* usage) {
+    double d;
+    if (napi_get_value_double(env, v, &d) != napi_ok || !(d >= 0.0) || d > 4294967295.0 || d != (double)(uint32_t)d) {
+        napi_throw_type_error(env, NULL, usage);
+        return 0;
+    }
+    *out = (uint32_t)d;
+    return 1;
+}
+
+static int arg_i32(napi_env env, napi_value v, int32_t* out, const char* usage) {
+    if (napi_get_value_int32(env, v, out) != napi_ok) {
+        napi_throw_type_error(env, NULL, usage);
+        return 0;
+    }
+    return 1;
+}
+
+static int is_nullish(napi_env env, napi_value v) {
+    napi_valuetype t;
+    return napi_typeof(env, v, &t) == napi_ok && (t == napi_undefined || t == napi_null);
+}
+
+/* A byte count or offset: a non-negative integer below 2^53. */
+static int arg_size(napi_env env, napi_value v, size_t* out, const char* usage) {
+    double d;
+    if (napi_get_value_double(env, v, &d) != napi_ok || !(d >= 0.0) || d > 9007199254740991.0 ||
+        d != (double)(uint64_t)d) {
+        napi_throw_type_error(env, NULL, usage);
+        return 0;
+    }
+    *out = (size_t)d;
+    return 1;
+}
+
+static int arg_bool(napi_env env, napi_value v, bool* out, const char* usage) {
+    if (napi_get_value_bool(env, v, out) != napi_ok) {
+        napi_throw_type_error(env, NULL, usage);
+        return 0;
+    }
+    return 1;
+}
+
 /* Data pointer + byte length of a TypedArray / ArrayBuffer / DataView;
  * null and undefined give NULL. */
 static int get_bytes(napi_env env, napi_value v, void** data, size_t* bytes) {
@@ -128,10 +173,18 @@ static napi_value pair(napi_env env, uint32_t a, uint32_t b) {
 static napi_value js_create(napi_env env, napi_callback_info info) {
     ARGS(1);
     int32_t dev = 0;
-    if (argc > 0) napi_get_value_int32(env, argv[0], &dev);
-    gs3d_ctx* c = NULL;
-    CHECK(env, NULL, gs3d_create(dev, &c), "gs3d_create");
+    if (argc > 0 && !is_nullish(env, argv[0]) && !arg_i32(env, argv[0], &dev, "create(device?)")) return NULL;
     gs3d_ctx** slot = (gs3d_ctx**)malloc(sizeof *slot);
+    if (!slot) {
+        napi_throw_error(env, NULL, "create: out of memory");
+        return NULL;
+    }
+    gs3d_ctx* c = NULL;
+    const int rc = gs3d_create(dev, &c);
+    if (rc != GS3D_OK) {
+        free(slot);
+        return throw_rc(env, NULL, rc, "gs3d_create");
+    }
     *slot = c;
     napi_value ext;
     NAPI_CALL(env, napi_create_external(env, slot, finalize_ctx, NULL, &ext));
@@ -190,8 +243,9 @@ static napi_value js_set_splats(napi_env env, napi_callback_info info) {
         }
     int32_t deg = 0;
     uint32_t n = 0;
-    napi_get_value_int32(env, argv[7], &deg);
-    get_u32(env, argv[8], &n);
+    if (!arg_i32(env, argv[7], &deg, "setSplats: shDegree must be an integer") ||
+        !arg_u32(env, argv[8], &n, "setSplats: n must be a non-negative integer"))
+        return NULL;
     for (int k = 0; k < 4; k++)
         if (b[k] < (size_t)n * 16) {
             napi_throw_range_error(env, NULL, "setSplats: vec4 array shorter than 16*n bytes");
@@ -213,8 +267,8 @@ static napi_value js_set_tile_size(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     uint32_t x = 0, y = 0, out[2];
-    get_u32(env, argv[1], &x);
-    get_u32(env, argv[2], &y);
+    if (!arg_u32(env, argv[1], &x, "setTileSize(ctx, x, y)") || !arg_u32(env, argv[2], &y, "setTileSize(ctx, x, y)"))
+        return NULL;
     CHECK(env, c, gs3d_set_tile_size(c, x, y, out), "gs3d_set_tile_size");
     return pair(env, out[0], out[1]);
 }
@@ -249,9 +303,9 @@ static napi_value js_rebin(napi_env env, napi_callback_info info) {
         return NULL;
     }
     uint32_t w = 0, h = 0, flags = 0;
-    get_u32(env, argv[2], &w);
-    get_u32(env, argv[3], &h);
-    if (argc > 4) get_u32(env, argv[4], &flags);
+    static const char* usage = "rebin(ctx, camera, width, height, flags?)";
+    if (!arg_u32(env, argv[2], &w, usage) || !arg_u32(env, argv[3], &h, usage)) return NULL;
+    if (argc > 4 && !is_nullish(env, argv[4]) && !arg_u32(env, argv[4], &flags, usage)) return NULL;
     CHECK(env, c, gs3d_rebin(c, (const gs3d_camera*)cam, w, h, flags), "gs3d_rebin");
     return NULL;
 }
@@ -273,7 +327,7 @@ static napi_value js_read(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     int32_t id = 0;
-    napi_get_value_int32(env, argv[1], &id);
+    if (!arg_i32(env, argv[1], &id, "read(ctx, bufferId)")) return NULL;
     size_t size = 0;
     CHECK(env, c, gs3d_read(c, id, NULL, 0, &size), "gs3d_read");
     void* data;
@@ -330,7 +384,7 @@ static napi_value js_profiler_reset(napi_env env, napi_callback_info info) {
 static napi_value js_stage_label(napi_env env, napi_callback_info info) {
     ARGS(1);
     int32_t i = 0;
-    napi_get_value_int32(env, argv[0], &i);
+    if (!arg_i32(env, argv[0], &i, "stageLabel(stage)")) return NULL;
     napi_value s;
     NAPI_CALL(env, napi_create_string_utf8(env, gs3d_stage_label(i), NAPI_AUTO_LENGTH, &s));
     return s;
@@ -341,18 +395,22 @@ static int get_name(napi_env env, napi_value v, char* buf, size_t cap) {
     return napi_get_value_string_utf8(env, v, buf, cap, &len) == napi_ok && len > 0 && len < cap - 1;
 }
 
+static int arg_name(napi_env env, napi_value v, char* buf, size_t cap) {
+    if (get_name(env, v, buf, cap)) return 1;
+    napi_throw_type_error(env, NULL, "buffer name must be a non-empty string of at most 62 bytes");
+    return 0;
+}
+
 static napi_value js_buffer_create(napi_env env, napi_callback_info info) {
     ARGS(3);
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     char name[64];
-    double bytes = 0;
-    if (!get_name(env, argv[1], name, sizeof name)) {
-        napi_throw_type_error(env, NULL, "buffer name must be a non-empty string");
+    size_t bytes = 0;
+    if (!arg_name(env, argv[1], name, sizeof name) ||
+        !arg_size(env, argv[2], &bytes, "bufferCreate(ctx, name, bytes)"))
         return NULL;
-    }
-    napi_get_value_double(env, argv[2], &bytes);
-    CHECK(env, c, gs3d_buffer_create(c, name, (size_t)bytes), "gs3d_buffer_create");
+    CHECK(env, c, gs3d_buffer_create(c, name, bytes), "gs3d_buffer_create");
     return NULL;
 }
 
@@ -361,10 +419,11 @@ static napi_value js_buffer_resize(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     char name[64];
-    double bytes = 0;
-    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
-    napi_get_value_double(env, argv[2], &bytes);
-    CHECK(env, c, gs3d_buffer_resize(c, name, (size_t)bytes), "gs3d_buffer_resize");
+    size_t bytes = 0;
+    if (!arg_name(env, argv[1], name, sizeof name) ||
+        !arg_size(env, argv[2], &bytes, "bufferResize(ctx, name, bytes)"))
+        return NULL;
+    CHECK(env, c, gs3d_buffer_resize(c, name, bytes), "gs3d_buffer_resize");
     return NULL;
 }
 
@@ -374,7 +433,7 @@ static napi_value js_buffer_size(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     char name[64];
-    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    if (!arg_name(env, argv[1], name, sizeof name)) return NULL;
     size_t bytes = 0;
     if (gs3d_buffer_size(c, name, &bytes) != GS3D_OK) return NULL;
     napi_value r;
@@ -388,9 +447,10 @@ static napi_value js_buffer_write(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     char name[64];
-    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
-    double off = 0;
-    napi_get_value_double(env, argv[2], &off);
+    size_t off = 0;
+    if (!arg_name(env, argv[1], name, sizeof name) ||
+        !arg_size(env, argv[2], &off, "bufferWrite(ctx, name, offset, data)"))
+        return NULL;
     void* data;
     size_t bytes;
     if (!get_bytes(env, argv[3], &data, &bytes)) {
@@ -398,7 +458,7 @@ static napi_value js_buffer_write(napi_env env, napi_callback_info info) {
         return NULL;
     }
     size_t written = 0;
-    int rc = gs3d_buffer_write(c, name, (size_t)off, data, bytes, &written);
+    int rc = gs3d_buffer_write(c, name, off, data, bytes, &written);
     if (rc != GS3D_OK && rc != GS3D_E_TOO_SMALL) return throw_rc(env, c, rc, "gs3d_buffer_write");
     napi_value r;
     napi_create_double(env, (double)written, &r);
@@ -410,14 +470,15 @@ static napi_value js_buffer_read(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     char name[64];
-    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
-    double off = 0, bytes = 0;
-    napi_get_value_double(env, argv[2], &off);
-    napi_get_value_double(env, argv[3], &bytes);
+    size_t off = 0, bytes = 0;
+    if (!arg_name(env, argv[1], name, sizeof name) ||
+        !arg_size(env, argv[2], &off, "bufferRead(ctx, name, offset, bytes)") ||
+        !arg_size(env, argv[3], &bytes, "bufferRead(ctx, name, offset, bytes)"))
+        return NULL;
     void* data;
     napi_value ab;
-    NAPI_CALL(env, napi_create_arraybuffer(env, (size_t)bytes, &data, &ab));
-    CHECK(env, c, gs3d_buffer_read(c, name, (size_t)off, data, (size_t)bytes), "gs3d_buffer_read");
+    NAPI_CALL(env, napi_create_arraybuffer(env, bytes, &data, &ab));
+    CHECK(env, c, gs3d_buffer_read(c, name, off, data, bytes), "gs3d_buffer_read");
     return ab;
 }
 
@@ -426,7 +487,7 @@ static napi_value js_buffer_destroy(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     char name[64];
-    if (!get_name(env, argv[1], name, sizeof name)) return NULL;
+    if (!arg_name(env, argv[1], name, sizeof name)) return NULL;
     CHECK(env, c, gs3d_buffer_destroy(c, name), "gs3d_buffer_destroy");
     return NULL;
 }
@@ -573,7 +634,9 @@ static napi_value parse_ply_bytes(napi_env env, const void* data, size_t bytes, 
 static napi_value js_parse_ply(napi_env env, napi_callback_info info) {
     ARGS(2);
     int32_t threads = 0;
-    if (argc > 1) napi_get_value_int32(env, argv[1], &threads);
+    if (argc > 1 && !is_nullish(env, argv[1]) &&
+        !arg_i32(env, argv[1], &threads, "parsePly(bytes: Buffer | ArrayBuffer | TypedArray | path, threads?)"))
+        return NULL;
     napi_valuetype t;
     napi_typeof(env, argv[0], &t);
     if (t == napi_string) {
@@ -672,7 +735,8 @@ static napi_value js_load_ply(napi_env env, napi_callback_info info) {
         return NULL;
     }
     int32_t threads = 0;
-    if (argc > 2) napi_get_value_int32(env, argv[2], &threads);
+    if (argc > 2 && !is_nullish(env, argv[2]) && !arg_i32(env, argv[2], &threads, "loadPly(ctx, path, threads?)"))
+        return NULL;
     gs3d_ply_info pi;
     CHECK(env, c, gs3d_load_ply(c, path, threads, &pi), "gs3d_load_ply");
     napi_value r, v, bounds;
@@ -758,8 +822,9 @@ static napi_value js_batch_read(napi_env env, napi_callback_info info) {
     if (!c) return NULL;
     uint32_t view = 0;
     int32_t id = 0;
-    get_u32(env, argv[1], &view);
-    napi_get_value_int32(env, argv[2], &id);
+    if (!arg_u32(env, argv[1], &view, "batchRead(ctx, view, bufferId)") ||
+        !arg_i32(env, argv[2], &id, "batchRead(ctx, view, bufferId)"))
+        return NULL;
     size_t size = 0;
     CHECK(env, c, gs3d_batch_read(c, view, id, NULL, 0, &size), "gs3d_batch_read");
     void* data;
@@ -775,7 +840,10 @@ static napi_value js_learned_init(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     int64_t seed = 0;
-    if (argc > 1) napi_get_value_int64(env, argv[1], &seed);
+    if (argc > 1 && !is_nullish(env, argv[1]) && napi_get_value_int64(env, argv[1], &seed) != napi_ok) {
+        napi_throw_type_error(env, NULL, "learnedInit(ctx, seed?)");
+        return NULL;
+    }
     CHECK(env, c, gs3d_learned_init(c, (uint64_t)seed), "gs3d_learned_init");
     return NULL;
 }
@@ -791,8 +859,12 @@ static napi_value js_learned_train(napi_env env, napi_callback_info info) {
         napi_throw_type_error(env, NULL, "learnedTrain(ctx, epochs, lr, lrLatent)");
         return NULL;
     }
-    if (argc > 2) napi_get_value_double(env, argv[2], &lr);
-    if (argc > 3) napi_get_value_double(env, argv[3], &lrz);
+    if ((argc > 2 && !is_nullish(env, argv[2]) && napi_get_value_double(env, argv[2], &lr) != napi_ok) ||
+        (argc > 3 && !is_nullish(env, argv[3]) && napi_get_value_double(env, argv[3], &lrz) != napi_ok) ||
+        !(lr > 0.0) || !(lrz >= 0.0)) {
+        napi_throw_type_error(env, NULL, "learnedTrain(ctx, epochs, lr > 0, lrLatent >= 0)");
+        return NULL;
+    }
     float* loss = (float*)calloc(epochs, sizeof(float));
     if (!loss) {
         napi_throw_error(env, NULL, "learnedTrain: out of memory");
@@ -841,7 +913,7 @@ static napi_value js_learned_enable(napi_env env, napi_callback_info info) {
     gs3d_ctx* c = get_ctx(env, argv[0]);
     if (!c) return NULL;
     bool on = false;
-    napi_get_value_bool(env, argv[1], &on);
+    if (!arg_bool(env, argv[1], &on, "learnedEnable(ctx, on: boolean)")) return NULL;
     CHECK(env, c, gs3d_learned_enable(c, on ? 1 : 0), "gs3d_learned_enable");
     return NULL;
 }

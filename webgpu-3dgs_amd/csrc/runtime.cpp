@@ -284,6 +284,11 @@ int set_device(gs3d_ctx* c) {
 // Keys per workgroup of the tile-id passes (32-bit keys): 16,384 unless set.
 uint32_t tile_key_tile(const gs3d_ctx* c) { return c->sort_key_tile ? c->sort_key_tile : 16384u; }
 
+// Key buffers hold at most 2^32 - 1 entries: K is a 32-bit count (the
+// reference's u32 ref_counter) and the kernels take the capacity as a 32-bit
+// bound, so a larger capacity would wrap there.
+uint64_t key_capacity_for(uint64_t want) { return std::min<uint64_t>(want, 0xFFFFFFFFull); }
+
 int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint32_t key_bytes,
                bool first_iota, const uint32_t* n_dev, uint64_t n_max, const uint32_t* shifts,
                uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b,
@@ -348,7 +353,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     c->profiled = (flags & GS3D_REBIN_PROFILE) != 0;
     c->keep_unsorted = (flags & GS3D_REBIN_KEEP_UNSORTED) != 0;
     GS3D_TRY(begin_profiled_frame(c));
-    if (c->key_capacity == 0) c->key_capacity = std::max<uint64_t>(8ull * n, 1u << 16);
+    if (c->key_capacity == 0) c->key_capacity = key_capacity_for(std::max<uint64_t>(8ull * n, 1u << 16));
     const uint64_t cap = c->key_capacity;
     const uint32_t bits = c->digit_bits;
     const uint32_t radix = 1u << bits;
@@ -520,7 +525,7 @@ int settle(gs3d_ctx* c) {
         c->k_pending = false;
         c->K = c->pinned[0];
         if (c->K <= c->key_capacity) return GS3D_OK;
-        c->key_capacity = c->K + c->K / 4 + kSortTile;  // the next ensure() reallocates
+        c->key_capacity = key_capacity_for(c->K + c->K / 4 + kSortTile);  // the next ensure() reallocates
         GS3D_TRY(enqueue_rebin(c, &c->last_cam, c->last_w, c->last_h, c->last_flags));
     }
     return fail(c, GS3D_E_STATE, "key buffer did not settle after re-runs");
@@ -892,7 +897,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
     if (!same) {
         drop_graph(b);
         for (int k = 0; k < kBatchBranches; k++)
-            if (b->cap[k] == 0) b->cap[k] = std::max<uint64_t>(8ull * c->n, 1u << 16);
+            if (b->cap[k] == 0) b->cap[k] = key_capacity_for(std::max<uint64_t>(8ull * c->n, 1u << 16));
     }
     b->shape = shape;
     b->n = c->n;
@@ -957,7 +962,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
             return GS3D_OK;
         }
         for (int k = 0; k < kBatchBranches; k++)
-            if (need[k] > b->cap[k]) b->cap[k] = need[k] + need[k] / 4 + kSortTile;
+            if (need[k] > b->cap[k]) b->cap[k] = key_capacity_for(need[k] + need[k] / 4 + kSortTile);
         drop_graph(b);
     }
     return fail(c, GS3D_E_STATE, "batch key capacity did not settle");
