@@ -266,7 +266,12 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
 // splat's attributes and SH coefficients are read once, then projected and
 // coloured per view with the same arithmetic as k_preprocess. Each view's
 // counters are zeroed (N stored) by the first threads of the grid's x range.
-template <int DEG>
+//
+// VIEW_COLORS (learned colours, degree 0): each view's colour comes from
+// view_colors instead of the DC term; the block's views' colours are loaded up
+// front, so their latency overlaps instead of stalling every view in turn. A
+// separate instantiation, so the plain degree-0 kernel keeps its registers.
+template <int DEG, bool VIEW_COLORS>
 __global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     const uint32_t v0 = blockIdx.y * kViewsPerBlock;
@@ -285,18 +290,29 @@ __global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a)
 #pragma unroll
             for (int ch = 0; ch < 3; ch++) r.sh[ch][k] = a.sh[(size_t)(k * 3 + ch) * a.n + i];
     }
-    for (uint32_t v = v0; v < v1; v++) {
+    // one view; with `has_col`, `col` is the splat's colour for this view (3 halves)
+    auto do_view = [&](uint32_t v, bool has_col, uint2 col) {
         const ViewParams& vp = a.views[v];
         const ViewGeometry g{vp.cam, vp.tiles_x, vp.tiles_y, vp.inv_tile_x, vp.inv_tile_y};
         float4 o0, o1, o2, o3;
         const uint32_t count = project_splat<DEG>(g, p, r, o0, o1, o2, o3);
-        if (a.view_colors && count) {
-            const uint2 c = a.view_colors[(size_t)v * a.n + i];
-            o2.x = __half2float(__ushort_as_half((unsigned short)(c.x & 0xFFFFu)));
-            o2.y = __half2float(__ushort_as_half((unsigned short)(c.x >> 16)));
-            o2.z = __half2float(__ushort_as_half((unsigned short)(c.y & 0xFFFFu)));
+        if (has_col && count) {
+            o2.x = __half2float(__ushort_as_half((unsigned short)(col.x & 0xFFFFu)));
+            o2.y = __half2float(__ushort_as_half((unsigned short)(col.x >> 16)));
+            o2.z = __half2float(__ushort_as_half((unsigned short)(col.y & 0xFFFFu)));
         }
         store_projected(vp.out, i, count, o0, o1, o2, o3, vp.depth_keys, vp.rects);
+    };
+    if constexpr (VIEW_COLORS) {
+        uint2 vc[kViewsPerBlock];
+#pragma unroll
+        for (int k = 0; k < kViewsPerBlock; k++)
+            vc[k] = (v0 + k < v1) ? a.view_colors[(size_t)(v0 + k) * a.n + i] : make_uint2(0u, 0u);
+#pragma unroll
+        for (int k = 0; k < kViewsPerBlock; k++)
+            if (v0 + k < v1) do_view(v0 + k, true, vc[k]);
+    } else {
+        for (uint32_t v = v0; v < v1; v++) do_view(v, false, make_uint2(0u, 0u));
     }
 }
 
@@ -335,11 +351,15 @@ hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s) 
     const dim3 grid((threads + 255u) / 256u, (a.n_views + kViewsPerBlock - 1) / kViewsPerBlock);
     // learned colours replace the SH evaluation (the DC colour path is the
     // cheapest projection; its colour is then overwritten)
-    switch (a.view_colors ? 0 : a.sh_degree) {
-        case 0: hipLaunchKernelGGL(k_preprocess_views<0>, grid, dim3(256), 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_preprocess_views<1>, grid, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_preprocess_views<2>, grid, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_preprocess_views<3>, grid, dim3(256), 0, s, a); break;
+    if (a.view_colors) {
+        hipLaunchKernelGGL((k_preprocess_views<0, true>), grid, dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
+    switch (a.sh_degree) {
+        case 0: hipLaunchKernelGGL((k_preprocess_views<0, false>), grid, dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_preprocess_views<1, false>), grid, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_preprocess_views<2, false>), grid, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_preprocess_views<3, false>), grid, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
