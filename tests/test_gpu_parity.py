@@ -218,6 +218,19 @@ def test_pipeline_empty_and_all_culled(renderer):
     assert renderer.read_tile_counts().sum() == 0
 
 
+@pytest.mark.parametrize("W,H,tile", [(1, 1, 16), (16, 16, 16), (17, 9, 8), (200, 100, 32), (33, 700, 16)])
+def test_pipeline_tiny_and_ragged_viewports(renderer, W, H, tile):
+    """Viewports of one tile (no tile-digit pass at all), a few tiles (one pass, so the
+    result ends in the second buffer), partial tiles at the right and bottom edges and
+    a tall, narrow one: every buffer bit-exact against the oracle."""
+    say(f"pipeline viewport {W}x{H} tile={tile}")
+    s = sc.activate(sc.uniform_cube(44, 3_000, log_scale_mean=-3.0))
+    img, ref, err = rebin_and_compare(renderer, s, (0.0, 0.0, 2.5), W, H, tile)
+    say(f"  K={ref['k']} passes={renderer.sort_passes()} image max abs {err:.3g}")
+    assert ref["k"] > 0
+    assert err <= TOL_IMAGE
+
+
 def test_full_size_1m_sh3_properties(renderer):
     """1M SH3 splats at 1920x1080: size-independent properties + determinism."""
     say("full-size clusters-1M-sh3-1080p")
