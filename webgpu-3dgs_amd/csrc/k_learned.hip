@@ -32,28 +32,40 @@ namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// f32 -> bf16, round to nearest even (inputs are finite).
-__device__ __forceinline__ short bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
+// f32 -> bf16, round to nearest even: the hardware conversion
+// (v_cvt_pk_bf16_f32, two values per instruction), the same bits as rounding
+// the f32 pattern by hand for finite inputs.
+__device__ __forceinline__ uint32_t bf16x2_bits(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){a, b}, bf16x2_t));
 }
+__device__ __forceinline__ short bf16(float f) { return (short)(bf16x2_bits(f, 0.0f) & 0xFFFFu); }
 __device__ __forceinline__ float bf16_to_f32(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 __device__ __forceinline__ float f16_bits_to_f32(uint16_t h) { return __half2float(__ushort_as_half(h)); }
 __device__ __forceinline__ uint16_t f32_to_f16_bits(float f) { return __half_as_ushort(__float2half_rn(f)); }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
-    bf16x4 r;
-    r[0] = bf16(a), r[1] = bf16(b), r[2] = bf16(c), r[3] = bf16(d);
-    return r;
+    const uint32_t lo = bf16x2_bits(a, b), hi = bf16x2_bits(c, d);
+    return __builtin_bit_cast(bf16x4, make_uint2(lo, hi));
+}
+// ReLU on the rounded bf16 patterns (v_pk_max_i16, two values per
+// instruction): read as signed 16-bit integers, every negative value and -0
+// is below 0 and the non-negative ones keep their order, and rounding
+// commutes with max(x, 0), so the bits equal bf16(max(x, 0)).
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t bits) {
+    const s16x2_t zero = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s16x2_t, bits), zero));
 }
 __device__ __forceinline__ bf16x4 pack_relu(f32x4 v) {
-    return pack4(fmaxf(v[0], 0.f), fmaxf(v[1], 0.f), fmaxf(v[2], 0.f), fmaxf(v[3], 0.f));
+    const uint32_t lo = relu_bf16x2(bf16x2_bits(v[0], v[1])), hi = relu_bf16x2(bf16x2_bits(v[2], v[3]));
+    return __builtin_bit_cast(bf16x4, make_uint2(lo, hi));
 }
 
 // Offsets into the packed parameter vector.
@@ -146,7 +158,9 @@ __device__ __forceinline__ void forward(const Forward& f, bf16x4 x, Acts& a) {
     a.d3 = acc;
 }
 
-__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf(-x)); }
+// v_rcp_f32 (1 ulp) instead of an IEEE division: the results go to f16 colours
+// or into an MSE loss, far coarser than the reciprocal's error
+__device__ __forceinline__ float sigmoid(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
 // Input tile of 16 samples: lane (j = l % 16, g = l / 16) supplies features
 // 4g..4g+3 of sample j: g 0, 1 the latent (as bf16), g 2 the direction and
@@ -189,7 +203,7 @@ __global__ __launch_bounds__(256) void k_learned_colors(LearnedColorArgs a) {
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const float dx = pos[t].x - ex, dy = pos[t].y - ey, dz = pos[t].z - ez;
-            const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+            const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);  // v_rsq_f32
             Acts act;
             forward(f, input_tile(z[t], dx * inv, dy * inv, dz * inv), act);
             const uint32_t i = base + 16u * t + j;
