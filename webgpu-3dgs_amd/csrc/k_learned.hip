@@ -184,28 +184,31 @@ __global__ __launch_bounds__(256) void k_learned_colors(LearnedColorArgs a) {
     if (base >= a.n) return;  // whole wavefronts exit together
     Forward f;
     load_forward(a.params, f);
-    float z[4][8];
-    float4 pos[4];
+    // Per tile, this lane's 4 input features that do not depend on the view
+    // (input_tile's layout: g 0, 1 the latent as bf16, g 3 padding), packed
+    // once; the g 2 lanes supply the direction per view. Only the position is
+    // kept besides, so the view loop holds few registers.
+    bf16x4 xin[4];
+    float px[4], py[4], pz[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const uint32_t i = min(base + 16u * t + j, a.n - 1u);
-        const uint4 q = reinterpret_cast<const uint4*>(a.latents)[i];
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            z[t][2 * k] = f16_bits_to_f32((uint16_t)(w[k] & 0xFFFFu));
-            z[t][2 * k + 1] = f16_bits_to_f32((uint16_t)(w[k] >> 16));
-        }
-        pos[t] = a.pos[i];
+        const uint2 h = reinterpret_cast<const uint2*>(a.latents)[2 * i + (g & 1)];  // halves 4(g&1)..+3
+        float z4[4] = {f16_bits_to_f32((uint16_t)(h.x & 0xFFFFu)), f16_bits_to_f32((uint16_t)(h.x >> 16)),
+                       f16_bits_to_f32((uint16_t)(h.y & 0xFFFFu)), f16_bits_to_f32((uint16_t)(h.y >> 16))};
+        xin[t] = g < 2 ? pack4(z4[0], z4[1], z4[2], z4[3]) : pack4(0.f, 0.f, 0.f, 0.f);
+        const float4 q = a.pos[i];
+        px[t] = q.x, py[t] = q.y, pz[t] = q.z;
     }
     for (uint32_t v = 0; v < a.n_views; v++) {
         const float ex = a.eyes[4 * v], ey = a.eyes[4 * v + 1], ez = a.eyes[4 * v + 2];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const float dx = pos[t].x - ex, dy = pos[t].y - ey, dz = pos[t].z - ez;
+            const float dx = px[t] - ex, dy = py[t] - ey, dz = pz[t] - ez;
             const float inv = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);  // v_rsq_f32
+            const bf16x4 x = g == 2 ? pack4(dx * inv, dy * inv, dz * inv, 1.0f) : xin[t];
             Acts act;
-            forward(f, input_tile(z[t], dx * inv, dy * inv, dz * inv), act);
+            forward(f, x, act);
             const uint32_t i = base + 16u * t + j;
             if (g == 0 && i < a.n) {
                 const uint32_t rg = f32_to_f16_bits(sigmoid(act.d3[0])) | ((uint32_t)f32_to_f16_bits(sigmoid(act.d3[1])) << 16);
