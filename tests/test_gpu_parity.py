@@ -218,11 +218,14 @@ def test_pipeline_empty_and_all_culled(renderer):
     assert renderer.read_tile_counts().sum() == 0
 
 
-@pytest.mark.parametrize("W,H,tile", [(1, 1, 16), (16, 16, 16), (17, 9, 8), (200, 100, 32), (33, 700, 16)])
+@pytest.mark.parametrize("W,H,tile", [(1, 1, 16), (16, 16, 16), (17, 9, 8), (200, 100, 32), (33, 700, 16),
+                                      (4096, 40, 16), (4200, 40, 16)])
 def test_pipeline_tiny_and_ragged_viewports(renderer, W, H, tile):
     """Viewports of one tile (no tile-digit pass at all), a few tiles (one pass, so the
-    result ends in the second buffer), partial tiles at the right and bottom edges and
-    a tall, narrow one: every buffer bit-exact against the oracle."""
+    result ends in the second buffer), partial tiles at the right and bottom edges, a
+    tall, narrow one, and wide ones at and past 256 tiles per row (tile rectangles
+    packed 8 bits per coordinate up to 256, gathered at full width past it): every
+    buffer bit-exact against the oracle."""
     say(f"pipeline viewport {W}x{H} tile={tile}")
     s = sc.activate(sc.uniform_cube(44, 3_000, log_scale_mean=-3.0))
     img, ref, err = rebin_and_compare(renderer, s, (0.0, 0.0, 2.5), W, H, tile)

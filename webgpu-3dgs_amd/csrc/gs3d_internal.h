@@ -69,6 +69,8 @@ struct PreprocessArgs {
     uint32_t* counts;
     uint32_t* depth_keys;  // bits(-depth) of visible splats, 0xFFFFFFFF for culled; or null
     uint2* rects;          // tile rectangle (x0 | y0 << 16, x1 | y1 << 16), zero for culled; or null
+    uint32_t* rects8;      // the same rectangle as x0 | y0 << 8 | x1 << 16 | y1 << 24 when both tile
+                           // counts are <= 256 (zero for culled); or null
     uint32_t* counters;    // if set: counters[0..counter_words) = 0 except [count_slot] = n
     uint32_t counter_words;
     uint32_t count_slot;
@@ -138,6 +140,8 @@ struct ScanRectsArgs {
     const uint32_t* order;       // depth position -> splat id
     const uint32_t* depth_keys;  // depth position -> depth key, 0xFFFFFFFF for a culled splat
     const uint2* rects;          // per splat id: tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
+    const uint32_t* rects8;      // or, when set: per depth position, the rectangle packed 8 bits per
+                                 // coordinate (carried into depth order by the last depth pass)
     uint4* recs;                 // n + 1 records
     uint32_t* ref_counter;
     uint64_t* status;            // lookback words, n / kScanTile + 2
@@ -198,14 +202,16 @@ uint32_t radix_status_tiles(uint32_t n_max);
 // One stable LSD pass over n = *n_dev (key, value) pairs, keys of key_bytes
 // (4 or 8); with iota_values the input values are the positions 0..n-1 and vin
 // is not read. The grid covers n_max; n > n_max is a no-op. With aux_in and
-// aux_out (32-bit keys, values read from vin): aux_out[p] = aux_in[vout[p]].
+// aux_out (32-bit keys): aux_out[p] = aux_in[vout[p]] (values read from vin),
+// or with aux_carry aux_in moves like a second value array: aux_out[p] =
+// aux_in[input index of the key that lands at p].
 hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                              uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
                              const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
                              const uint32_t* epoch_base = nullptr, uint32_t key_tile = 0,
-                             uint32_t match_bits = 0);
+                             uint32_t match_bits = 0, bool aux_carry = false);
 // 64-bit key passes use 16,384-key tiles unless 8,192 is asked for: the
 // smaller tiles (two workgroups per CU) measured slower at 3.6M keys (72 vs
 // 64 us for two passes) and at 118.9M keys (1.54 vs 1.37 ms).

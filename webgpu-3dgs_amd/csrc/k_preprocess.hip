@@ -234,7 +234,8 @@ __device__ __forceinline__ uint32_t project_splat(const ViewGeometry& g, float4 
 
 __device__ __forceinline__ void store_projected(gs3d_projected_splat* out_base, uint32_t i, uint32_t count,
                                                 const float4& o0, const float4& o1, const float4& o2,
-                                                const float4& o3, uint32_t* depth_keys, uint2* rects) {
+                                                const float4& o3, uint32_t* depth_keys, uint2* rects,
+                                                uint32_t* rects8 = nullptr) {
     float4* out = reinterpret_cast<float4*>(out_base + i);
     out[0] = o0;
     out[1] = o1;
@@ -245,6 +246,10 @@ __device__ __forceinline__ void store_projected(gs3d_projected_splat* out_base, 
     if (rects)
         rects[i] = make_uint2(__float_as_uint(o3.x) | (__float_as_uint(o3.y) << 16),
                               __float_as_uint(o3.z) | (__float_as_uint(o3.w) << 16));
+    // the host sets rects8 only when every tile coordinate is < 256
+    if (rects8)
+        rects8[i] = __float_as_uint(o3.x) | (__float_as_uint(o3.y) << 8) | (__float_as_uint(o3.z) << 16) |
+                    (__float_as_uint(o3.w) << 24);
 }
 
 template <int DEG>
@@ -258,7 +263,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     const ViewGeometry g{a.cam, a.tiles_x, a.tiles_y, a.inv_tile_x, a.inv_tile_y};
     float4 o0, o1, o2, o3;
     const uint32_t count = project_splat<DEG>(g, p, GlobalSplat{a, i}, o0, o1, o2, o3);
-    store_projected(a.out, i, count, o0, o1, o2, o3, a.depth_keys, a.rects);
+    store_projected(a.out, i, count, o0, o1, o2, o3, a.depth_keys, a.rects, a.rects8);
     a.counts[i] = count;
 }
 

@@ -136,8 +136,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restric
 }
 
 // The frame's scan: the counts are not read from memory but counted from the
-// tile rectangles of the splats in depth order (gathered through `order`, 8 B
-// per visible splat; culled splats have depth key 0xFFFFFFFF and count 0).
+// tile rectangles of the splats in depth order: gathered through `order` (8 B
+// per visible splat), or, with rects8, read in order (4 B packed rectangles
+// that the last depth pass carried into depth order). Culled splats have
+// depth key 0xFFFFFFFF and count 0.
 // Besides the offsets it writes what emit needs per position, so emit reads
 // records in order instead of gathering splats: recs[p] = (first key, first
 // tile id, width in tiles, depth key); recs[n].x = total = ref_counter[0].
@@ -157,9 +159,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
     if (part >= parts) return;
     const size_t base = (size_t)part * kScanTile + (size_t)tid * kScanItems;
 
+    // per position: the splat id (the gather path) or the packed rectangle (rects8)
     uint32_t sid[kScanItems], dk[kScanItems];
+    const uint32_t* first = a.rects8 ? a.rects8 : a.order;
     if (base + kScanItems <= n) {
-        const uint4* o = reinterpret_cast<const uint4*>(a.order + base);
+        const uint4* o = reinterpret_cast<const uint4*>(first + base);
         const uint4* d = reinterpret_cast<const uint4*>(a.depth_keys + base);
         const uint4 o0 = o[0], o1 = o[1], d0 = d[0], d1 = d[1];
         sid[0] = o0.x; sid[1] = o0.y; sid[2] = o0.z; sid[3] = o0.w;
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
-            sid[k] = base + k < n ? a.order[base + k] : 0u;
+            sid[k] = base + k < n ? first[base + k] : 0u;
             dk[k] = base + k < n ? a.depth_keys[base + k] : 0xFFFFFFFFu;
         }
     }
@@ -180,8 +184,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
         t0[k] = 0u;
         wd[k] = 0u;
         if (dk[k] != 0xFFFFFFFFu) {
-            const uint2 r = a.rects[sid[k]];
-            const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+            uint32_t x0, y0, x1, y1;
+            if (a.rects8) {
+                const uint32_t r = sid[k];
+                x0 = r & 0xFFu, y0 = (r >> 8) & 0xFFu, x1 = (r >> 16) & 0xFFu, y1 = r >> 24;
+            } else {
+                const uint2 r = a.rects[sid[k]];
+                x0 = r.x & 0xFFFFu, y0 = r.x >> 16, x1 = r.y & 0xFFFFu, y1 = r.y >> 16;
+            }
             wd[k] = x1 - x0 + 1u;
             v[k] = wd[k] * (y1 - y0 + 1u);
             t0[k] = y0 * a.tiles_x + x0;

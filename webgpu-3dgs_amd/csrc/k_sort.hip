@@ -68,9 +68,14 @@ constexpr int kLookbackGroup = 8;
 // values reuse it as u32), the per-wavefront digit counts, which the digit byte
 // of every sorted position later overwrites, and one signed per-digit delta:
 // global output index = delta[d] + position in the tile.
-// AUX: also write aux_out[position] = aux_in[value] (a gather through the
-// sorted values, overlapped with the key and value write-out).
-template <int BITS, typename KeyT, bool IOTA, int THREADS, int ITEMS, bool AUX>
+// AUX: a third 32-bit array moves with the keys. kAuxGather: aux_out[position]
+// = aux_in[value] (a gather through the sorted values, overlapped with the key
+// and value write-out). kAuxCarry: aux_in is read at the key's own index, with
+// the values, and lands beside it, like a second value array (coalesced reads;
+// a pass chain carries it through every pass).
+constexpr int kAuxNone = 0, kAuxGather = 1, kAuxCarry = 2;
+
+template <int BITS, typename KeyT, bool IOTA, int THREADS, int ITEMS, int AUX>
 __global__ __launch_bounds__(THREADS) void k_radix_pass(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev,
@@ -178,6 +183,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
             s_stage[rank[it]] = k[it];
             s_digit[rank[it]] = (uint8_t)dig[it];
             val[it] = IOTA ? (uint32_t)idx : vin[idx];
+            if constexpr (AUX == kAuxCarry) aux[it] = aux_in[idx];
         }
     }
 
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     }
     __syncthreads();
 
-    if constexpr (AUX) {
+    if constexpr (AUX == kAuxGather) {
         // issued after the lookback: loads return in order, so an earlier
         // gather would hold up the lookback's status loads
 #pragma unroll
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         const uint32_t j = (uint32_t)(it * THREADS + tid);
         if (j < tile_n) vout[s_delta[s_digit[j]] + (int32_t)j] = s_vals[j];
     }
-    if constexpr (AUX) {
+    if constexpr (AUX != kAuxNone) {
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
@@ -275,7 +281,7 @@ constexpr int kSort32Threads = 1024, kSort32Items = 8;
 constexpr int kSort32SmallThreads = 512, kSort32SmallItems = 8;
 constexpr int kSort32LargeThreads = 1024, kSort32LargeItems = 16;
 
-template <int BITS, typename KeyT, bool IOTA, bool AUX, int THREADS, int ITEMS>
+template <int BITS, typename KeyT, bool IOTA, int AUX, int THREADS, int ITEMS>
 static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                         const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass,
                         uint64_t* status, uint32_t* ticket, uint32_t shift, uint32_t epoch,
@@ -287,7 +293,7 @@ static void launch_pass(const void* kin, const uint32_t* vin, void* kout, uint32
                        hist_pass, status, ticket, n_max, shift, epoch, aux_in, aux_out, epoch_base, match_bits);
 }
 
-template <int BITS, typename KeyT, bool IOTA, bool AUX>
+template <int BITS, typename KeyT, bool IOTA, int AUX>
 static void launch_shape(int shape, const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                          const uint32_t* n_dev, uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                          uint32_t* ticket, uint32_t shift, uint32_t epoch, const uint32_t* aux_in,
@@ -309,7 +315,7 @@ static void launch_shape(int shape, const void* kin, const uint32_t* vin, void* 
                 epoch_base, match_bits);
         } else if (shape > 0) {
             // the 16,384-key shape has no gather variant (rejected by launch_radix_pass)
-            if constexpr (!AUX)
+            if constexpr (AUX == kAuxNone)
                 launch_pass<BITS, KeyT, IOTA, AUX, kSort32LargeThreads, kSort32LargeItems>(
                     kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out,
                     s, epoch_base, match_bits);
@@ -326,7 +332,8 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,
                              uint32_t* ticket, uint32_t shift, uint32_t digit_bits, uint32_t epoch,
                              const uint32_t* aux_in, uint32_t* aux_out, hipStream_t s,
-                             const uint32_t* epoch_base, uint32_t key_tile, uint32_t match_bits) {
+                             const uint32_t* epoch_base, uint32_t key_tile, uint32_t match_bits,
+                             bool aux_carry) {
     if (n_max == 0) return hipSuccess;
     if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
     // key_tile: keys per workgroup, 0 = the default shape for the key width
@@ -341,24 +348,29 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
         shape = key_tile == 4096 ? -1 : (key_tile == 16384 ? 1 : 0);
     }
     const bool aux = aux_in != nullptr && aux_out != nullptr;
-    // the gather rides on 32-bit key passes whose values are read from vin
-    if (aux && (key_bytes != 4 || iota_values || shape > 0)) return hipErrorInvalidValue;
+    // aux rides on 32-bit key passes of the 4,096- and 8,192-key shapes; the
+    // gather also needs the values read from vin
+    if (aux && (key_bytes != 4 || shape > 0 || (!aux_carry && iota_values))) return hipErrorInvalidValue;
     const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
-                    (iota_values ? 1 : 0) + (aux ? 2 : 0);
+                    (iota_values ? 1 : 0) + (aux ? (aux_carry ? 4 : 2) : 0);
 #define GS3D_PASS(B, K, I, A)                                                                      \
     launch_shape<B, K, I, A>(shape, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, \
                              epoch, aux_in, aux_out, s, epoch_base, match_bits)
     switch (key) {
-        case 110: GS3D_PASS(8, uint64_t, false, false); break;
-        case 111: GS3D_PASS(8, uint64_t, true, false); break;
-        case 120: GS3D_PASS(8, uint32_t, false, false); break;
-        case 121: GS3D_PASS(8, uint32_t, true, false); break;
-        case 122: GS3D_PASS(8, uint32_t, false, true); break;
-        case 210: GS3D_PASS(4, uint64_t, false, false); break;
-        case 211: GS3D_PASS(4, uint64_t, true, false); break;
-        case 220: GS3D_PASS(4, uint32_t, false, false); break;
-        case 221: GS3D_PASS(4, uint32_t, true, false); break;
-        case 222: GS3D_PASS(4, uint32_t, false, true); break;
+        case 110: GS3D_PASS(8, uint64_t, false, kAuxNone); break;
+        case 111: GS3D_PASS(8, uint64_t, true, kAuxNone); break;
+        case 120: GS3D_PASS(8, uint32_t, false, kAuxNone); break;
+        case 121: GS3D_PASS(8, uint32_t, true, kAuxNone); break;
+        case 122: GS3D_PASS(8, uint32_t, false, kAuxGather); break;
+        case 124: GS3D_PASS(8, uint32_t, false, kAuxCarry); break;
+        case 125: GS3D_PASS(8, uint32_t, true, kAuxCarry); break;
+        case 210: GS3D_PASS(4, uint64_t, false, kAuxNone); break;
+        case 211: GS3D_PASS(4, uint64_t, true, kAuxNone); break;
+        case 220: GS3D_PASS(4, uint32_t, false, kAuxNone); break;
+        case 221: GS3D_PASS(4, uint32_t, true, kAuxNone); break;
+        case 222: GS3D_PASS(4, uint32_t, false, kAuxGather); break;
+        case 224: GS3D_PASS(4, uint32_t, false, kAuxCarry); break;
+        case 225: GS3D_PASS(4, uint32_t, true, kAuxCarry); break;
         default: return hipErrorInvalidValue;
     }
 #undef GS3D_PASS

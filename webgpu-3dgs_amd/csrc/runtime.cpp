@@ -293,19 +293,27 @@ int run_passes(gs3d_ctx* c, void* ka, uint32_t* va, void* kb, uint32_t* vb, uint
                bool first_iota, const uint32_t* n_dev, uint64_t n_max, const uint32_t* shifts,
                uint32_t passes, const uint32_t* hist, uint32_t* tickets, bool* in_b,
                const uint32_t* aux_in = nullptr, uint32_t* aux_out = nullptr, uint32_t key_bits = 64,
-               uint32_t key_tile = 0) {
+               uint32_t key_tile = 0, uint32_t* carry_a = nullptr, uint32_t* carry_b = nullptr) {
+    // carry_a / carry_b (or null): a 32-bit array that every pass moves with the
+    // keys, starting in carry_a and ending beside the keys (in carry_b exactly
+    // when *in_b is set)
     uint64_t* radix_status = nullptr;
     const uint32_t radix = 1u << c->digit_bits;
     GS3D_TRY(ensure_status(c, "radix_status",
                            (size_t)radix_status_tiles((uint32_t)n_max) * radix, &radix_status));
     for (uint32_t p = 0; p < passes; p++) {
         const bool even = (p % 2) == 0;
+        const uint32_t* ain = p + 1 == passes ? aux_in : nullptr;
+        uint32_t* aout = p + 1 == passes ? aux_out : nullptr;
+        if (carry_a) {
+            ain = even ? carry_a : carry_b;
+            aout = even ? carry_b : carry_a;
+        }
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va,
                                      key_bytes, first_iota && p == 0, n_dev, (uint32_t)n_max,
                                      hist + p * radix, radix_status, tickets + p, shifts[p],
-                                     c->digit_bits, next_epoch(c), p + 1 == passes ? aux_in : nullptr,
-                                     p + 1 == passes ? aux_out : nullptr, c->stream, nullptr, key_tile,
-                                     pass_match_bits(shifts[p], c->digit_bits, key_bits)));
+                                     c->digit_bits, next_epoch(c), ain, aout, c->stream, nullptr, key_tile,
+                                     pass_match_bits(shifts[p], c->digit_bits, key_bits), carry_a != nullptr));
     }
     *in_b = (passes % 2) == 1;
     return GS3D_OK;
@@ -365,8 +373,18 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     uint16_t* image;
     GS3D_TRY(ensure_t(c, "projected_splats", n, &projected));
     GS3D_TRY(ensure_t(c, "splat_ref_counts", n, &counts));
-    uint2* rects;
-    GS3D_TRY(ensure_t(c, "splat_tile_rects", n, &rects));
+    // Tile rectangles: packed 8 bits per coordinate when both tile counts fit,
+    // so the last depth pass carries them into depth order and the scan reads
+    // them in order; otherwise 8 B per splat, gathered by the scan.
+    const bool rects_packed = c->tiles_x <= 256u && c->tiles_y <= 256u;
+    uint2* rects = nullptr;
+    uint32_t *rects8 = nullptr, *rects8_sorted = nullptr;
+    if (rects_packed) {
+        GS3D_TRY(ensure_t(c, "splat_tile_rects8_a", n, &rects8));
+        GS3D_TRY(ensure_t(c, "splat_tile_rects8_b", n, &rects8_sorted));
+    } else {
+        GS3D_TRY(ensure_t(c, "splat_tile_rects", n, &rects));
+    }
     GS3D_TRY(ensure_t(c, "counters", kCtrTotalWords, &counters));
     GS3D_TRY(ensure_status(c, "scan_status", (size_t)n / kScanTile + 2, &scan_status));
     GS3D_TRY(ensure_t(c, "splat_emit_records", (size_t)n + 1, &recs));
@@ -406,6 +424,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     pa.counts = counts;
     pa.depth_keys = dka;
     pa.rects = rects;
+    pa.rects8 = rects8;
     pa.n = n;
     pa.sh_degree = c->sh_degree;
     pa.tiles_x = c->tiles_x;
@@ -421,9 +440,10 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
                                     bits, depth_passes, 0, c->stream));
     bool order_in_b = false;
     // pass 0 takes the positions as values (ida is not read); odd passes write ida
+    // the packed rectangles move with the depth keys (rects8 <-> rects8_sorted)
     GS3D_TRY(run_passes(c, dka, ida, dkb, idb, 4, true, counters + kCtrSplatCount, n, c->shifts,
                         depth_passes, hist_depth, counters + kCtrSortTicket, &order_in_b, nullptr, nullptr, 32,
-                        c->sort_key_tile32));
+                        c->sort_key_tile32, rects8, rects8_sorted));
     const uint32_t* order = order_in_b ? idb : ida;
     const uint32_t* depth_sorted = order_in_b ? dkb : dka;
     record(c, GS3D_STAGE_DEPTH_SORT, 1);
@@ -435,6 +455,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     sa.order = order;
     sa.depth_keys = depth_sorted;
     sa.rects = rects;
+    sa.rects8 = rects_packed ? (order_in_b ? rects8_sorted : rects8) : nullptr;
     sa.recs = recs;
     sa.ref_counter = ref_counter;
     sa.status = scan_status;
