@@ -37,24 +37,29 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg):
+def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg, packed=True):
     """Bytes each stage must move at minimum, per launch group of one frame.
 
+    With `packed` (both tile counts <= 256) the tile rectangle is 4 B and moves
+    with the depth keys; otherwise it is 8 B and the scan gathers it.
     preprocess: vec4 inputs + SH coefficients in, 64-B projected splat + count
-    + depth key + 8-B tile rectangle out; depth sort: per pass a u32 key + u32 id read and written
-    (the first pass reads keys only), plus one read of the keys to histogram;
-    scan: the sorted id and depth key and the 8-B tile rectangle of each
-    splat in, a 16-B record per splat out; emit: the records and ids in,
+    + depth key + tile rectangle out; depth sort: per pass a u32 key + u32 id
+    (+ the 4-B rectangle when packed) read and written (the first pass reads
+    keys, and rectangles, only), plus one read of the keys to histogram;
+    scan: the depth key and the rectangle (packed: read in order; else the id
+    and the 8-B rectangle) of each splat in, a 16-B record per splat out; emit: the records and ids in,
     8-B (tile id, splat id) out; key sort: 16 B per key per pass (32-bit tile id
     + 32-bit splat id, read and written); identify: one (T+1)-entry write plus, per tile, the ceil(log2 K)
     sorted keys its binary search reads; raster: the 48 B used of each listed
     splat, tile ranges, 8-B pixel out.
     """
     pre_in = 64 + (12 * (deg + 1) ** 2 if deg else 0)
+    rect = 4 if packed else 8
+    carry = 4 if packed else 0
     return {
-        "preprocess-splats": n * (pre_in + 64 + 4 + 4 + 8),
-        "radix-sort-splats": n * (4 + 12 + 16 * (depth_passes - 1)) if depth_passes else 0,
-        "prefix-scan": 16 * n + 16 * (n + 1),
+        "preprocess-splats": n * (pre_in + 64 + 4 + 4 + rect),
+        "radix-sort-splats": n * (4 + 12 + 2 * carry + (16 + 2 * carry) * (depth_passes - 1)) if depth_passes else 0,
+        "prefix-scan": (8 if packed else 16) * n + 16 * (n + 1),
         "emit-tile-refs": 16 * (n + 1) + 4 * n + 8 * k,
         "radix-sort": tile_passes * 16 * k,
         "identify-tile-ranges": 8 * tiles * max(1, int(k).bit_length()) + 4 * (tiles + 1),
@@ -309,7 +314,8 @@ def main() -> int:
     shifts = r.sort_passes()
     depth_passes = sum(1 for b in shifts if b < 32)
     passes = len(shifts) - depth_passes
-    algo = algorithmic_bytes(splats.n, k, T, W * H, depth_passes, passes, splats.sh_degree)
+    algo = algorithmic_bytes(splats.n, k, T, W * H, depth_passes, passes, splats.sh_degree,
+                             packed=tiles_x <= 256 and tiles_y <= 256)
 
     cpu = None
     visited = None
