@@ -87,7 +87,8 @@ struct ViewParams {
     float inv_tile_x, inv_tile_y;
     gs3d_projected_splat* out;
     uint32_t* depth_keys;
-    uint2* rects;
+    uint2* rects;        // 8-B tile rectangles, or null
+    uint32_t* rects8;    // or the packed rectangles (both tile counts <= 256), or null
     uint32_t* counters;  // counters[0..counter_words) = 0 except [count_slot] = n
 };
 

@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a)
             o2.y = __half2float(__ushort_as_half((unsigned short)(col.x >> 16)));
             o2.z = __half2float(__ushort_as_half((unsigned short)(col.y & 0xFFFFu)));
         }
-        store_projected(vp.out, i, count, o0, o1, o2, o3, vp.depth_keys, vp.rects);
+        store_projected(vp.out, i, count, o0, o1, o2, o3, vp.depth_keys, vp.rects, vp.rects8);
     };
     if constexpr (VIEW_COLORS) {
         uint2 vc[kViewsPerBlock];

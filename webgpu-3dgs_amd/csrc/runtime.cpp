@@ -720,14 +720,16 @@ int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, u
                  uint32_t key_bytes, bool first_iota, const uint32_t* n_dev, uint64_t n_max,
                  const uint32_t* shifts, uint32_t passes, const uint32_t* hist, uint32_t* tickets,
                  uint64_t* status, const uint32_t* epoch_base, uint32_t* epoch_ctr, bool* in_b,
-                 uint32_t key_bits, uint32_t key_tile) {
+                 uint32_t key_bits, uint32_t key_tile, uint32_t* carry_a = nullptr, uint32_t* carry_b = nullptr) {
     const uint32_t radix = 1u << c->digit_bits;
     for (uint32_t p = 0; p < passes; p++) {
         const bool even = (p % 2) == 0;
         HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va, key_bytes,
                                      first_iota && p == 0, n_dev, (uint32_t)n_max, hist + p * radix, status,
-                                     tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr, nullptr, nullptr, s,
-                                     epoch_base, key_tile, pass_match_bits(shifts[p], c->digit_bits, key_bits)));
+                                     tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr,
+                                     carry_a ? (even ? carry_a : carry_b) : nullptr,
+                                     carry_a ? (even ? carry_b : carry_a) : nullptr, s, epoch_base, key_tile,
+                                     pass_match_bits(shifts[p], c->digit_bits, key_bits), carry_a != nullptr));
     }
     *in_b = (passes % 2) == 1;
     return GS3D_OK;
@@ -807,13 +809,18 @@ int batch_enqueue(gs3d_ctx* c) {
         HIP_TRY(c, launch_key_histogram(dka, 4, ctr + kCtrSplatCount, n, hist_depth, hist_partial, bits,
                                         depth_passes, 0, s));
         bool order_in_b = false;
+        // packed rectangles (as set up for k_preprocess_views) move with the depth keys
+        const bool packed = v.tiles_x <= 256u && v.tiles_y <= 256u;
+        uint32_t* r8a = packed ? reinterpret_cast<uint32_t*>(rects + (size_t)vi * n) : nullptr;
+        uint32_t* r8b = packed ? r8a + n : nullptr;
         GS3D_TRY(batch_passes(c, s, dka, ida, dkb, idb, 4, true, ctr + kCtrSplatCount, n, shifts, depth_passes,
                               hist_depth, ctr + kCtrSortTicket, radix_status, epoch_base, &epoch_ctr[k],
-                              &order_in_b, 32, c->sort_key_tile32));
+                              &order_in_b, 32, c->sort_key_tile32, r8a, r8b));
         ScanRectsArgs sa{};
         sa.order = order_in_b ? idb : ida;
         sa.depth_keys = order_in_b ? dkb : dka;
-        sa.rects = rects + (size_t)vi * n;
+        sa.rects = packed ? nullptr : rects + (size_t)vi * n;
+        sa.rects8 = packed ? (order_in_b ? r8b : r8a) : nullptr;
         sa.recs = recs;
         sa.ref_counter = ctr + kCtrRefCounter;
         sa.status = scan_status;
@@ -948,7 +955,11 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
             vp[i].inv_tile_y = (float)(1.0 / (double)c->tile_y);
             vp[i].out = projected + (size_t)i * c->n;
             vp[i].depth_keys = dk + (size_t)i * c->n;
-            vp[i].rects = rects + (size_t)i * c->n;
+            // a view's 8 B per splat of rectangle memory holds, when packed, the two
+            // 4-B arrays its depth passes carry the rectangles between
+            const bool packed = b->views[i].tiles_x <= 256u && b->views[i].tiles_y <= 256u;
+            vp[i].rects = packed ? nullptr : rects + (size_t)i * c->n;
+            vp[i].rects8 = packed ? reinterpret_cast<uint32_t*>(rects + (size_t)i * c->n) : nullptr;
             vp[i].counters = counters + (size_t)i * kCtrTotalWords;
         }
         HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_views"), vp.data(), vp.size() * sizeof(ViewParams),
