@@ -314,7 +314,7 @@ static void launch_shape(int shape, const void* kin, const uint32_t* vin, void* 
                 kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out, s,
                 epoch_base, match_bits);
         } else if (shape > 0) {
-            // the 16,384-key shape has no gather variant (rejected by launch_radix_pass)
+            // the 16,384-key shape has no aux variant (rejected by launch_radix_pass)
             if constexpr (AUX == kAuxNone)
                 launch_pass<BITS, KeyT, IOTA, AUX, kSort32LargeThreads, kSort32LargeItems>(
                     kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, epoch, aux_in, aux_out,
@@ -338,7 +338,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
     if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
     // key_tile: keys per workgroup, 0 = the default shape for the key width
     // (16,384 for 64-bit keys, 8,192 for 32-bit keys); 64-bit: 8,192 or
-    // 16,384; 32-bit: 4,096, 8,192 or 16,384 (16,384 not with the gather)
+    // 16,384; 32-bit: 4,096, 8,192 or 16,384 (16,384 not with aux)
     int shape;  // -1 the half-size tile, 0 the default, 1 the double-size tile
     if (key_bytes == 8) {
         if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
