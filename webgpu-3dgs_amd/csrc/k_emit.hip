@@ -7,10 +7,11 @@
 // owns 256 consecutive splats and therefore one contiguous output range
 // [offsets[i0], offsets[i0+256]); thread j of the range finds its splat from
 // the splats' first slots marked in LDS (see emit_chunk), so every store of
-// keys and ids is coalesced and no thread loops over a splat's tiles. In the frame the splats come in
-// depth order with per-position records from k_scan_rects (first key, first
-// tile, width, depth key), read coalesced; with offsets instead, the tile
-// rectangle and depth are read from the projected splats.
+// keys and ids is coalesced and no thread loops over a splat's tiles. In the
+// frame the splats come in depth order with per-position records from
+// k_scan_rects (first key, first tile, width, depth key), read coalesced; with
+// offsets instead, the tile rectangle and depth are read from the projected
+// splats.
 //
 // With `tiles` set, only the tile id is stored (4 B instead of 8): the stream
 // is emitted in stable depth order, so sorting the tile ids alone finishes
@@ -243,7 +244,12 @@ void launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t words, 
 
 // Digit histograms of n = *n_dev keys for every pass, one plain row per
 // workgroup (grid-stride, kHistKeys keys per thread per step, LDS atomics).
+// The LDS histogram is kept in kHistCopies copies, copy = lane % kHistCopies,
+// interleaved word by word: lanes of a wavefront that share a digit (most of
+// them, for the high digits of depth keys) then add to kHistCopies different
+// banks instead of one word. The copies are summed into the row.
 constexpr uint32_t kHistKeys = 4;
+constexpr uint32_t kHistCopies = 4;
 
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ keys,
@@ -251,11 +257,12 @@ __global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ 
                                                        uint32_t n_max, uint32_t* __restrict__ partial,
                                                        uint32_t digit_bits, uint32_t n_passes,
                                                        uint32_t shift0) {
-    __shared__ uint32_t s_hist[kMaxPasses * 256];
+    extern __shared__ uint32_t s_hist[];  // [pass * radix + digit][copy]
     const uint32_t radix = 1u << digit_bits;
     const uint32_t words = n_passes * radix;
-    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) s_hist[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < words * kHistCopies; k += blockDim.x) s_hist[k] = 0u;
     __syncthreads();
+    const uint32_t copy = threadIdx.x % kHistCopies;
     uint32_t n = *n_dev;
     if (n > n_max) n = 0;  // keys did not fit: count nothing, the frame is re-run
     const size_t stride = (size_t)gridDim.x * blockDim.x * kHistKeys;
@@ -269,14 +276,20 @@ __global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ 
 #pragma unroll
         for (uint32_t j = 0; j < kHistKeys; j++) {
             if (base + j * blockDim.x + threadIdx.x >= n) continue;
-            for (uint32_t p = 0; p < n_passes; p++)
-                atomicAdd(&s_hist[p * radix + ((uint32_t)(k[j] >> (shift0 + p * digit_bits)) & (radix - 1u))],
-                          1u);
+            for (uint32_t p = 0; p < n_passes; p++) {
+                const uint32_t w = p * radix + ((uint32_t)(k[j] >> (shift0 + p * digit_bits)) & (radix - 1u));
+                atomicAdd(&s_hist[w * kHistCopies + copy], 1u);
+            }
         }
     }
     __syncthreads();
     uint32_t* row = partial + (size_t)blockIdx.x * words;
-    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) row[k] = s_hist[k];
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < kHistCopies; c++) sum += s_hist[k * kHistCopies + c];
+        row[k] = sum;
+    }
 }
 
 }  // namespace
@@ -308,12 +321,13 @@ hipError_t launch_key_histogram(const void* keys, uint32_t key_bytes, const uint
     if (n_max == 0 || n_passes == 0) return hipSuccess;
     if (n_passes > kMaxPasses || (digit_bits != 4 && digit_bits != 8)) return hipErrorInvalidValue;
     const uint32_t grid = key_hist_rows(n_max);
+    const size_t lds = (size_t)n_passes * (1u << digit_bits) * kHistCopies * 4u;  // <= 64 KB
     if (key_bytes == 8) {
-        hipLaunchKernelGGL(k_key_histogram<uint64_t>, dim3(grid), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_key_histogram<uint64_t>, dim3(grid), dim3(256), lds, s,
                            static_cast<const uint64_t*>(keys), n_dev, n_max, hist_partial, digit_bits,
                            n_passes, shift0);
     } else if (key_bytes == 4) {
-        hipLaunchKernelGGL(k_key_histogram<uint32_t>, dim3(grid), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_key_histogram<uint32_t>, dim3(grid), dim3(256), lds, s,
                            static_cast<const uint32_t*>(keys), n_dev, n_max, hist_partial, digit_bits,
                            n_passes, shift0);
     } else {
