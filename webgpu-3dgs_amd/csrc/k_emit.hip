@@ -56,10 +56,17 @@ struct EmitShared {
     uint32_t mark[2][kMarkWords];  // slot masks, alternating; the one a window uses starts clear
     int32_t prev[kMarkWords];      // last marked slot before each mask word (-1: none)
     uint8_t owner[kOwnerWindow];   // at marked slots: the chunk-local splat
-    uint32_t hist[kMaxPasses * 256];
 };
 
-__device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, EmitShared& s, uint32_t& cur) {
+// The digit histograms (emit and k_key_histogram) are kept in LDS in
+// kHistCopies copies, copy = thread % kHistCopies, interleaved word by word:
+// lanes of a wavefront that share a digit (most of them, for high digits) add
+// to kHistCopies different banks instead of one word. The copies are summed
+// when the row is written.
+constexpr uint32_t kHistCopies = 4;
+
+__device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, EmitShared& s, uint32_t* s_hist,
+                                           uint32_t& cur) {
     const int tid = threadIdx.x;
     const uint32_t i = i0 + tid;
     const uint32_t last = min(i0 + kEmitThreads, a.n);
@@ -176,7 +183,7 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, EmitS
                     if (a.hist_partial) {
                         for (uint32_t p = 0; p < a.n_passes; p++) {
                             const uint32_t d = (uint32_t)(key >> (a.hist_shift0 + p * a.digit_bits)) & (radix - 1u);
-                            atomicAdd(&s.hist[p * radix + d], 1u);
+                            atomicAdd(&s_hist[(p * radix + d) * kHistCopies + tid % kHistCopies], 1u);
                         }
                     }
                 }
@@ -192,17 +199,23 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, EmitS
 // (hist_partial[block][pass][digit]); k_hist_reduce folds the rows.
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
     __shared__ EmitShared s;
+    extern __shared__ uint32_t s_hist[];  // [pass * radix + digit][copy], launch-sized
     const uint32_t radix = 1u << a.digit_bits;
     const uint32_t hist_words = a.hist_partial ? a.n_passes * radix : 0u;
-    for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) s.hist[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < hist_words * kHistCopies; k += kEmitThreads) s_hist[k] = 0u;
     for (uint32_t k = threadIdx.x; k < 2 * kMarkWords; k += kEmitThreads) (&s.mark[0][0])[k] = 0u;
     uint32_t cur = 0;
     const uint32_t chunks = (a.n + kEmitThreads - 1) / kEmitThreads;
-    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) emit_chunk(a, c * kEmitThreads, s, cur);
+    for (uint32_t c = blockIdx.x; c < chunks; c += gridDim.x) emit_chunk(a, c * kEmitThreads, s, s_hist, cur);
     if (a.hist_partial) {
         __syncthreads();
         uint32_t* row = a.hist_partial + (size_t)blockIdx.x * hist_words;
-        for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) row[k] = s.hist[k];
+        for (uint32_t k = threadIdx.x; k < hist_words; k += kEmitThreads) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < kHistCopies; c++) sum += s_hist[k * kHistCopies + c];
+            row[k] = sum;
+        }
     }
 }
 
@@ -243,13 +256,9 @@ void launch_hist_reduce(const uint32_t* partial, uint32_t rows, uint32_t words, 
 }
 
 // Digit histograms of n = *n_dev keys for every pass, one plain row per
-// workgroup (grid-stride, kHistKeys keys per thread per step, LDS atomics).
-// The LDS histogram is kept in kHistCopies copies, copy = lane % kHistCopies,
-// interleaved word by word: lanes of a wavefront that share a digit (most of
-// them, for the high digits of depth keys) then add to kHistCopies different
-// banks instead of one word. The copies are summed into the row.
+// workgroup (grid-stride, kHistKeys keys per thread per step, LDS atomics into
+// kHistCopies copies).
 constexpr uint32_t kHistKeys = 4;
-constexpr uint32_t kHistCopies = 4;
 
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_key_histogram(const KeyT* __restrict__ keys,
@@ -303,7 +312,8 @@ hipError_t launch_emit(const EmitArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     if (a.n_passes > kMaxPasses || (a.digit_bits != 4 && a.digit_bits != 8)) return hipErrorInvalidValue;
     const uint32_t grid = emit_grid(a.n);
-    hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kEmitThreads), 0, s, a);
+    const size_t lds = a.hist_partial ? (size_t)a.n_passes * (1u << a.digit_bits) * kHistCopies * 4u : 0u;
+    hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kEmitThreads), lds, s, a);
     if (a.hist_partial && a.hist)
         launch_hist_reduce(a.hist_partial, grid, a.n_passes * (1u << a.digit_bits), a.hist, s);
     return hipGetLastError();
