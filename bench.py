@@ -239,7 +239,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path runs on a single MI355X "
-              "(see DESIGN.md for how it would shard); use --gpus 1", file=sys.stderr)
+              "(a multi-GPU run would shard batches by view); use --gpus 1", file=sys.stderr)
         return 2
 
     if args.copy_calibration:

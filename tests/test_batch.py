@@ -107,6 +107,18 @@ def test_batch_single_view_and_tile_sizes(renderer, scene_sh3):
     check_batch(renderer, views_for(6, 1))
 
 
+def test_batch_four_bit_digits(renderer, scene_sh3):
+    """4-bit digits: eight depth passes carry the packed tile rectangles through the
+    batch's ping-pong buffers; the batch still equals single views."""
+    say("batch: 4-bit digits")
+    renderer.set_splat_data(scene_sh3)
+    renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 4)
+    try:
+        check_batch(renderer, views_for(7, 3))
+    finally:
+        renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)
+
+
 def test_batch_capacity_regrow(renderer):
     """Large splats make K exceed the initial 8N key budget of a branch: the
     batch grows its buffers, captures again and still equals single views."""

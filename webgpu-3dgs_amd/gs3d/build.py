@@ -26,7 +26,8 @@ ORACLE_LIB = REPO / "oracle" / "liboracle_gs3d.so"
 INCLUDE = REPO / "include"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-# -packed-fp32-ops: no v_pk_*_f32 generation (see the build notes in DESIGN.md).
+# -packed-fp32-ops: no v_pk_*_f32 generation; some packed fp32 forms have given
+# wrong results on MI355X without an error, so all HIP code is built without them.
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
