@@ -119,6 +119,19 @@ def test_batch_four_bit_digits(renderer, scene_sh3):
         renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)
 
 
+def test_batch_records_follow_the_batch_splats(renderer, scene_sh3):
+    """A batch view's projected records are made again on read from the splats the
+    batch projected: after a new upload they are refused, the image still reads."""
+    say("batch: records after a new upload")
+    renderer.set_splat_data(scene_sh3)
+    views = views_for(8, 2)
+    got = check_batch(renderer, views)
+    renderer.set_splat_data(scene_sh3)
+    with pytest.raises(L.Gs3dError):
+        renderer.batch_read(0, L.BUF_PROJECTED_SPLATS, np.uint32)
+    assert np.array_equal(renderer.batch_read(1, L.BUF_IMAGE_RGBA16F, np.uint16), got[1]["image"])
+
+
 def test_batch_capacity_regrow(renderer):
     """Large splats make K exceed the initial 8N key budget of a branch: the
     batch grows its buffers, captures again and still equals single views."""

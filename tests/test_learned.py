@@ -195,6 +195,7 @@ def test_fallback_to_exact_sh_below_threshold(renderer, small_scene):
     views = [(cam_mod.view_camera(eye, w, h), w, h) for eye, w, h in S.orbit_views(4, 2, fixed=(200, 120))]
     renderer.rebin_batch(views)
     exact = [renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16) for i in range(2)]
+    exact_rec = renderer.batch_read(1, L.BUF_PROJECTED_SPLATS, np.uint32).reshape(-1, 16)
     psnr, enabled = renderer.learned_validate(views, 200.0)
     assert not enabled and psnr < 200.0
     renderer.rebin_batch(views, learned=True)
@@ -206,6 +207,13 @@ def test_fallback_to_exact_sh_below_threshold(renderer, small_scene):
     differs = any(not np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), exact[i])
                   for i in range(2))
     assert differs
+    # the learned batch's records: the same geometry, the learned colours
+    rec = renderer.batch_read(1, L.BUF_PROJECTED_SPLATS, np.uint32).reshape(-1, 16)
+    geometry = [0, 1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14, 15]
+    assert np.array_equal(rec[:, geometry], exact_rec[:, geometry])
+    visible = exact_rec[:, 11] == 1
+    assert visible.any() and not np.array_equal(rec[visible][:, 8:11], exact_rec[visible][:, 8:11])
+    assert not rec[~visible].any()
 
 
 @pytest.mark.gpu

@@ -85,15 +85,15 @@ struct ViewParams {
     gs3d_camera cam;
     uint32_t tiles_x, tiles_y;
     float inv_tile_x, inv_tile_y;
-    gs3d_projected_splat* out;
+    // 48-B records (the first three float4s of a gs3d_projected_splat) of the
+    // visible splats; culled splats' records are not written. The full-record
+    // kernel writes gs3d_projected_splat records of every splat here instead.
+    float4* out;
     uint32_t* depth_keys;
     uint2* rects;        // 8-B tile rectangles, or null
     uint32_t* rects8;    // or the packed rectangles (both tile counts <= 256), or null
     uint32_t* counters;  // counters[0..counter_words) = 0 except [count_slot] = n
 };
-
-// Views one workgroup of the multi-view preprocess projects each splat into.
-constexpr uint32_t kViewsPerBlock = 8;
 
 struct MultiPreprocessArgs {
     const ViewParams* views;  // device array of n_views
@@ -110,6 +110,10 @@ struct MultiPreprocessArgs {
     uint32_t count_slot;
 };
 hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s);
+// The same arithmetic writing, for every splat, the 64-B gs3d_projected_splat
+// a single-view frame writes (zeros when culled): how a batch view's records
+// are read back.
+hipError_t launch_preprocess_views_full(const MultiPreprocessArgs& a, hipStream_t s);
 
 struct EmitArgs {
     const gs3d_projected_splat* projected;
@@ -241,6 +245,12 @@ hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_
                             const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                             uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
                             uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
+// The same over records of `rec` float4s (4: gs3d_projected_splat; 3: the
+// batch's 48-B records, the first three float4s of one).
+hipError_t launch_rasterize_records(const float4* records, uint32_t rec, const uint32_t* values,
+                                    const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
+                                    uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
+                                    uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
 
 // ---- learned colour ---------------------------------------------------------------

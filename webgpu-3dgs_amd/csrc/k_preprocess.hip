@@ -267,7 +267,19 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
     a.counts[i] = count;
 }
 
-// One splat into the views [view0, view0 + kViewsPerBlock) of a batch: the
+// Views one workgroup of the multi-view preprocess projects each splat into:
+// the splat's attributes and SH coefficients are read once per kViewsPerBlock
+// views (measured on 64 views of 2M SH3 splats: 2,803 / 2,925 / 2,964 / 2,988
+// views/s at 4 / 8 / 16 / 32). The learned-colour kernel prefetches one colour
+// per view and keeps 8 (1,722 views/s at 8, 1,700 at 32, whose prefetch takes
+// 167 VGPRs).
+constexpr uint32_t kViewsPerBlock = 32;
+constexpr uint32_t kViewsPerBlockColors = 8;
+
+template <bool VIEW_COLORS>
+constexpr uint32_t views_per_block() { return VIEW_COLORS ? kViewsPerBlockColors : kViewsPerBlock; }
+
+// One splat into the views [view0, view0 + views_per_block) of a batch: the
 // splat's attributes and SH coefficients are read once, then projected and
 // coloured per view with the same arithmetic as k_preprocess. Each view's
 // counters are zeroed (N stored) by the first threads of the grid's x range.
@@ -276,11 +288,18 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
 // view_colors instead of the DC term; the block's views' colours are loaded up
 // front, so their latency overlaps instead of stalling every view in turn. A
 // separate instantiation, so the plain degree-0 kernel keeps its registers.
-template <int DEG, bool VIEW_COLORS>
+//
+// A batch view's records are read only by its rasterizer, which reads the
+// first 48 B, and only for splats in its tile lists: each visible splat gets a
+// 48-B record (centre/depth/radius, conic/opacity, colour) and its rectangle;
+// a culled splat only its depth key 0xFFFFFFFF. FULL writes the 64-B
+// gs3d_projected_splat of every splat instead, as k_preprocess does.
+template <int DEG, bool VIEW_COLORS, bool FULL>
 __global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t v0 = blockIdx.y * kViewsPerBlock;
-    const uint32_t v1 = min(v0 + kViewsPerBlock, a.n_views);
+    constexpr uint32_t VPB = views_per_block<VIEW_COLORS>();
+    const uint32_t v0 = blockIdx.y * VPB;
+    const uint32_t v1 = min(v0 + VPB, a.n_views);
     if (i < a.counter_words)
         for (uint32_t v = v0; v < v1; v++) a.views[v].counters[i] = (i == a.count_slot) ? a.n : 0u;
     if (i >= a.n) return;
@@ -306,15 +325,32 @@ __global__ __launch_bounds__(256) void k_preprocess_views(MultiPreprocessArgs a)
             o2.y = __half2float(__ushort_as_half((unsigned short)(col.x >> 16)));
             o2.z = __half2float(__ushort_as_half((unsigned short)(col.y & 0xFFFFu)));
         }
-        store_projected(vp.out, i, count, o0, o1, o2, o3, vp.depth_keys, vp.rects, vp.rects8);
+        if constexpr (FULL) {
+            store_projected(reinterpret_cast<gs3d_projected_splat*>(vp.out), i, count, o0, o1, o2, o3,
+                            vp.depth_keys, vp.rects, vp.rects8);
+        } else if (count) {
+            float4* out = vp.out + (size_t)i * 3;
+            out[0] = o0;
+            out[1] = o1;
+            out[2] = o2;
+            vp.depth_keys[i] = __float_as_uint(-o0.z);
+            if (vp.rects)
+                vp.rects[i] = make_uint2(__float_as_uint(o3.x) | (__float_as_uint(o3.y) << 16),
+                                         __float_as_uint(o3.z) | (__float_as_uint(o3.w) << 16));
+            if (vp.rects8)
+                vp.rects8[i] = __float_as_uint(o3.x) | (__float_as_uint(o3.y) << 8) |
+                               (__float_as_uint(o3.z) << 16) | (__float_as_uint(o3.w) << 24);
+        } else {
+            vp.depth_keys[i] = 0xFFFFFFFFu;
+        }
     };
     if constexpr (VIEW_COLORS) {
-        uint2 vc[kViewsPerBlock];
+        uint2 vc[VPB];
 #pragma unroll
-        for (int k = 0; k < kViewsPerBlock; k++)
+        for (int k = 0; k < (int)VPB; k++)
             vc[k] = (v0 + k < v1) ? a.view_colors[(size_t)(v0 + k) * a.n + i] : make_uint2(0u, 0u);
 #pragma unroll
-        for (int k = 0; k < kViewsPerBlock; k++)
+        for (int k = 0; k < (int)VPB; k++)
             if (v0 + k < v1) do_view(v0 + k, true, vc[k]);
     } else {
         for (uint32_t v = v0; v < v1; v++) do_view(v, false, make_uint2(0u, 0u));
@@ -350,24 +386,33 @@ hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s) {
+template <bool FULL>
+static hipError_t launch_views(const MultiPreprocessArgs& a, hipStream_t s) {
     const uint32_t threads = std::max(a.n, a.counter_words);
     if (threads == 0 || a.n_views == 0) return hipSuccess;
-    const dim3 grid((threads + 255u) / 256u, (a.n_views + kViewsPerBlock - 1) / kViewsPerBlock);
+    const uint32_t vpb = views_per_block<false>(), vpb_colors = views_per_block<true>();
+    const dim3 grid((threads + 255u) / 256u, (a.n_views + vpb - 1) / vpb);
     // learned colours replace the SH evaluation (the DC colour path is the
     // cheapest projection; its colour is then overwritten)
     if (a.view_colors) {
-        hipLaunchKernelGGL((k_preprocess_views<0, true>), grid, dim3(256), 0, s, a);
+        const dim3 grid_colors(grid.x, (a.n_views + vpb_colors - 1) / vpb_colors);
+        hipLaunchKernelGGL((k_preprocess_views<0, true, FULL>), grid_colors, dim3(256), 0, s, a);
         return hipGetLastError();
     }
     switch (a.sh_degree) {
-        case 0: hipLaunchKernelGGL((k_preprocess_views<0, false>), grid, dim3(256), 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_preprocess_views<1, false>), grid, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_preprocess_views<2, false>), grid, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_preprocess_views<3, false>), grid, dim3(256), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_preprocess_views<0, false, FULL>), grid, dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_preprocess_views<1, false, FULL>), grid, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_preprocess_views<2, false, FULL>), grid, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_preprocess_views<3, false, FULL>), grid, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_preprocess_views(const MultiPreprocessArgs& a, hipStream_t s) { return launch_views<false>(a, s); }
+
+hipError_t launch_preprocess_views_full(const MultiPreprocessArgs& a, hipStream_t s) {
+    return launch_views<true>(a, s);
 }
 
 hipError_t launch_sh_relayout(const float* f_dc, const float* rest, float* sh, uint32_t n, int deg,

@@ -130,7 +130,9 @@ struct alignas(16) Survivor {
     float2 pad;
 };
 
-// TW x TW tile; RasterShape gives the wavefronts' pixel blocks.
+// TW x TW tile; RasterShape gives the wavefronts' pixel blocks. `proj` holds
+// one record of `rec` float4s per splat (4: gs3d_projected_splat, 3: a batch
+// view's 48-B record); only the first three float4s are read.
 //
 // Per batch of 64 list entries a wavefront gathers one splat per lane, keeps
 // the ones that may touch its block (may_touch) and compacts them, in list
@@ -142,6 +144,7 @@ struct alignas(16) Survivor {
 // C + T * 0 * c = C). The all-saturated exit is checked once per 4 splats.
 template <int TW, int PPL>
 __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __restrict__ proj,
+                                                             uint32_t rec,
                                                              const uint32_t* __restrict__ values,
                                                              const uint32_t* __restrict__ tile_offsets,
                                                              uint32_t tiles_x, uint32_t width,
@@ -215,9 +218,9 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __res
     float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
     if (start < end && start + lane < end) {
         const uint32_t sid = values[start + lane];
-        n0 = proj[(size_t)sid * 4 + 0];
-        n1 = proj[(size_t)sid * 4 + 1];
-        n2 = proj[(size_t)sid * 4 + 2];
+        n0 = proj[(size_t)sid * rec + 0];
+        n1 = proj[(size_t)sid * rec + 1];
+        n2 = proj[(size_t)sid * rec + 2];
     }
     for (uint32_t base = start; base < end; base += kWave) {
         if (all_done()) break;
@@ -226,9 +229,9 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __res
         const uint32_t nidx = base + kWave + lane;
         if (nidx < end) {
             const uint32_t sid = values[nidx];
-            n0 = proj[(size_t)sid * 4 + 0];
-            n1 = proj[(size_t)sid * 4 + 1];
-            n2 = proj[(size_t)sid * 4 + 2];
+            n0 = proj[(size_t)sid * rec + 0];
+            n1 = proj[(size_t)sid * rec + 1];
+            n2 = proj[(size_t)sid * rec + 2];
         }
         const bool hit = have && may_touch(f0, f1, rx0, rx1, ry0, ry1);
         const uint64_t mask = __ballot(hit);
@@ -306,10 +309,10 @@ hipError_t launch_rebuild_keys(const uint32_t* tiles, const uint32_t* values, co
 }
 
 template <int TW, int PPL>
-static void launch_raster_variant(uint32_t tiles, const float4* p, const uint32_t* values,
+static void launch_raster_variant(uint32_t tiles, const float4* p, uint32_t rec, const uint32_t* values,
                                   const uint32_t* tile_offsets, uint32_t tiles_x, uint32_t width,
                                   uint32_t height, uint2* img, hipStream_t s) {
-    hipLaunchKernelGGL((k_rasterize<TW, PPL>), dim3(tiles), dim3(TW * TW / PPL), 0, s, p, values,
+    hipLaunchKernelGGL((k_rasterize<TW, PPL>), dim3(tiles), dim3(TW * TW / PPL), 0, s, p, rec, values,
                        tile_offsets, tiles_x, width, height, img);
 }
 
@@ -323,20 +326,28 @@ hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_
                             const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                             uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
                             uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s) {
+    return launch_rasterize_records(reinterpret_cast<const float4*>(projected), 4, values, tile_offsets, tile_x,
+                                    tile_y, tiles_x, tiles_y, width, height, pixels_per_lane, image_rgba16f, s);
+}
+
+hipError_t launch_rasterize_records(const float4* p, uint32_t rec, const uint32_t* values,
+                                    const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
+                                    uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
+                                    uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s) {
     const uint32_t tiles = tiles_x * tiles_y;
     if (tiles == 0) return hipSuccess;
     if (tile_x != tile_y || !raster_variant_exists(tile_x, pixels_per_lane)) return hipErrorInvalidValue;
-    const float4* p = reinterpret_cast<const float4*>(projected);
+    if (rec != 3 && rec != 4) return hipErrorInvalidValue;
     uint2* img = reinterpret_cast<uint2*>(image_rgba16f);
     const int key = tile_x * 10 + pixels_per_lane;
     switch (key) {
-        case 81: launch_raster_variant<8, 1>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
-        case 161: launch_raster_variant<16, 1>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
-        case 162: launch_raster_variant<16, 2>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
-        case 164: launch_raster_variant<16, 4>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
-        case 321: launch_raster_variant<32, 1>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
-        case 322: launch_raster_variant<32, 2>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
-        case 324: launch_raster_variant<32, 4>(tiles, p, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 81: launch_raster_variant<8, 1>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 161: launch_raster_variant<16, 1>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 162: launch_raster_variant<16, 2>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 164: launch_raster_variant<16, 4>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 321: launch_raster_variant<32, 1>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 322: launch_raster_variant<32, 2>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        case 324: launch_raster_variant<32, 4>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

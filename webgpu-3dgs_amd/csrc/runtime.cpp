@@ -81,6 +81,7 @@ struct gs3d_ctx {
 
     // last frame (for re-runs after a capacity miss)
     bool have_frame = false;
+    uint64_t splat_gen = 0;  // counts uploads: what a batch's outputs were made from
     gs3d_camera last_cam{};
     uint32_t last_w = 0, last_h = 0, last_flags = 0;
     uint32_t* pinned = nullptr;  // host-mapped; [0] = K of the last frame, written by identify
@@ -568,8 +569,11 @@ int settle(gs3d_ctx* c) {
 // in device memory, so a new set of cameras replays the same graph.
 
 constexpr int kBatchBranches = 4;
+// a batch view's record per splat: the first three float4s of a gs3d_projected_splat
+constexpr size_t kBatchRecordBytes = 48;
 
 struct BatchView {
+    gs3d_camera cam{};
     uint32_t W = 0, H = 0, tiles_x = 0, tiles_y = 0;
     size_t img_off = 0, to_off = 0;  // offsets into the batch image / tile-range arrays (elements)
     int branch = 0;
@@ -594,6 +598,7 @@ struct BatchState {
     bool have = false;
     uint64_t replays = 0;
     bool learned = false;  // the captured graph computes learned colours
+    uint64_t splat_gen = 0;  // the splats the last batch projected
 };
 
 void drop_graph(BatchState* b) {
@@ -659,7 +664,7 @@ int batch_buffers(gs3d_ctx* c, bool* moved) {
     const uint32_t radix = 1u << c->digit_bits;
     std::vector<std::pair<std::string, size_t>> want = {
         {"batch_views", V * sizeof(ViewParams)},
-        {"batch_projected", V * (size_t)n * sizeof(gs3d_projected_splat)},
+        {"batch_projected", V * (size_t)n * kBatchRecordBytes},
         {"batch_depth_keys", V * (size_t)n * 4},
         {"batch_rects", V * (size_t)n * 8},
         {"batch_counters", V * kCtrTotalWords * 4},
@@ -742,7 +747,7 @@ int batch_enqueue(gs3d_ctx* c) {
     const uint32_t V = (uint32_t)b->views.size();
     const uint32_t bits = c->digit_bits, radix = 1u << bits;
     ViewParams* vparams = buf<ViewParams>(c, "batch_views");
-    auto* projected = buf<gs3d_projected_splat>(c, "batch_projected");
+    auto* projected = buf<float4>(c, "batch_projected");
     uint32_t* dk = buf<uint32_t>(c, "batch_depth_keys");
     uint2* rects = buf<uint2>(c, "batch_rects");
     uint32_t* counters = buf<uint32_t>(c, "batch_counters");
@@ -804,7 +809,7 @@ int batch_enqueue(gs3d_ctx* c) {
         uint32_t* vb = buf<uint32_t>(c, branch_name(k, "values_b").c_str());
         uint64_t* scan_status = buf<uint64_t>(c, branch_name(k, "scan_status").c_str());
         uint64_t* radix_status = buf<uint64_t>(c, branch_name(k, "radix_status").c_str());
-        const gs3d_projected_splat* proj = projected + (size_t)vi * n;
+        const float4* proj = projected + (size_t)vi * n * 3;
 
         HIP_TRY(c, launch_key_histogram(dka, 4, ctr + kCtrSplatCount, n, hist_depth, hist_partial, bits,
                                         depth_passes, 0, s));
@@ -831,7 +836,7 @@ int batch_enqueue(gs3d_ctx* c) {
         sa.epoch_base = epoch_base;
         HIP_TRY(c, launch_scan_rects(sa, s));
         EmitArgs ea{};
-        ea.projected = proj;
+        ea.projected = nullptr;  // emit reads the scan's records
         ea.order = sa.order;
         ea.recs = recs;
         ea.tiles = reinterpret_cast<uint32_t*>(ka);
@@ -854,8 +859,8 @@ int batch_enqueue(gs3d_ctx* c) {
         uint32_t* to = tile_offsets + v.to_off;
         HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, 4, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
                                          b->pinned_k + vi, s));
-        HIP_TRY(c, launch_rasterize(proj, in_b ? vb : va, to, c->tile_x, c->tile_y, v.tiles_x, v.tiles_y, v.W,
-                                    v.H, c->raster_ppl, images + v.img_off, s));
+        HIP_TRY(c, launch_rasterize_records(proj, 3, in_b ? vb : va, to, c->tile_x, c->tile_y, v.tiles_x,
+                                            v.tiles_y, v.W, v.H, c->raster_ppl, images + v.img_off, s));
     }
     for (int k = 0; k < kBatchBranches; k++) {
         if (epoch_ctr[k] >= kEpochStride)
@@ -893,6 +898,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
     size_t img = 0, to = 0;
     for (uint32_t i = 0; i < n_views; i++) {
         BatchView& v = b->views[i];
+        v.cam = views[i].camera;
         v.W = views[i].width;
         v.H = views[i].height;
         v.tiles_x = (v.W + c->tile_x - 1) / c->tile_x;
@@ -935,6 +941,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
     b->ppl = c->raster_ppl;
     b->learned = learned;
     b->have = false;
+    b->splat_gen = c->splat_gen;
     const bool eager = (flags & GS3D_BATCH_EAGER) != 0;
     for (int attempt = 0; attempt < 3; attempt++) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -943,7 +950,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
         if (moved) drop_graph(b);
         // cameras and per-view buffer addresses
         std::vector<ViewParams> vp(n_views);
-        auto* projected = buf<gs3d_projected_splat>(c, "batch_projected");
+        auto* projected = buf<float4>(c, "batch_projected");
         uint32_t* dk = buf<uint32_t>(c, "batch_depth_keys");
         uint2* rects = buf<uint2>(c, "batch_rects");
         uint32_t* counters = buf<uint32_t>(c, "batch_counters");
@@ -953,7 +960,7 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
             vp[i].tiles_y = b->views[i].tiles_y;
             vp[i].inv_tile_x = (float)(1.0 / (double)c->tile_x);
             vp[i].inv_tile_y = (float)(1.0 / (double)c->tile_y);
-            vp[i].out = projected + (size_t)i * c->n;
+            vp[i].out = projected + (size_t)i * c->n * 3;
             vp[i].depth_keys = dk + (size_t)i * c->n;
             // a view's 8 B per splat of rectangle memory holds, when packed, the two
             // 4-B arrays its depth passes carry the rectangles between
@@ -1217,6 +1224,7 @@ int gs3d_set_splats(gs3d_ctx* c, const float* pos4, const float* scale4, const f
     c->n = n;
     c->sh_degree = sh_degree;
     c->have_frame = false;
+    c->splat_gen++;
     c->key_capacity = 0;
     // latents belong to the splats they were trained on
     c->learned_init = false;
@@ -1840,6 +1848,43 @@ int gs3d_batch_ref_counts(gs3d_ctx* c, uint64_t* out, uint32_t n) {
     return GS3D_OK;
 }
 
+// A batch keeps 48-B records of its views' visible splats only: the 64-B
+// gs3d_projected_splat records of one view (zeros where culled) are made
+// again by the batch's projection kernel, with the batch's cameras, tiles and
+// learned colours, into a readback buffer.
+int batch_view_records(gs3d_ctx* c, uint32_t view, const void** out) {
+    BatchState* b = c->batch;
+    if (b->splat_gen != c->splat_gen)
+        return fail(c, GS3D_E_STATE, "the splats changed after the batch; its records are gone");
+    const BatchView& v = b->views[view];
+    const uint32_t n = b->n;
+    gs3d_projected_splat* rec;
+    ViewParams* one;
+    GS3D_TRY(ensure_t(c, "readback_projected", std::max<size_t>(n, 1), &rec));
+    GS3D_TRY(ensure_t(c, "readback_view", 1, &one));
+    ViewParams vp{};
+    vp.cam = v.cam;
+    vp.tiles_x = v.tiles_x;
+    vp.tiles_y = v.tiles_y;
+    vp.inv_tile_x = vp.inv_tile_y = (float)(1.0 / (double)b->tile);
+    vp.out = reinterpret_cast<float4*>(rec);
+    HIP_TRY(c, hipMemcpyAsync(one, &vp, sizeof(vp), hipMemcpyHostToDevice, c->stream));
+    MultiPreprocessArgs ma{};
+    ma.views = one;
+    ma.n_views = 1;
+    ma.view_colors = b->learned ? buf<uint2>(c, "batch_view_colors") + (size_t)view * n : nullptr;
+    ma.pos = buf<float4>(c, "splat_positions");
+    ma.scale = buf<float4>(c, "splat_scales");
+    ma.rot = buf<float4>(c, "splat_rotations");
+    ma.color = buf<float4>(c, "splat_colors");
+    ma.sh = b->deg > 0 ? buf<float>(c, "splat_sh") : nullptr;
+    ma.n = n;
+    ma.sh_degree = b->deg;
+    HIP_TRY(c, launch_preprocess_views_full(ma, c->stream));
+    *out = rec;
+    return GS3D_OK;
+}
+
 int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes, size_t* written) {
     if (!c) return GS3D_E_INVALID_ARG;
     BatchState* b = c->batch;
@@ -1858,7 +1903,9 @@ int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes,
         case GS3D_BUF_TILE_OFFSETS:
             src = buf<uint32_t>(c, "batch_tile_offsets") + v.to_off; size = ((size_t)v.tiles_x * v.tiles_y + 1) * 4; break;
         case GS3D_BUF_PROJECTED_SPLATS:
-            src = buf<gs3d_projected_splat>(c, "batch_projected") + (size_t)view * c->n; size = (size_t)c->n * 64; break;
+            size = (size_t)b->n * sizeof(gs3d_projected_splat);
+            if (dst && bytes >= size) GS3D_TRY(batch_view_records(c, view, &src));
+            break;
         case GS3D_BUF_REF_COUNTER:
             src = buf<uint32_t>(c, "batch_counters") + (size_t)view * kCtrTotalWords + kCtrRefCounter; size = 4; break;
         default: return fail(c, GS3D_E_UNKNOWN_NAME, "buffer %d is not kept per batch view", id);
@@ -1866,6 +1913,7 @@ int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes,
     if (written) *written = size;
     if (!dst) return GS3D_OK;
     if (bytes < size) return fail(c, GS3D_E_TOO_SMALL, "buffer %d needs %zu bytes, got %zu", id, size, bytes);
+    if (size == 0) return GS3D_OK;
     if (widen) {
         float* tmp;
         GS3D_TRY(ensure_t(c, "readback_f32", px * 4, &tmp));
