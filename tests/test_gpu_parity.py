@@ -218,6 +218,25 @@ def test_pipeline_empty_and_all_culled(renderer):
     assert renderer.read_tile_counts().sum() == 0
 
 
+def test_pipeline_no_splats(renderer):
+    """An empty scene (SH3, N = 0): K = 0, zero tile ranges and a cleared image, for a
+    single view and for a batch; a batch view's projected splats read back empty."""
+    s = sc.activate(sc.uniform_cube(45, 0, sh_degree=3))
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    cam = cam_mod.view_camera((0.0, 0.0, 3.0), 96, 64)
+    renderer.rebin(cam, 96, 64)
+    assert renderer.ref_count() == 0
+    assert not renderer.read(L.BUF_TILE_OFFSETS, np.uint32).any()
+    assert not renderer.read(L.BUF_IMAGE_RGBA32F, np.float32).any()
+    renderer.rebin_batch([(cam, 96, 64), (cam_mod.view_camera((0.0, 1.0, 3.0), 40, 30), 40, 30)])
+    assert [int(k) for k in renderer.batch_ref_counts()] == [0, 0]
+    for i in range(2):
+        assert not renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16).any()
+        assert not renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32).any()
+        assert renderer.batch_read(i, L.BUF_PROJECTED_SPLATS, np.uint32).size == 0
+
+
 @pytest.mark.parametrize("W,H,tile", [(1, 1, 16), (16, 16, 16), (17, 9, 8), (200, 100, 32), (33, 700, 16),
                                       (4096, 40, 16), (4200, 40, 16)])
 def test_pipeline_tiny_and_ragged_viewports(renderer, W, H, tile):

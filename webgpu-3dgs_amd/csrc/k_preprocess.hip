@@ -269,11 +269,11 @@ __global__ __launch_bounds__(256) void k_preprocess(PreprocessArgs a) {
 
 // Views one workgroup of the multi-view preprocess projects each splat into:
 // the splat's attributes and SH coefficients are read once per kViewsPerBlock
-// views (measured on 64 views of 2M SH3 splats: 2,803 / 2,925 / 2,964 / 2,988
-// views/s at 4 / 8 / 16 / 32). The learned-colour kernel prefetches one colour
-// per view and keeps 8 (1,722 views/s at 8, 1,700 at 32, whose prefetch takes
-// 167 VGPRs).
-constexpr uint32_t kViewsPerBlock = 32;
+// views (measured on 64 views of 2M SH3 splats: 2,803 / 2,925 / 2,964 / 2,975
+// / 3,015 views/s at 4 / 8 / 16 / 32 / 64). The learned-colour kernel
+// prefetches one colour per view and keeps 8 (1,722 views/s at 8, 1,700 at 32,
+// whose prefetch takes 167 VGPRs).
+constexpr uint32_t kViewsPerBlock = 64;
 constexpr uint32_t kViewsPerBlockColors = 8;
 
 template <bool VIEW_COLORS>
