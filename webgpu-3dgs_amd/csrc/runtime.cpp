@@ -1868,7 +1868,9 @@ int batch_view_records(gs3d_ctx* c, uint32_t view, const void** out) {
     vp.tiles_y = v.tiles_y;
     vp.inv_tile_x = vp.inv_tile_y = (float)(1.0 / (double)b->tile);
     vp.out = reinterpret_cast<float4*>(rec);
+    // vp lives on this stack frame: the copy completes before it returns
     HIP_TRY(c, hipMemcpyAsync(one, &vp, sizeof(vp), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     MultiPreprocessArgs ma{};
     ma.views = one;
     ma.n_views = 1;
