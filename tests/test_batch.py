@@ -119,6 +119,17 @@ def test_batch_four_bit_digits(renderer, scene_sh3):
         renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)
 
 
+def test_batch_wide_view(renderer, scene_sh3):
+    """A view more than 256 tiles wide keeps 8-B tile rectangles that the scan gathers
+    (no packed rectangles carried by the depth passes), beside a packed view."""
+    say("batch: wide view")
+    renderer.set_splat_data(scene_sh3)
+    cams = [(cam_mod.view_camera((5.0, 2.0, 6.0), 4200, 40), 4200, 40),
+            (cam_mod.view_camera((4.0, -1.0, 7.0), 320, 180), 320, 180)]
+    got = check_batch(renderer, cams)
+    assert got[0]["k"] > 0 and got[1]["k"] > 0
+
+
 def test_batch_records_follow_the_batch_splats(renderer, scene_sh3):
     """A batch view's projected records are made again on read from the splats the
     batch projected: after a new upload they are refused, the image still reads."""
