@@ -6,8 +6,8 @@
 // the last populated one keep the 0 of the per-rebin clear. That array equals,
 // for every tile t: lower_bound(key.hi >= t) if t <= last+1, else 0 (last =
 // tile of the final key; all zero when there are no keys). One thread per tile
-// evaluates that closed form with a binary search over the sorted keys, so no
-// thread walks a long gap serially.
+// evaluates that closed form with a search over the sorted keys (8-ary rounds,
+// then binary), so no thread walks a long gap serially.
 //
 // rasterize: same per-pixel arithmetic and order as rasterize_tiles.wgsl:43-110
 // (pixel centre +0.5; skip if |d|^2 > r^2; power = -1/2 d^T conic d, skip if
@@ -35,6 +35,8 @@ namespace gs3d {
 
 namespace {
 
+constexpr uint32_t kWideSearchKeys = 32u << 20;
+
 // The tile id of sorted key i sits at word stride * i + word (64-bit keys:
 // stride 2, the high word; 32-bit tile ids: stride 1).
 __global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ kw, uint32_t stride,
@@ -51,7 +53,23 @@ __global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ k
     if (n > 0) {
         const uint32_t last = kw[(size_t)stride * (n - 1) + word];
         if (t <= last + 1u) {
-            uint32_t lo = 0, hi = n;  // first index whose tile id >= t
+            // first index whose tile id >= t, in [lo, hi]. While the range is
+            // wide, 7 independent probes p_j = lo + j*step (all < hi <= n) cut it
+            // about 8-fold per round: the c probes below t are a prefix, so the
+            // answer lies in (p_c, p_{c+1}]. Then a binary search. Past
+            // kWideSearchKeys keys the extra probes miss the cache and the
+            // binary search alone is faster (measured: 3.6M keys 10.5 -> 9.3 us,
+            // 118.9M keys 19.5 -> 23 us with the probes).
+            uint32_t lo = 0, hi = n;
+            while (n <= kWideSearchKeys && hi - lo > 8u) {
+                const uint32_t step = (hi - lo) >> 3;
+                uint32_t c = 0;
+#pragma unroll
+                for (uint32_t j = 1; j < 8; j++) c += kw[(size_t)stride * (lo + j * step) + word] < t ? 1u : 0u;
+                const uint32_t nlo = c ? lo + c * step + 1u : lo;
+                hi = c < 7u ? lo + (c + 1u) * step : hi;
+                lo = nlo;
+            }
             while (lo < hi) {
                 const uint32_t mid = lo + ((hi - lo) >> 1);
                 if (kw[(size_t)stride * mid + word] < t) lo = mid + 1; else hi = mid;
