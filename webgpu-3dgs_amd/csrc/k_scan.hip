@@ -34,6 +34,9 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
+// The in-range paths of both scans load a thread's items as two uint4.
+static_assert(kScanItems == 8, "k_scan and k_scan_rects load 8 items per thread as two uint4");
+
 __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ order,
                                                        uint32_t* __restrict__ offsets,
