@@ -226,6 +226,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs a) {
 // 8 / 16 / 32 / 64 / 128 / 256 -> 0.425 / 0.424 / 0.424 / 0.427 / 0.438 /
 // 0.457 ms.
 constexpr uint32_t kReduceWords = 64, kReduceRows = 32;
+static_assert(kReduceWords * 4 == 256 && kReduceRows % 4 == 0,
+              "k_hist_reduce: four groups of kReduceWords threads, a quarter of the rows each");
 
 __global__ __launch_bounds__(256) void k_hist_reduce(const uint32_t* __restrict__ partial,
                                                      uint32_t rows, uint32_t words,
