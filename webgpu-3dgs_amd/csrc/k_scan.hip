@@ -34,8 +34,9 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
     return v;
 }
 
-// The in-range paths of both scans load a thread's items as two uint4.
-static_assert(kScanItems == 8, "k_scan and k_scan_rects load 8 items per thread as two uint4");
+// The in-range paths of both scans load (and k_scan stores) a thread's items
+// as uint4.
+static_assert(kScanItems % 4 == 0, "k_scan and k_scan_rects load a thread's items as uint4");
 
 __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restrict__ counts,
                                                        const uint32_t* __restrict__ order,
@@ -60,9 +61,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restric
         for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? counts[order[base + k]] : 0u;
     } else if (base + kScanItems <= n) {
         const uint4* src = reinterpret_cast<const uint4*>(counts + base);
-        uint4 x0 = src[0], x1 = src[1];
-        v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w;
-        v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; q++) {
+            const uint4 x = src[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? counts[base + k] : 0u;
@@ -129,8 +132,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restric
     }
     if (base + kScanItems <= n) {
         uint4* dst = reinterpret_cast<uint4*>(offsets + base);
-        dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-        dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; q++) dst[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; k++)
@@ -168,11 +171,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_rects(ScanRectsArgs a) {
     if (base + kScanItems <= n) {
         const uint4* o = reinterpret_cast<const uint4*>(first + base);
         const uint4* d = reinterpret_cast<const uint4*>(a.depth_keys + base);
-        const uint4 o0 = o[0], o1 = o[1], d0 = d[0], d1 = d[1];
-        sid[0] = o0.x; sid[1] = o0.y; sid[2] = o0.z; sid[3] = o0.w;
-        sid[4] = o1.x; sid[5] = o1.y; sid[6] = o1.z; sid[7] = o1.w;
-        dk[0] = d0.x; dk[1] = d0.y; dk[2] = d0.z; dk[3] = d0.w;
-        dk[4] = d1.x; dk[5] = d1.y; dk[6] = d1.z; dk[7] = d1.w;
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; q++) {
+            const uint4 oq = o[q], dq = d[q];
+            sid[4 * q] = oq.x; sid[4 * q + 1] = oq.y; sid[4 * q + 2] = oq.z; sid[4 * q + 3] = oq.w;
+            dk[4 * q] = dq.x; dk[4 * q + 1] = dq.y; dk[4 * q + 2] = dq.z; dk[4 * q + 3] = dq.w;
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; k++) {
