@@ -115,13 +115,16 @@ def forward(p, z, d, bf16=True):
     return y, dict(xb=xb, d1=d1, h1=h1, d2=d2, h2=h2, y=y, W1b=W1b, W2b=W2b, W3b=W3b)
 
 
-def loss_and_grads(p, z, d, t, bf16=True):
+def loss_and_grads(p, z, d, t, bf16=True, s_total=None):
     """Mean squared error over samples x channels and its gradients.
 
     z (S, 8) latents per sample, d (S, 3), t (S, 3) targets. Returns
-    (loss, grad_params (5123,), grad_z (S, 8)).
+    (loss, grad_params (5123,), grad_z (S, 8)). With s_total, the samples are
+    a subset of s_total and the mean (and every gradient, rounded where the
+    kernels round) is taken over s_total: grad_z rows then equal the full
+    computation's rows for these samples.
     """
-    S = z.shape[0]
+    S = z.shape[0] if s_total is None else s_total
     y, c = forward(p, z, d, bf16)
     r = to_bf16 if bf16 else (lambda a: np.asarray(a, np.float64))
     diff = y - t
@@ -142,15 +145,19 @@ def loss_and_grads(p, z, d, t, bf16=True):
     return loss, pack(gW1, gb1, gW2, gb2, gW3, gb3), gz
 
 
-def step_grads(p, latents, f_dc, sh_rest, deg, first_dir, dirs=None, bf16=True):
+def step_grads(p, latents, f_dc, sh_rest, deg, first_dir, dirs=None, bf16=True, n_total=None):
     """The kernel's step: all splats x directions [first_dir, first_dir + 16).
 
-    Returns (loss, grad_params, grad_latents (n, 8) summed over the 16 directions)."""
+    Returns (loss, grad_params, grad_latents (n, 8) summed over the 16 directions).
+    With n_total, the splats given are a subset of a scene of n_total splats:
+    grad_latents are those splats' rows of the whole scene's step (the loss and
+    grad_params are then partial sums over the subset)."""
     dirs = fibonacci_dirs() if dirs is None else dirs
     n = latents.shape[0]
     dsel = dirs[first_dir:first_dir + 16].astype(np.float64)
     t = sh_targets(f_dc, sh_rest, deg, dsel)  # (n, 16, 3)
     z = np.repeat(np.asarray(latents, np.float64), 16, axis=0)
     d = np.tile(dsel, (n, 1))
-    loss, gp, gz = loss_and_grads(p, z, d, t.reshape(-1, 3), bf16)
+    loss, gp, gz = loss_and_grads(p, z, d, t.reshape(-1, 3), bf16,
+                                  None if n_total is None else 16 * n_total)
     return loss, gp, gz.reshape(n, 16, 8).sum(axis=1)

@@ -201,3 +201,60 @@ def test_batch_all_culled_views(renderer, scene_sh3):
     got = check_batch(renderer, cams)
     assert got[0]["k"] == 0 and got[2]["k"] == 0 and got[1]["k"] > 0
     assert not got[0]["image"].any() and not got[0]["tile_offsets"].any()
+
+
+def test_batch_odd_splat_count(renderer):
+    """An odd splat count (n % 4 != 0): every view's depth-key and rectangle slices of the
+    batch arrays still start 16-B aligned, and every view equals its single-view rebin."""
+    say("batch: odd splat count")
+    s = S.activate(S.clustered(33, 40_001, 12, sh_degree=3))
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    check_batch(renderer, views_for(9, 5))
+    s3 = S.activate(S.clustered(34, 3, 1, sh_degree=1))
+    renderer.set_splat_data(s3)
+    check_batch(renderer, views_for(10, 3))
+
+
+@pytest.mark.timeout(900)
+def test_workload_2m_sh3_64_views(renderer):
+    """The batched workload itself: 2M SH3 splats, 64 seeded orbit views of uneven
+    resolution (640x360 / 1280x720 / 1920x1080) in one call. Every view: K equals the view's
+    counter, its tile ranges are consistent (monotone up to the last populated tile + 1, then
+    zero, ending at K); one view of each resolution against the oracle: K and tile ranges
+    bit-exact, image within 1/255."""
+    say("batch: workload clusters-2M-sh3-64views")
+    O.set_threads(16)
+    wl = S.workload(3)
+    s = S.activate(wl.raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    views = [(cam_mod.view_camera(eye, w, h, wl.fov_y), w, h) for eye, w, h in wl.views]
+    renderer.rebin_batch(views)
+    ks = renderer.batch_ref_counts()
+    say(f"  K total {int(ks.sum())}, largest view {int(ks.max())}")
+    assert len(ks) == 64 and ks.min() > 0
+    for i, (_, w, h) in enumerate(views):
+        to = renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32)
+        assert to.size == (-(-w // 16)) * (-(-h // 16)) + 1
+        assert int(renderer.batch_read(i, L.BUF_REF_COUNTER, np.uint32)[0]) == int(ks[i])
+        nz = np.nonzero(to)[0]
+        last = int(nz[-1])
+        assert to[last] == ks[i]
+        assert np.all(np.diff(to[:last + 1].astype(np.int64)) >= 0)
+    checked = set()
+    for i, (cam, w, h) in enumerate(views):
+        if (w, h) in checked:
+            continue
+        checked.add((w, h))
+        ref = O.pipeline_f32(cam, s, w, h, 16)
+        assert int(ks[i]) == ref["k"], (i, w, h)
+        assert np.array_equal(renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32), ref["tile_offsets"])
+        img = renderer.batch_read(i, L.BUF_IMAGE_RGBA32F, np.float32).reshape(h, w, 4)
+        err = float(np.abs(img - ref["image"]).max())
+        say(f"  view {i} {w}x{h}: K={ref['k']} image max abs vs oracle f32 {err:.3g}")
+        assert err <= TOL_IMAGE
+    assert len(checked) == 3
+
+
+TOL_IMAGE = 1.0 / 255.0

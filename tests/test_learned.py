@@ -90,6 +90,24 @@ def test_oracle_sh_basis_and_targets():
     assert np.allclose(t, np.clip(0.5 + 0.28209479177387814 * f_dc, 0, 1)[:, None, :])
 
 
+def test_oracle_subset_rows_equal_whole_scene_rows():
+    """step_grads with n_total: the latent-gradient rows of a subset of splats equal the
+    rows of the whole scene's step, bit for bit (every latent row depends only on its own
+    splat's samples and the shared mean scale)."""
+    rng = np.random.default_rng(6)
+    n = 40
+    p = random_params(rng)
+    z = rng.normal(0, 0.3, (n, 8))
+    f_dc = rng.normal(0, 0.5, (n, 3))
+    rest = rng.normal(0, 0.1, (n, 45))
+    _, _, full = LO.step_grads(p, z, f_dc, rest, 3, 32)
+    pick = np.array([0, 7, 8, 39])
+    _, _, sub = LO.step_grads(p, z[pick], f_dc[pick], rest[pick], 3, 32, n_total=n)
+    assert np.array_equal(sub, full[pick])
+    _, _, wrong = LO.step_grads(p, z[pick], f_dc[pick], rest[pick], 3, 32)
+    assert not np.allclose(wrong, full[pick])
+
+
 def test_library_directions_equal_oracle():
     from gs3d.renderer import GaussianSplatRenderer
 
@@ -120,12 +138,18 @@ def test_step_gradients_match_oracle(renderer, small_scene, first_dir):
     s = small_scene
     rl, rgp, rgl = LO.step_grads(p, z, s.f_dc, s.sh_rest, 3, first_dir)
     say(f"  loss gpu {loss:.6g} oracle {rl:.6g}; rel err params {rel_err(gp, rgp):.3g} latents {rel_err(gl, rgl):.3g}")
-    assert abs(loss - rl) <= 1e-3 * rl + 1e-7
-    assert rel_err(gp, rgp) < 2e-2
-    assert rel_err(gl, rgl) < 2e-2
+    # The kernels and the oracle round to bf16 at the same points and differ only in fp32
+    # summation order: measured 1e-7 to 1e-6 (parameters) and up to 1.6e-5 (latents).
+    # A missing bf16 rounding point or a wrong direction index is off by 1e-3 or more.
+    assert abs(loss - rl) <= 1e-5 * rl + 1e-9
+    assert rel_err(gp, rgp) < 1e-4
+    assert rel_err(gl, rgl) < 1e-4
     # per block (W1, b1, W2, b2, W3, b3), so no block hides behind a larger one
-    for a, b in [(0, 704), (704, 768), (768, 4864), (4864, 4928), (4928, 5120), (5120, 5123)]:
-        assert rel_err(gp[a:b], rgp[a:b]) < 3e-2, (a, b)
+    blocks = [(0, 704), (704, 768), (768, 4864), (4864, 4928), (4928, 5120), (5120, 5123)]
+    per_block = [rel_err(gp[a:b], rgp[a:b]) for a, b in blocks]
+    say("  per block: " + " ".join(f"{e:.2g}" for e in per_block))
+    for (a, b), e in zip(blocks, per_block):
+        assert e < 3e-4, (a, b, e)
 
 
 @pytest.mark.gpu
@@ -232,3 +256,48 @@ def test_new_splats_reset_the_learned_state(renderer, small_scene):
     exact = renderer.batch_read(0, L.BUF_IMAGE_RGBA16F, np.uint16)
     renderer.rebin_batch(views, learned=True)
     assert np.array_equal(renderer.batch_read(0, L.BUF_IMAGE_RGBA16F, np.uint16), exact)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_workload_3m_sh3_32_views_learned(renderer):
+    """The learned-colour workload itself: 3M SH3 splats, 32 views at 1920x1080.
+    Latent gradients of one step for a sample of splats against the oracle (rows of the
+    whole scene's step), a short training that lowers the loss, inference colours of a
+    sample against the oracle's forward, a gate decision consistent with the PSNR, and,
+    below the gate, learned batches that render exactly the exact-SH images."""
+    say("learned: workload clusters-3M-sh3-32views-learned")
+    wl = S.workload(4)
+    s = S.activate(wl.raw)
+    renderer.set_splat_data(s)
+    renderer.set_tile_size(16, 16)
+    renderer.learned_init(seed=4)
+    p, z, _ = renderer.learned_get()
+    rng = np.random.default_rng(4)
+    pick = np.sort(rng.choice(s.n, 3000, replace=False))
+    for first_dir in (0, 128):
+        _, _, gl = renderer.learned_grad(first_dir)
+        _, _, rgl = LO.step_grads(p, z[pick], s.f_dc[pick], s.sh_rest[pick], 3, first_dir, n_total=s.n)
+        e = rel_err(gl[pick], rgl)
+        say(f"  latent gradients, directions {first_dir}..{first_dir + 15}, 3000 splats: rel err {e:.3g}")
+        assert e < 1e-4
+    losses = renderer.learned_train(2, lr=3e-3, lr_latent=3e-2)
+    say(f"  2 epochs: loss {losses[0]:.5f} -> {losses[-1]:.5f}")
+    assert losses[-1] < losses[0]
+    p, z, _ = renderer.learned_get()
+    eye = np.array(wl.views[5][0], np.float32)
+    got = renderer.learned_colors(eye)
+    d = s.positions[pick, :3].astype(np.float32) - eye
+    d = d / np.sqrt((d * d).sum(axis=1, keepdims=True))
+    want, _ = LO.forward(p, z[pick], d, bf16=True)
+    assert np.abs(got[pick] - want).max() < 4e-3
+    views = [(cam_mod.view_camera(e, w, h, wl.fov_y), w, h) for e, w, h in wl.views]
+    renderer.rebin_batch(views)
+    exact = [renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16) for i in range(len(views))]
+    psnr, enabled = renderer.learned_validate(views, 45.0)
+    say(f"  PSNR vs exact SH over 32 views: {psnr:.2f} dB -> {'learned' if enabled else 'exact SH fallback'}")
+    assert enabled == (psnr >= 45.0)
+    renderer.rebin_batch(views, learned=True)
+    same = all(np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), exact[i])
+               for i in range(len(views)))
+    assert same == (not enabled)

@@ -37,8 +37,16 @@ __host__ __device__ inline uint64_t status_word(uint32_t epoch, uint64_t flag, u
 // node raises by kEpochStride on every replay; the argument is then a distinct
 // offset in [1, kEpochStride) per use of a status array within one replay, so
 // no status word left by the previous replay can match.
+// Every batch within GS3D_MAX_BATCH_VIEWS fits one stride: a view takes at
+// most 32/4 depth passes + 1 scan + 8 tile-id passes = 17 epochs, and even all
+// 1,024 views on one stream would take 17,408 < kEpochStride. The base wraps
+// after 2^30 / kEpochStride bumps; the host zeroes the status arrays and the
+// base before that, so a word of an earlier round of bases never matches.
 constexpr uint32_t kEpochMask = (1u << 30) - 1u;
-constexpr uint32_t kEpochStride = 1024;
+constexpr uint32_t kEpochStride = 32768;
+constexpr uint32_t kMaxEpochsPerView = 32 / 4 + 1 + 32 / 4;
+static_assert(GS3D_MAX_BATCH_VIEWS * kMaxEpochsPerView < kEpochStride, "replay epochs of a batch");
+constexpr uint32_t kEpochBumpsPerRound = (kEpochMask + 1u) / kEpochStride;
 __device__ __forceinline__ uint32_t resolve_epoch(uint32_t epoch, const uint32_t* epoch_base) {
     return epoch_base ? ((epoch + *epoch_base) & kEpochMask) : epoch;
 }
@@ -52,6 +60,20 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 // the lookback status words and the key-capacity slack.
 constexpr int kSortTile = 4096;
 constexpr int kMaxPasses = 16;
+
+// Radix scatter destination. The keys of digit d in a tile are at tile
+// positions [local_start, local_start + c); they go to global_start + excl +
+// (j - local_start) for position j, where global_start is the digit's start
+// over all keys and excl the count in the preceding tiles. The per-digit delta
+// global_start + excl - local_start is kept in u32 and may wrap below zero;
+// delta + j modulo 2^32 is then the exact index, which is below the key count
+// (at most 2^32 - 1), so no index is negative or truncated for any K.
+__host__ __device__ inline uint32_t scatter_delta(uint32_t global_start, uint32_t excl, uint32_t local_start) {
+    return global_start + excl - local_start;
+}
+__host__ __device__ inline size_t scatter_index(uint32_t delta, uint32_t j) {
+    return (size_t)(uint32_t)(delta + j);
+}
 
 // Counters buffer slots (u32), cleared at the start of every rebin.
 constexpr int kCtrScanTicket = 0;

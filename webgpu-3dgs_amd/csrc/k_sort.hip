@@ -66,8 +66,11 @@ constexpr int kLookbackGroup = 8;
 
 // THREADS x ITEMS keys per tile. LDS: the staging buffer (one KeyT per key;
 // values reuse it as u32), the per-wavefront digit counts, which the digit byte
-// of every sorted position later overwrites, and one signed per-digit delta:
-// global output index = delta[d] + position in the tile.
+// of every sorted position later overwrites, and one per-digit delta: global
+// output index = delta[d] + position in the tile, both u32 and added modulo
+// 2^32 (scatter_index). The delta alone may wrap below zero; the sum is the
+// true index, which is below n <= 2^32 - 1, so key counts of 2^31 and more
+// index correctly.
 // AUX: a third 32-bit array moves with the keys. kAuxGather: aux_out[position]
 // = aux_in[value] (a gather through the sorted values, overlapped with the key
 // and value write-out). kAuxCarry: aux_in is read at the key's own index, with
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     constexpr int kShared = kCountBytes > TILE ? kCountBytes : TILE;
     __shared__ KeyT s_stage[TILE];
     __shared__ __align__(16) uint8_t s_shared[kShared];
-    __shared__ int32_t s_delta[RADIX];
+    __shared__ uint32_t s_delta[RADIX];
     __shared__ uint32_t s_tmp[WAVES];
     __shared__ uint32_t s_part;
     uint32_t (*s_wave_hist)[RADIX] = reinterpret_cast<uint32_t (*)[RADIX]>(s_shared);
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         if (part != 0)
             store_status(&status[(size_t)part * RADIX + tid],
                          status_word(epoch, kFlagPrefix, excl + tile_count));
-        s_delta[tid] = (int32_t)(global_start + excl) - (int32_t)local_start;
+        s_delta[tid] = scatter_delta(global_start, excl, local_start);
     }
     __syncthreads();
 
@@ -233,7 +236,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
-        if (j < tile_n) kout[s_delta[s_digit[j]] + (int32_t)j] = s_stage[j];
+        if (j < tile_n) kout[scatter_index(s_delta[s_digit[j]], j)] = s_stage[j];
     }
     __syncthreads();
     // values: same permutation, reusing the staging memory
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
-        if (j < tile_n) vout[s_delta[s_digit[j]] + (int32_t)j] = s_vals[j];
+        if (j < tile_n) vout[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
     }
     if constexpr (AUX != kAuxNone) {
         __syncthreads();
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const uint32_t j = (uint32_t)(it * THREADS + tid);
-            if (j < tile_n) aux_out[s_delta[s_digit[j]] + (int32_t)j] = s_vals[j];
+            if (j < tile_n) aux_out[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
         }
     }
 }

@@ -571,6 +571,10 @@ int settle(gs3d_ctx* c) {
 constexpr int kBatchBranches = 4;
 // a batch view's record per splat: the first three float4s of a gs3d_projected_splat
 constexpr size_t kBatchRecordBytes = 48;
+// Per-view stride (elements) of the batch's depth-key and tile-rectangle
+// arrays: a multiple of 4, so every view's slice (and the second packed
+// rectangle array inside it) starts 16-B aligned for the scan's uint4 loads.
+inline size_t batch_view_stride(uint32_t n) { return ((size_t)n + 3u) & ~size_t(3); }
 
 struct BatchView {
     gs3d_camera cam{};
@@ -599,6 +603,7 @@ struct BatchState {
     uint64_t replays = 0;
     bool learned = false;  // the captured graph computes learned colours
     uint64_t splat_gen = 0;  // the splats the last batch projected
+    uint32_t epoch_bumps = 0;  // launches since the replay base was last zeroed
 };
 
 void drop_graph(BatchState* b) {
@@ -665,8 +670,8 @@ int batch_buffers(gs3d_ctx* c, bool* moved) {
     std::vector<std::pair<std::string, size_t>> want = {
         {"batch_views", V * sizeof(ViewParams)},
         {"batch_projected", V * (size_t)n * kBatchRecordBytes},
-        {"batch_depth_keys", V * (size_t)n * 4},
-        {"batch_rects", V * (size_t)n * 8},
+        {"batch_depth_keys", V * batch_view_stride(n) * 4},
+        {"batch_rects", V * batch_view_stride(n) * 8},
         {"batch_counters", V * kCtrTotalWords * 4},
         {"batch_tile_offsets", to * 4},
         {"batch_images", img * 2},
@@ -716,6 +721,21 @@ int batch_buffers(gs3d_ctx* c, bool* moved) {
         b->pinned_words = (uint32_t)V;
         *moved = true;
     }
+    return GS3D_OK;
+}
+
+// Zero the replay base and every branch's lookback status words (on the
+// context's stream, outside any capture; the branches are idle between calls).
+int batch_reset_epochs(gs3d_ctx* c) {
+    BatchState* b = c->batch;
+    HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, "batch_epoch"), 0u, 1, c->stream));
+    for (int k = 0; k < kBatchBranches; k++) {
+        for (const char* what : {"scan_status", "radix_status"}) {
+            const DevBuf& d = c->bufs[branch_name(k, what)];
+            if (d.ptr) HIP_TRY(c, launch_fill_u32(static_cast<uint32_t*>(d.ptr), 0u, d.cap / 4, c->stream));
+        }
+    }
+    b->epoch_bumps = 0;
     return GS3D_OK;
 }
 
@@ -798,7 +818,8 @@ int batch_enqueue(gs3d_ctx* c) {
         uint32_t* hist_depth = ctr + kCtrHist;
         uint32_t* hist_tile = ctr + kCtrHist + depth_passes * radix;
         uint32_t* hist_partial = buf<uint32_t>(c, branch_name(k, "hist_partial").c_str());
-        uint32_t* dka = dk + (size_t)vi * n;
+        const size_t ns = batch_view_stride(n);
+        uint32_t* dka = dk + (size_t)vi * ns;
         uint32_t* dkb = buf<uint32_t>(c, branch_name(k, "depth_keys_b").c_str());
         uint32_t* ida = buf<uint32_t>(c, branch_name(k, "order_a").c_str());
         uint32_t* idb = buf<uint32_t>(c, branch_name(k, "order_b").c_str());
@@ -816,15 +837,15 @@ int batch_enqueue(gs3d_ctx* c) {
         bool order_in_b = false;
         // packed rectangles (as set up for k_preprocess_views) move with the depth keys
         const bool packed = v.tiles_x <= 256u && v.tiles_y <= 256u;
-        uint32_t* r8a = packed ? reinterpret_cast<uint32_t*>(rects + (size_t)vi * n) : nullptr;
-        uint32_t* r8b = packed ? r8a + n : nullptr;
+        uint32_t* r8a = packed ? reinterpret_cast<uint32_t*>(rects + (size_t)vi * ns) : nullptr;
+        uint32_t* r8b = packed ? r8a + ns : nullptr;
         GS3D_TRY(batch_passes(c, s, dka, ida, dkb, idb, 4, true, ctr + kCtrSplatCount, n, shifts, depth_passes,
                               hist_depth, ctr + kCtrSortTicket, radix_status, epoch_base, &epoch_ctr[k],
                               &order_in_b, 32, c->sort_key_tile32, r8a, r8b));
         ScanRectsArgs sa{};
         sa.order = order_in_b ? idb : ida;
         sa.depth_keys = order_in_b ? dkb : dka;
-        sa.rects = packed ? nullptr : rects + (size_t)vi * n;
+        sa.rects = packed ? nullptr : rects + (size_t)vi * ns;
         sa.rects8 = packed ? (order_in_b ? r8b : r8a) : nullptr;
         sa.recs = recs;
         sa.ref_counter = ctr + kCtrRefCounter;
@@ -863,8 +884,8 @@ int batch_enqueue(gs3d_ctx* c) {
                                             v.tiles_y, v.W, v.H, c->raster_ppl, images + v.img_off, s));
     }
     for (int k = 0; k < kBatchBranches; k++) {
-        if (epoch_ctr[k] >= kEpochStride)
-            return fail(c, GS3D_E_INVALID_ARG, "too many views on one branch for the replay epochs");
+        if (epoch_ctr[k] >= kEpochStride)  // rebin_batch checks this before anything is queued
+            return fail(c, GS3D_E_STATE, "replay epochs of branch %d exceed the stride", k);
         HIP_TRY(c, hipEventRecord(b->join[k], b->streams[k]));
         HIP_TRY(c, hipStreamWaitEvent(c->stream, b->join[k], 0));
     }
@@ -923,6 +944,17 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
         b->views[i].branch = best;
         load[best] += (uint64_t)b->views[i].W * b->views[i].H;
     }
+    // replay epochs each branch's launches take (a scan and the passes per view);
+    // checked before anything is queued
+    uint64_t epochs_needed[kBatchBranches] = {};
+    for (const BatchView& v : b->views) {
+        uint32_t shifts[kMaxPasses];
+        epochs_needed[v.branch] += 1u + batch_plan(v.tiles_x * v.tiles_y, c->digit_bits, shifts);
+    }
+    for (int k = 0; k < kBatchBranches; k++)
+        if (epochs_needed[k] >= kEpochStride)
+            return fail(c, GS3D_E_INVALID_ARG, "batch needs %llu replay epochs on one stream (at most %u)",
+                        (unsigned long long)epochs_needed[k], kEpochStride - 1u);
     const bool learned = (flags & GS3D_BATCH_LEARNED) != 0 && c->learned_init && c->learned_n == c->n &&
                          (c->learned_enabled || c->learned_force);
     const bool same = b->captured && shape == b->shape && b->n == c->n && b->deg == c->sh_degree &&
@@ -961,12 +993,13 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
             vp[i].inv_tile_x = (float)(1.0 / (double)c->tile_x);
             vp[i].inv_tile_y = (float)(1.0 / (double)c->tile_y);
             vp[i].out = projected + (size_t)i * c->n * 3;
-            vp[i].depth_keys = dk + (size_t)i * c->n;
+            const size_t ns = batch_view_stride(c->n);
+            vp[i].depth_keys = dk + (size_t)i * ns;
             // a view's 8 B per splat of rectangle memory holds, when packed, the two
             // 4-B arrays its depth passes carry the rectangles between
             const bool packed = b->views[i].tiles_x <= 256u && b->views[i].tiles_y <= 256u;
-            vp[i].rects = packed ? nullptr : rects + (size_t)i * c->n;
-            vp[i].rects8 = packed ? reinterpret_cast<uint32_t*>(rects + (size_t)i * c->n) : nullptr;
+            vp[i].rects = packed ? nullptr : rects + (size_t)i * ns;
+            vp[i].rects8 = packed ? reinterpret_cast<uint32_t*>(rects + (size_t)i * ns) : nullptr;
             vp[i].counters = counters + (size_t)i * kCtrTotalWords;
         }
         HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_views"), vp.data(), vp.size() * sizeof(ViewParams),
@@ -979,6 +1012,11 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
             HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_eyes"), eyes.data(), eyes.size() * 4,
                                       hipMemcpyHostToDevice, c->stream));
         }
+        if (b->epoch_bumps + 1u >= kEpochBumpsPerRound) {
+            // the replay base would wrap: start a new round of bases from zeroed status words
+            GS3D_TRY(batch_reset_epochs(c));
+        }
+        b->epoch_bumps++;
         if (eager) {
             GS3D_TRY(batch_enqueue(c));
         } else {
