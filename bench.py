@@ -79,9 +79,64 @@ def load_traffic(workload: str, kernel: str):
         return None
 
 
-def cpu_baseline(cam, splats, W, H, tile, seconds):
+def host_cpu() -> dict:
+    """CPU model and core counts of the host this runs on."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "host_cores_total": os.cpu_count() or 1, "host_cores_usable": usable}
+
+
+def probe_cpu_webgpu() -> dict:
+    """Look (install nothing) for a WebGPU implementation that could run the
+    reference's WGSL on the host CPU: a Dawn or wgpu-native library, the
+    `webgpu` Node package, or a CPU Vulkan driver (lavapipe / SwiftShader) for
+    Dawn to sit on."""
+    import glob
+    import shutil
+    import subprocess
+
+    found = {}
+    pats = {"wgpu_native": ["/usr/lib*/**/libwgpu_native*", "/usr/local/lib*/**/libwgpu_native*",
+                            "/opt/**/libwgpu_native*"],
+            "dawn": ["/usr/lib*/**/libdawn*", "/usr/local/lib*/**/libdawn*", "/opt/**/libwebgpu_dawn*"],
+            "node_webgpu": ["/usr/lib/node_modules/webgpu", "/usr/local/lib/node_modules/webgpu",
+                            "/root/node_modules/webgpu"],
+            "vulkan_cpu_icd": ["/usr/share/vulkan/icd.d/*lvp*", "/usr/share/vulkan/icd.d/*swiftshader*",
+                               "/etc/vulkan/icd.d/*lvp*", "/etc/vulkan/icd.d/*swiftshader*"]}
+    for key, globs in pats.items():
+        hits = []
+        for g in globs:
+            hits += glob.glob(g, recursive=True)[:4]
+        found[key] = sorted(set(hits))[:4]
+    try:
+        ld = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
+        found["ldconfig"] = sorted({l.split()[0] for l in ld.splitlines()
+                                    if any(x in l for x in ("wgpu", "dawn", "vulkan"))})[:8]
+    except (OSError, subprocess.SubprocessError):
+        found["ldconfig"] = []
+    node = shutil.which("node")
+    found["node"] = subprocess.run([node, "--version"], capture_output=True, text=True,
+                                   timeout=10).stdout.strip() if node else None
+    usable = bool(found["wgpu_native"] or found["dawn"] or found["node_webgpu"])
+    found["usable_cpu_adapter"] = usable and bool(found["vulkan_cpu_icd"])
+    return found
+
+
+def cpu_baseline(cam, splats, W, H, tile, seconds, gpu_image=None):
     """The oracle's fp32 pipeline (a port of the reference WGSL passes) on the
-    host cores, whole frames of the same workload, until `seconds` of work."""
+    host cores, whole frames of the same workload, until `seconds` of work.
+    With gpu_image (H x W x 4), also the GPU frame's max-abs error against the
+    fp32 oracle frame and the fp64 render."""
     sys.path.insert(0, str(REPO / "tests"))
     import oracle_lib as O
 
@@ -89,14 +144,21 @@ def cpu_baseline(cam, splats, W, H, tile, seconds):
     O.set_threads(threads)
     frames, t0, visited = 0, time.perf_counter(), 0
     while True:
-        _, k, visited, blended = O.render(cam, splats, W, H, tile, "f32")
+        img, k, visited, blended = O.render(cam, splats, W, H, tile, "f32")
         frames += 1
         el = time.perf_counter() - t0
         if el >= seconds or frames >= 500:
             break
-    return {"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} whole frames of the same workload (oracle fp32 pipeline, "
-                      f"{threads} OpenMP threads, {el:.1f} s)"}, visited, blended
+    out = {"value": frames / el, "unit": "frames/s", "cores": threads, "kind": "port",
+           "sample": f"{frames} whole frames of the same workload (oracle fp32 pipeline, "
+                     f"{threads} OpenMP threads, {el:.1f} s)", **host_cpu()}
+    err = None
+    if gpu_image is not None:
+        f64, k64, _, _ = O.render(cam, splats, W, H, tile, "f64")
+        err = {"vs_oracle_f32": float(np.abs(gpu_image - img).max()),
+               "vs_render_f64": float(np.abs(gpu_image - f64).max()),
+               "oracle_keys": int(k), "f64_keys": int(k64), "tolerance": 1.0 / 255.0}
+    return out, visited, blended, err
 
 
 def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
@@ -322,12 +384,47 @@ def main() -> int:
     algo = algorithmic_bytes(splats.n, k, T, W * H, depth_passes, passes, splats.sh_degree,
                              packed=tiles_x <= 256 and tiles_y <= 256)
 
+    # Moving camera: the headline frame is one fixed camera, whose splat data
+    # (64 MB of vec4 + 192 MB of SH at 1M SH3) could stay in the 256 MB
+    # last-level cache between frames. The 8 orbit positions (azimuth k * 45
+    # deg at the same radius and elevation) are timed each on its own, and
+    # cycled frame by frame, so every frame renders a different view.
+    orbit = []
+    img_headline = r.read_image(W, H)
+    if wl.orbit_radius:
+        cams = [camera.view_camera(scene.orbit_eye(wl.orbit_radius, 45.0 * j, 0.0), W, H, wl.fov_y)
+                for j in range(8)]
+        reps = max(8, args.steps // 8)
+        for j, cj in enumerate(cams):
+            for _ in range(3):
+                r.rebin(cj, W, H)
+            r.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r.rebin(cj, W, H)
+            r.sync()
+            orbit.append({"azimuth_deg": 45 * j, "ms": round((time.perf_counter() - t0) * 1e3 / reps, 4),
+                          "keys": r.ref_count()})
+        r.sync()
+        t0 = time.perf_counter()
+        cyc = 8 * reps
+        for i in range(cyc):
+            r.rebin(cams[i % 8], W, H)
+        r.sync()
+        cycle_ms = (time.perf_counter() - t0) * 1e3 / cyc
+        log("[bench] orbit: " + ", ".join(f"{o['azimuth_deg']}deg {o['ms']:.3f} ms" for o in orbit)
+            + f"; cycling all 8 every frame {cycle_ms:.3f} ms/frame")
+
     cpu = None
     visited = None
+    image_error = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle port) ...")
-        cpu, visited, _ = cpu_baseline(cu, splats, W, H, args.tile, args.cpu_seconds)
-        log(f"[bench] cpu baseline {cpu['value']:.3f} frames/s on {cpu['cores']} cores")
+        cpu, visited, _, image_error = cpu_baseline(cu, splats, W, H, args.tile, args.cpu_seconds,
+                                                    gpu_image=img_headline)
+        log(f"[bench] cpu baseline {cpu['value']:.3f} frames/s on {cpu['cores']} of {cpu['host_cores_total']} "
+            f"cores ({cpu['cpu_model']}); image max abs vs oracle f32 {image_error['vs_oracle_f32']:.3g}, "
+            f"vs fp64 {image_error['vs_render_f64']:.3g}")
 
     per_stage = {}
     for name, b in algo.items():
@@ -354,7 +451,16 @@ def main() -> int:
         gbs = ref_bytes / (sort_us * 1e-6) / 1e9
         per_stage["sort-reference-layout"] = {"us": round(sort_us, 2), "algo_bytes": int(ref_bytes),
                                               "passes": ref_passes, "GB/s": round(gbs, 1),
-                                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+                                              "reference_layout_equivalent_frac": round(gbs / HBM_PEAK_GBS, 4),
+                                              "note": "the reference's sort bytes over this sort's time: "
+                                                      "not this sort's own traffic, not a roofline fraction"}
+        own = algo["radix-sort-splats"] + algo["radix-sort"]
+        own_gbs = own / (sort_us * 1e-6) / 1e9
+        sort_roofline = {"bound": "hbm", "achieved": round(own_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(own_gbs / HBM_PEAK_GBS, 4), "algo_bytes": int(own),
+                         "us": round(sort_us, 2),
+                         "depth_passes_frac": per_stage["radix-sort-splats"]["hbm_frac"],
+                         "tile_passes_frac": per_stage["radix-sort"]["hbm_frac"]}
     d = per_stage[dominant]
     d["us_timed_region"] = round(dominant_us, 2)
     d["GB/s"] = round(algo[dominant] / (dominant_us * 1e-6) / 1e9, 1)
@@ -393,7 +499,14 @@ def main() -> int:
         "dominant_sampled_frames": frames,
         "per_stage": per_stage,
         "roofline": roofline,
+        "sort_roofline": sort_roofline if sort_us > 0 else None,
         "cpu_baseline": cpu,
+        "image_error": image_error,
+        "orbit": {"positions": orbit,
+                  "mean_ms": round(sum(o["ms"] for o in orbit) / len(orbit), 4) if orbit else None,
+                  "cycling_ms_per_frame": round(cycle_ms, 4) if orbit else None,
+                  "cycling_frames_per_s": round(1000.0 / cycle_ms, 2) if orbit else None},
+        "cpu_webgpu_probe": probe_cpu_webgpu(),
     }
     if args.composite > 0:
         # The display step (not part of frames/s): premultiplied over onto the

@@ -173,6 +173,7 @@ class Workload:
     views: list  # list of (eye xyz, width, height)
     tile: int = 16
     fov_y: float = math.pi / 3.0
+    orbit_radius: float = 0.0  # the orbit the single view sits on (0: not an orbit camera)
 
 
 def orbit_eye(radius: float, azimuth_deg: float, elevation_deg: float) -> tuple:
@@ -198,10 +199,12 @@ def workload(index: int, n: int | None = None, views: int | None = None) -> Work
         return Workload("uniform-100k-sh0-720p", raw, [((0.0, 0.0, 3.0), 1280, 720)])
     if index == 1:
         raw = clustered(1, n or 1_000_000, 64)
-        return Workload("clusters-1M-sh3-1080p", raw, [(orbit_eye(8.0, 0.0, 0.0), 1920, 1080)])
+        return Workload("clusters-1M-sh3-1080p", raw, [(orbit_eye(8.0, 0.0, 0.0), 1920, 1080)],
+                        orbit_radius=8.0)
     if index == 2:
         raw = clustered(2, n or 6_000_000, 256, -4.0, 0.7)
-        return Workload("clusters-6M-sh3-2160p", raw, [(orbit_eye(8.0, 0.0, 0.0), 3840, 2160)])
+        return Workload("clusters-6M-sh3-2160p", raw, [(orbit_eye(8.0, 0.0, 0.0), 3840, 2160)],
+                        orbit_radius=8.0)
     if index == 3:
         raw = clustered(3, n or 2_000_000, 128)
         return Workload("clusters-2M-sh3-64views", raw, orbit_views(3, views or 64))
