@@ -397,6 +397,22 @@ int gs3d_stage_sort(gs3d_ctx* ctx, const uint64_t* keys_in, const uint32_t* valu
                     uint32_t n, uint32_t digit_bits,
                     uint64_t* keys_out, uint32_t* values_out);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the sort of gs3d_stage_sort (same order, same digit
+ * widths) over caller DEVICE buffers, queued on the context's stream, with no
+ * allocation, free, copy or synchronisation. Everything beyond the inputs and
+ * outputs (histograms, lookback words, the second ping-pong buffer) lives in
+ * a static device array of about 58 MiB that each GPU the library runs on
+ * holds until the process ends. n above GS3D_SORT_NOALLOC_MAX_PAIRS is
+ * GS3D_E_INVALID_ARG. keys_out/values_out may be keys_in/values_in. The
+ * results are ready after gs3d_sync or any later blocking call on the context.
+ * Every call on one GPU shares that GPU's array, from every context: calls
+ * that overlap on one GPU (two contexts' streams, or two threads) race. */
+#define GS3D_SORT_NOALLOC_MAX_PAIRS (1u << 22)
+int gs3d_sort_pairs_noalloc(gs3d_ctx* ctx, const uint64_t* keys_in, const uint32_t* values_in,
+                            uint32_t n, uint32_t digit_bits, uint64_t* keys_out, uint32_t* values_out);
+
 /* identify_tile_ranges.wgsl:30-52 over caller sorted keys; tile_offsets holds
  * tile_count+1 entries, zero-initialised inside the call. */
 int gs3d_stage_identify(gs3d_ctx* ctx, const uint64_t* sorted_keys, uint32_t n,

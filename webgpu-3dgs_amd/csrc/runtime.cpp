@@ -22,6 +22,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1153,6 +1154,68 @@ __attribute__((visibility("hidden"))) int gs3d_internal_fail(gs3d_ctx* c, int co
 }
 __attribute__((visibility("hidden"))) int gs3d_internal_set_device(gs3d_ctx* c) { return set_device(c); }
 
+// ---- allocation-free sort: static scratch --------------------------------------
+//
+// gs3d_sort_pairs_noalloc keeps everything beyond the caller's inputs and
+// outputs in `scratch`, a file-scope device array each GPU holds from module
+// load to process end. Host code never passes the array's name to a kernel
+// (in host code it is a stand-in, not a device address): gs3d_create looks up
+// its address on each device once, with hipGetSymbolAddress, and the entry
+// point uses that per-device address. Layout, 256-B aligned sections:
+//   control   64 u32: [0] = n (the pass kernels read the key count on the
+//             device), [1 + p] = pass p's workgroup ticket
+//   hist      [kMaxPasses][256] u32 digit histograms of the input
+//   partial   the histogram kernel's per-workgroup rows (<= 1,024 rows of at
+//             most 8 passes x 256 or 16 x 16 digits)
+//   status    lookback words of one pass over the maximum key count (u64,
+//             tagged with a process-wide epoch, never cleared)
+//   keys      the ping-pong side of the passes that is not the caller's output:
+//   values    GS3D_SORT_NOALLOC_MAX_PAIRS u64 keys and u32 values
+namespace {
+
+constexpr size_t na_align(size_t x) { return (x + 255u) & ~size_t(255); }
+constexpr size_t kNaMaxPairs = GS3D_SORT_NOALLOC_MAX_PAIRS;
+constexpr size_t kNaCtlWords = 64;
+constexpr size_t kNaHistOff = na_align(kNaCtlWords * 4);
+constexpr size_t kNaPartialOff = na_align(kNaHistOff + (size_t)kMaxPasses * 256 * 4);
+constexpr size_t kNaPartialWords = 1024u * 2048u;
+constexpr size_t kNaStatusWords = ((kNaMaxPairs + kSortTile - 1) / kSortTile + 1) * 256;
+constexpr size_t kNaStatusOff = na_align(kNaPartialOff + kNaPartialWords * 4);
+constexpr size_t kNaKeysOff = na_align(kNaStatusOff + kNaStatusWords * 8);
+constexpr size_t kNaValuesOff = na_align(kNaKeysOff + kNaMaxPairs * 8);
+constexpr size_t kNaBytes = na_align(kNaValuesOff + kNaMaxPairs * 4);
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+uint8_t* g_na_scratch[kNaMaxDevices] = {};  // device address of `scratch` on each device
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+// With `device` current: find this device's copy of `scratch` (once per device).
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (g_na_scratch[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_na_scratch[device] = static_cast<uint8_t*>(ptr);
+    return hipSuccess;
+}
+
+uint8_t* noalloc_scratch(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    return g_na_scratch[device];
+}
+
+}  // namespace
+
 extern "C" {
 
 int gs3d_abi_version(void) { return GS3D_ABI_VERSION; }
@@ -1182,7 +1245,8 @@ int gs3d_create(int device, gs3d_ctx** out) {
     c->limits[5] = (uint32_t)prop.sharedMemPerBlock;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&c->pinned, 64, hipHostMallocMapped) != hipSuccess) {
+        hipHostMalloc((void**)&c->pinned, 64, hipHostMallocMapped) != hipSuccess ||
+        noalloc_scratch_init(device) != hipSuccess) {
         gs3d_destroy(c);
         return GS3D_E_HIP;
     }
@@ -2210,6 +2274,53 @@ int gs3d_stage_composite(gs3d_ctx* c, const uint16_t* target_f16, const uint32_t
     if (debug && debug_out)
         HIP_TRY(c, hipMemcpyAsync(debug_out, dd, px * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return GS3D_OK;
+}
+
+int gs3d_sort_pairs_noalloc(gs3d_ctx* c, const uint64_t* keys_in, const uint32_t* values_in, uint32_t n,
+                            uint32_t digit_bits, uint64_t* keys_out, uint32_t* values_out) {
+    if (!c) return GS3D_E_INVALID_ARG;
+    if (n && (!keys_in || !values_in || !keys_out || !values_out))
+        return fail(c, GS3D_E_INVALID_ARG, "null device pointer");
+    if (digit_bits != 4 && digit_bits != 8)
+        return fail(c, GS3D_E_INVALID_ARG, "digit width %u not 4 or 8", digit_bits);
+    if (n > GS3D_SORT_NOALLOC_MAX_PAIRS)
+        return fail(c, GS3D_E_INVALID_ARG,
+                    "%u pairs exceed the %u the allocation-free sort's static scratch holds "
+                    "(gs3d_stage_sort has no such limit)", n, GS3D_SORT_NOALLOC_MAX_PAIRS);
+    GS3D_TRY(set_device(c));
+    uint8_t* base = noalloc_scratch(c->device);
+    if (!base) return fail(c, GS3D_E_STATE, "no scratch address for device %d", c->device);
+    if (n == 0) return GS3D_OK;
+    uint32_t* ctl = reinterpret_cast<uint32_t*>(base);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(base + kNaHistOff);
+    uint32_t* partial = reinterpret_cast<uint32_t*>(base + kNaPartialOff);
+    uint64_t* status = reinterpret_cast<uint64_t*>(base + kNaStatusOff);
+    uint64_t* tmp_keys = reinterpret_cast<uint64_t*>(base + kNaKeysOff);
+    uint32_t* tmp_values = reinterpret_cast<uint32_t*>(base + kNaValuesOff);
+    const uint32_t passes = 64u / digit_bits, radix = 1u << digit_bits;
+    // the scratch keeps the last call's contents: control words and histograms
+    // start at zero (fill kernels, no memset nodes), then the key count
+    HIP_TRY(c, launch_fill_u32(ctl, 0u, kNaCtlWords, c->stream));
+    HIP_TRY(c, launch_fill_u32(ctl, n, 1, c->stream));
+    HIP_TRY(c, launch_fill_u32(hist, 0u, (size_t)passes * radix, c->stream));
+    HIP_TRY(c, launch_key_histogram(keys_in, 8, ctl, n, hist, partial, digit_bits, passes, 0, c->stream));
+    // Ping-pong between the caller's output and the scratch side so that the
+    // last pass lands in the output: pass p writes the output when
+    // passes - 1 - p is even. The passes are even in number, so pass 0 writes
+    // the scratch side and the input may be the output buffer itself.
+    const void* src_k = keys_in;
+    const uint32_t* src_v = values_in;
+    for (uint32_t p = 0; p < passes; p++) {
+        const bool to_out = ((passes - 1u - p) % 2u) == 0u;
+        void* dst_k = to_out ? static_cast<void*>(keys_out) : static_cast<void*>(tmp_keys);
+        uint32_t* dst_v = to_out ? values_out : tmp_values;
+        HIP_TRY(c, launch_radix_pass(src_k, src_v, dst_k, dst_v, 8, false, ctl, n, hist + p * radix, status,
+                                     ctl + 1 + p, p * digit_bits, digit_bits, next_epoch(c), nullptr, nullptr,
+                                     c->stream, nullptr, c->sort_key_tile, digit_bits, false));
+        src_k = dst_k;
+        src_v = dst_v;
+    }
     return GS3D_OK;
 }
 

@@ -91,6 +91,7 @@ _PROTOS = {
     "gs3d_stage_scan": (I32, [P, P, U32, P, C.POINTER(U32)]),
     "gs3d_stage_emit": (I32, [P, P, P, U32, U32, P, P]),
     "gs3d_stage_sort": (I32, [P, P, P, U32, U32, P, P]),
+    "gs3d_sort_pairs_noalloc": (I32, [P, P, P, U32, U32, P, P]),
     "gs3d_stage_identify": (I32, [P, P, U32, U32, P]),
     "gs3d_stage_rasterize": (I32, [P, P, U32, P, U32, P, U32, U32, U32, U32, P]),
     "gs3d_set_debug_mode": (I32, [P, U32, U32]),
