@@ -192,7 +192,16 @@ def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
     per = timed(lambda: r.rebin_batch(views), steps)
     ks = r.batch_ref_counts()
     out.update({"value": round(len(views) / per, 2), "unit": "views/s", "ms_per_batch": round(per * 1e3, 3),
-                "mode": "hipGraph replay", "keys_total": int(ks.sum()), "keys_max_view": int(ks.max())})
+                "mode": "hipGraph replay, one persistent rasterizer over every view's tiles",
+                "keys_total": int(ks.sum()), "keys_max_view": int(ks.max())})
+    # the same batch with one raster launch per view on its binning stream
+    r.set_kernel_variant("rasterize-tiles-batch", "GROUPS", 0)
+    for _ in range(warmup):
+        r.rebin_batch(views)
+    per_v = timed(lambda: r.rebin_batch(views), steps)
+    out["per_view_raster_views_per_s"] = round(len(views) / per_v, 2)
+    r.set_kernel_variant("rasterize-tiles-batch", "GROUPS", 1)
+    r.rebin_batch(views)
     r.rebin_batch(views, eager=True)
     per_e = timed(lambda: r.rebin_batch(views, eager=True), max(1, steps // 2))
     out["eager_views_per_s"] = round(len(views) / per_e, 2)
@@ -207,16 +216,21 @@ def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
         from gs3d import autotune
 
         best, results = autotune.tune_batch(r, views, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8, 4),
-                                            warmup=1, reps=3)
+                                            raster_groups=(1, 0), warmup=1, reps=3)
         out["autotune"] = {"best": {"tile": best.tile, "pixels_per_lane": best.pixels_per_lane,
-                                    "radix_bits": best.radix_bits, "views_per_s": round(best.views_per_s, 2)},
-                           "candidates": [[c.tile, c.pixels_per_lane, c.radix_bits, round(c.views_per_s, 1)]
-                                          for c in results]}
+                                    "radix_bits": best.radix_bits, "raster_groups": best.raster_groups,
+                                    "views_per_s": round(best.views_per_s, 2)},
+                           "candidates_note": "tile, pixels per lane, digit bits, raster groups (1: one persistent "
+                                              "rasterizer, 0: a raster launch per view), views/s",
+                           "candidates": [[c.tile, c.pixels_per_lane, c.radix_bits, c.raster_groups,
+                                           round(c.views_per_s, 1)] for c in results]}
         per_t = timed(lambda: r.rebin_batch(views), steps)
         out["tuned_views_per_s"] = round(len(views) / per_t, 2)
         log(f"[bench] batch autotune: best tile {best.tile}, ppl {best.pixels_per_lane}, "
-            f"{best.radix_bits}-bit digits: {out['tuned_views_per_s']} views/s")
-    log(f"[bench] batched: {out['value']} views/s (graph), {out['eager_views_per_s']} eager, "
+            f"{best.radix_bits}-bit digits, raster groups {best.raster_groups}: "
+            f"{out['tuned_views_per_s']} views/s")
+    log(f"[bench] batched: {out['value']} views/s (graph, persistent raster), "
+        f"{out['per_view_raster_views_per_s']} with a raster launch per view, {out['eager_views_per_s']} eager, "
         f"{out['single_view_rebins_views_per_s']} one rebin per view; K total {out['keys_total']}")
     r.close()
     return out

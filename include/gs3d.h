@@ -255,7 +255,9 @@ int gs3d_rebin_batch(gs3d_ctx* ctx, const gs3d_view* views, uint32_t n_views, ui
 /* K (keys) of every view of the last batch. */
 int gs3d_batch_ref_counts(gs3d_ctx* ctx, uint64_t* out, uint32_t n);
 /* Per-view results of the last batch: GS3D_BUF_IMAGE_RGBA16F / _RGBA32F,
- * GS3D_BUF_TILE_OFFSETS, GS3D_BUF_PROJECTED_SPLATS, GS3D_BUF_REF_COUNTER.
+ * GS3D_BUF_TILE_OFFSETS, GS3D_BUF_PROJECTED_SPLATS, GS3D_BUF_REF_COUNTER,
+ * GS3D_BUF_SORT_VALUES and GS3D_BUF_SORT_KEYS (the keys rebuilt from the sorted
+ * ids, the tile ranges and the view's projected depths).
  * A batch keeps only what its rasterizer reads of the projected splats; the
  * N x gs3d_projected_splat of a view are projected again on read, so they
  * need the batch's splats: GS3D_E_STATE after a new gs3d_set_splats. */

@@ -38,6 +38,8 @@ def single_view(renderer, cam, w, h):
         "image": renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16),
         "tile_offsets": renderer.read(L.BUF_TILE_OFFSETS, np.uint32),
         "projected": renderer.read(L.BUF_PROJECTED_SPLATS, np.uint32),
+        "values": renderer.read(L.BUF_SORT_VALUES, np.uint32),
+        "keys": renderer.read(L.BUF_SORT_KEYS, np.uint64),
     }
 
 
@@ -49,11 +51,13 @@ def check_batch(renderer, views, eager=False):
         got.append({"k": int(ks[i]),
                     "image": renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16),
                     "tile_offsets": renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32),
-                    "projected": renderer.batch_read(i, L.BUF_PROJECTED_SPLATS, np.uint32)})
+                    "projected": renderer.batch_read(i, L.BUF_PROJECTED_SPLATS, np.uint32),
+                    "values": renderer.batch_read(i, L.BUF_SORT_VALUES, np.uint32),
+                    "keys": renderer.batch_read(i, L.BUF_SORT_KEYS, np.uint64)})
         assert int(renderer.batch_read(i, L.BUF_REF_COUNTER, np.uint32)[0]) == got[-1]["k"]
     for i, (cam, w, h) in enumerate(views):
         ref = single_view(renderer, cam, w, h)
-        for key in ("k", "tile_offsets", "projected", "image"):
+        for key in ("k", "tile_offsets", "projected", "image", "values", "keys"):
             assert np.array_equal(np.asarray(got[i][key]), np.asarray(ref[key])), (i, key)
     return got
 
@@ -165,7 +169,7 @@ def test_batch_errors(renderer, scene_sh3):
     renderer.rebin_batch(views_for(1, 2))
     size = __import__("ctypes").c_size_t()
     assert lib.gs3d_batch_read(renderer._h, 5, L.BUF_IMAGE_RGBA16F, None, 0, size) == -1
-    assert lib.gs3d_batch_read(renderer._h, 0, L.BUF_SORT_KEYS, None, 0, size) == -6
+    assert lib.gs3d_batch_read(renderer._h, 0, L.BUF_UNSORTED_KEYS, None, 0, size) == -6
 
 
 def test_batch_autotuner_picks_the_fastest_candidate(renderer, scene_sh3):
@@ -179,7 +183,9 @@ def test_batch_autotuner_picks_the_fastest_candidate(renderer, scene_sh3):
     try:
         best, results = autotune.tune_batch(renderer, views, tiles=(8, 16), ppls=(1, 2), radix_bits=(8, 4),
                                             warmup=1, reps=2)
-        assert len(results) == 6  # 8x8 tiles take only 1 pixel per lane
+        assert len(results) == 12  # 8x8 tiles take only 1 pixel per lane; two raster modes each
+        assert {r.raster_groups for r in results} == {0, 1}
+        assert renderer.get_kernel_variant("rasterize-tiles-batch", "GROUPS") == best.raster_groups
         assert best.views_per_s == max(r.views_per_s for r in results)
         assert renderer.get_tile_size() == (best.tile, best.tile)
         assert renderer.get_kernel_variant("radix-sort", "RADIX_BITS") == best.radix_bits
@@ -188,6 +194,23 @@ def test_batch_autotuner_picks_the_fastest_candidate(renderer, scene_sh3):
         renderer.set_tile_size(16, 16)
         renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 1)
         renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 8)
+        renderer.set_kernel_variant("rasterize-tiles-batch", "GROUPS", 1)
+
+
+@pytest.mark.parametrize("groups,wave_items", [(0, 0), (3, 0), (1, 1)])
+def test_batch_raster_modes(renderer, scene_sh3, groups, wave_items):
+    """The batch's other raster schedules give the same views: one raster launch per view
+    on its binning stream (GROUPS 0), three persistent launches over view groups, and
+    pixel-block work items (WAVE_ITEMS 1)."""
+    say(f"batch: raster groups {groups}, wave items {wave_items}")
+    renderer.set_splat_data(scene_sh3)
+    renderer.set_kernel_variant("rasterize-tiles-batch", "GROUPS", groups)
+    renderer.set_kernel_variant("rasterize-tiles-batch", "WAVE_ITEMS", wave_items)
+    try:
+        check_batch(renderer, views_for(12, 7))
+    finally:
+        renderer.set_kernel_variant("rasterize-tiles-batch", "GROUPS", 1)
+        renderer.set_kernel_variant("rasterize-tiles-batch", "WAVE_ITEMS", 0)
 
 
 def test_batch_all_culled_views(renderer, scene_sh3):
@@ -219,10 +242,11 @@ def test_batch_odd_splat_count(renderer):
 @pytest.mark.timeout(900)
 def test_workload_2m_sh3_64_views(renderer):
     """The batched workload itself: 2M SH3 splats, 64 seeded orbit views of uneven
-    resolution (640x360 / 1280x720 / 1920x1080) in one call. Every view: K equals the view's
-    counter, its tile ranges are consistent (monotone up to the last populated tile + 1, then
-    zero, ending at K); one view of each resolution against the oracle: K and tile ranges
-    bit-exact, image within 1/255."""
+    resolution (640x360 / 1280x720 / 1920x1080) in one call. Every view: K equals its
+    counter and its splats' tile counts, the sorted keys are sorted and stable, each key's
+    tile lies in its splat's rectangle, and the tile ranges are the closed form of the
+    sorted keys. One view of each resolution against the oracle: keys, values and tile
+    ranges bit-exact, image within 1/255."""
     say("batch: workload clusters-2M-sh3-64views")
     O.set_threads(16)
     wl = S.workload(3)
@@ -234,25 +258,33 @@ def test_workload_2m_sh3_64_views(renderer):
     ks = renderer.batch_ref_counts()
     say(f"  K total {int(ks.sum())}, largest view {int(ks.max())}")
     assert len(ks) == 64 and ks.min() > 0
-    for i, (_, w, h) in enumerate(views):
-        to = renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32)
-        assert to.size == (-(-w // 16)) * (-(-h // 16)) + 1
-        assert int(renderer.batch_read(i, L.BUF_REF_COUNTER, np.uint32)[0]) == int(ks[i])
-        nz = np.nonzero(to)[0]
-        last = int(nz[-1])
-        assert to[last] == ks[i]
-        assert np.all(np.diff(to[:last + 1].astype(np.int64)) >= 0)
     checked = set()
     for i, (cam, w, h) in enumerate(views):
+        tx, ty = -(-w // 16), -(-h // 16)
+        to = renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32)
+        keys = renderer.batch_read(i, L.BUF_SORT_KEYS, np.uint64)
+        vals = renderer.batch_read(i, L.BUF_SORT_VALUES, np.uint32)
+        rec = renderer.batch_read(i, L.BUF_PROJECTED_SPLATS, np.uint32).reshape(-1, 16)
+        assert int(renderer.batch_read(i, L.BUF_REF_COUNTER, np.uint32)[0]) == int(ks[i]) == keys.size == vals.size
+        counts = np.where(rec[:, 11] == 1, (rec[:, 14] - rec[:, 12] + 1) * (rec[:, 15] - rec[:, 13] + 1), 0)
+        assert np.array_equal(np.bincount(vals, minlength=s.n), counts)
+        same = keys[1:] == keys[:-1]
+        assert np.all(keys[1:] >= keys[:-1]) and np.all(vals[1:][same] > vals[:-1][same])
+        tile = (keys >> np.uint64(32)).astype(np.int64)
+        r = rec[vals]
+        assert np.all((tile % tx >= r[:, 12]) & (tile % tx <= r[:, 14]) &
+                      (tile // tx >= r[:, 13]) & (tile // tx <= r[:, 15]))
+        assert np.array_equal(to, O.identify(keys, tx * ty))
         if (w, h) in checked:
             continue
         checked.add((w, h))
         ref = O.pipeline_f32(cam, s, w, h, 16)
         assert int(ks[i]) == ref["k"], (i, w, h)
-        assert np.array_equal(renderer.batch_read(i, L.BUF_TILE_OFFSETS, np.uint32), ref["tile_offsets"])
+        assert np.array_equal(keys, ref["sorted_keys"]) and np.array_equal(vals, ref["sorted_values"])
+        assert np.array_equal(to, ref["tile_offsets"])
         img = renderer.batch_read(i, L.BUF_IMAGE_RGBA32F, np.float32).reshape(h, w, 4)
         err = float(np.abs(img - ref["image"]).max())
-        say(f"  view {i} {w}x{h}: K={ref['k']} image max abs vs oracle f32 {err:.3g}")
+        say(f"  view {i} {w}x{h}: K={ref['k']} keys, values, ranges bit-exact; image max abs vs oracle f32 {err:.3g}")
         assert err <= TOL_IMAGE
     assert len(checked) == 3
 

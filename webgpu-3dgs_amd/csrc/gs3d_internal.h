@@ -275,6 +275,36 @@ hipError_t launch_rasterize_records(const float4* records, uint32_t rec, const u
                                     uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
 
+// One view of the batch's persistent rasterizer (device memory).
+struct RasterView {
+    const float4* proj;            // the view's 48-B records (3 float4s per splat)
+    const uint32_t* values;        // its sorted splat ids
+    const uint32_t* tile_offsets;  // its tiles + 1 ranges
+    uint2* image;                  // its W*H rgba16f target
+    uint32_t tiles_x, tiles, width, height;
+};
+// Control words of the raster queue (zeroed before the build): bucket counts,
+// bucket cursors, the rasterizer's ticket.
+constexpr uint32_t kRasterQueueCounts = 0;
+constexpr uint32_t kRasterQueueCursor = 64;
+constexpr uint32_t kRasterQueueTicket = 128;
+constexpr uint32_t kRasterQueueWords = 192;
+// queue[0 .. sum of tiles) = (view, tile) items, longest tile lists first.
+// max_tiles: the largest view's tile count.
+hipError_t launch_raster_queue_build(const RasterView* views, uint32_t n_views, uint32_t max_tiles,
+                                     uint32_t* ctl, uint2* queue, hipStream_t s);
+// Every item of the queue, by a grid of `grid` workgroups (rasterize_queue_grid:
+// what the current device holds at once) that take items with a ticket
+// (ctl[kRasterQueueTicket]).
+// wg_per_cu > 0 caps the workgroups per CU (room for other streams' kernels).
+// wave_items: each wavefront takes one pixel block of a tile per ticket
+// instead of a workgroup taking the whole tile.
+hipError_t rasterize_queue_grid(uint32_t tile, uint32_t pixels_per_lane, uint32_t wg_per_cu, bool wave_items,
+                                uint32_t* grid);
+hipError_t launch_rasterize_queue(const RasterView* views, const uint2* queue, uint32_t n_items,
+                                  uint32_t* ctl, uint32_t tile, uint32_t pixels_per_lane, uint32_t grid,
+                                  bool wave_items, hipStream_t s);
+
 // ---- learned colour ---------------------------------------------------------------
 struct LearnedColorArgs {
     const float* params;     // GS3D_LEARNED_PARAMS fp32

@@ -160,22 +160,19 @@ struct alignas(16) Survivor {
 // broadcast) and blends branch-free: a splat that fails a rule is blended with
 // alpha 0, which leaves C and T bit-identical (T - T * 0 = T,
 // C + T * 0 * c = C). The all-saturated exit is checked once per 4 splats.
+// One wavefront's pixel block `wave` (0 .. kWaves-1) of `tile`, with that
+// wavefront's own LDS slots.
 template <int TW, int PPL>
-__global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __restrict__ proj,
-                                                             uint32_t rec,
-                                                             const uint32_t* __restrict__ values,
-                                                             const uint32_t* __restrict__ tile_offsets,
-                                                             uint32_t tiles_x, uint32_t width,
-                                                             uint32_t height, uint2* __restrict__ image) {
+__device__ __forceinline__ void raster_block(const float4* __restrict__ proj, uint32_t rec,
+                                             const uint32_t* __restrict__ values,
+                                             const uint32_t* __restrict__ tile_offsets, uint32_t tile,
+                                             int wave, uint32_t tiles_x, uint32_t width, uint32_t height,
+                                             uint2* __restrict__ image, Survivor* slots) {
     using Sh = RasterShape<TW, PPL>;
     constexpr float kLog2e = 1.4426950408889634f;
-    __shared__ Survivor s_slot[Sh::kWaves][kWave];
 
-    const uint32_t tile = blockIdx.x;
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int wave = tid / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
     const uint32_t bx0 = tx * TW + (wave % Sh::kBlocksX) * Sh::BW;
     const uint32_t by0 = ty * TW + (wave / Sh::kBlocksX) * Sh::BH;
     const uint32_t px = bx0 + (lane % Sh::BW);
@@ -230,7 +227,6 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __res
 
     const uint32_t start = tile_offsets[tile];
     const uint32_t end = tile_offsets[tile + 1];
-    Survivor* slots = s_slot[wave];
 
     // software pipeline: batch b+1 is gathered while batch b is composited
     float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
@@ -297,6 +293,145 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __res
             image[(size_t)py * width + px] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
         }
     }
+}
+
+// Every pixel block of `tile`, one per wavefront of the workgroup.
+template <int TW, int PPL>
+__device__ __forceinline__ void raster_tile(const float4* __restrict__ proj, uint32_t rec,
+                                            const uint32_t* __restrict__ values,
+                                            const uint32_t* __restrict__ tile_offsets, uint32_t tile,
+                                            uint32_t tiles_x, uint32_t width, uint32_t height,
+                                            uint2* __restrict__ image, Survivor (*s_slot)[kWave]) {
+    const int wave = threadIdx.x / kWave;
+    raster_block<TW, PPL>(proj, rec, values, tile_offsets, tile, wave, tiles_x, width, height, image,
+                          s_slot[wave]);
+}
+
+// One workgroup per tile (the single-view frame).
+template <int TW, int PPL>
+__global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __restrict__ proj,
+                                                             uint32_t rec,
+                                                             const uint32_t* __restrict__ values,
+                                                             const uint32_t* __restrict__ tile_offsets,
+                                                             uint32_t tiles_x, uint32_t width,
+                                                             uint32_t height, uint2* __restrict__ image) {
+    __shared__ Survivor s_slot[RasterShape<TW, PPL>::kWaves][kWave];
+    raster_tile<TW, PPL>(proj, rec, values, tile_offsets, blockIdx.x, tiles_x, width, height, image, s_slot);
+}
+
+// The batch's persistent rasterizer: a grid sized to what the GPU holds at
+// once, every workgroup taking (view, tile) items from `queue` in order with
+// a ticket until the queue is exhausted. Tickets are taken one item ahead
+// (double-buffered in LDS) so the atomic's latency hides behind a tile. No
+// workgroup waits on another, so any grid size drains the queue.
+// The wave-item form below takes pixel blocks instead of tiles: each
+// wavefront of a 256-thread workgroup takes its own tickets (item = tile item
+// * blocks per tile + block), so no wavefront waits at a barrier for the
+// slowest block of its tile.
+template <int TW, int PPL>
+__global__ __launch_bounds__(TW * TW / PPL) void k_rasterize_queue(const RasterView* __restrict__ views,
+                                                                   const uint2* __restrict__ queue,
+                                                                   uint32_t n_items, uint32_t* ticket) {
+    __shared__ Survivor s_slot[RasterShape<TW, PPL>::kWaves][kWave];
+    __shared__ uint32_t s_next[2];
+    if (threadIdx.x == 0) s_next[0] = atomicAdd(ticket, 1u);
+    __syncthreads();
+    for (uint32_t it = 0;; it++) {
+        const uint32_t item = s_next[it & 1u];
+        if (item >= n_items) break;  // uniform: every thread read the same word
+        if (threadIdx.x == 0) s_next[(it + 1u) & 1u] = atomicAdd(ticket, 1u);
+        const uint2 q = queue[item];
+        const RasterView& v = views[q.x];
+        raster_tile<TW, PPL>(v.proj, 3, v.values, v.tile_offsets, q.y, v.tiles_x, v.width, v.height, v.image,
+                             s_slot);
+        // the next ticket is visible to all; the slot read at the top of this
+        // iteration is the one thread 0 overwrites in the next
+        __syncthreads();
+    }
+}
+
+template <int TW, int PPL>
+__global__ __launch_bounds__(256) void k_rasterize_queue_waves(const RasterView* __restrict__ views,
+                                                               const uint2* __restrict__ queue,
+                                                               uint32_t n_items, uint32_t* ticket) {
+    constexpr uint32_t kBlocks = RasterShape<TW, PPL>::kWaves;
+    __shared__ Survivor s_slot[256 / kWave][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    Survivor* slots = s_slot[threadIdx.x / kWave];
+    const uint32_t total = n_items * kBlocks;
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(ticket, 1u);
+    uint32_t item = __shfl(t, 0, kWave);
+    while (item < total) {  // uniform over the wavefront
+        if (lane == 0) t = atomicAdd(ticket, 1u);  // the next item, in flight during this one
+        const uint2 q = queue[item / kBlocks];
+        const RasterView& v = views[q.x];
+        raster_block<TW, PPL>(v.proj, 3, v.values, v.tile_offsets, q.y, (int)(item % kBlocks), v.tiles_x,
+                              v.width, v.height, v.image, slots);
+        item = __shfl(t, 0, kWave);
+    }
+}
+
+// ---- the batch's raster queue: (view, tile) items, longest tile lists first ----
+//
+// A tile's list [off[t], off[t+1]) is empty when off[t+1] <= off[t] (the tiles
+// after the last populated one keep zero). Bucket = bit length of the list
+// length (0 for empty), placed in descending bucket order; within a bucket the
+// order is whatever the atomics give, which changes no pixel: every tile's
+// result depends on its own list only.
+constexpr uint32_t kQueueBuckets = 33;
+
+__device__ __forceinline__ uint32_t list_bucket(const uint32_t* off, uint32_t t) {
+    const uint32_t a = off[t], b = off[t + 1];
+    return b > a ? 32u - (uint32_t)__clz(b - a) : 0u;
+}
+
+// grid (ceil(max tiles / 256), views): per workgroup, bucket counts in LDS,
+// one global add per non-empty bucket.
+__global__ __launch_bounds__(256) void k_queue_count(const RasterView* __restrict__ views,
+                                                     uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_count[kQueueBuckets];
+    if (threadIdx.x < kQueueBuckets) s_count[threadIdx.x] = 0u;
+    __syncthreads();
+    const RasterView& v = views[blockIdx.y];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < v.tiles) atomicAdd(&s_count[list_bucket(v.tile_offsets, t)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kQueueBuckets && s_count[threadIdx.x]) atomicAdd(&counts[threadIdx.x], s_count[threadIdx.x]);
+}
+
+// One wavefront: cursor[b] = items in longer buckets (exclusive scan from the top bucket down).
+__global__ void k_queue_place(const uint32_t* __restrict__ counts, uint32_t* __restrict__ cursor) {
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = (int)kQueueBuckets - 1; b >= 0; b--) {
+            cursor[b] = run;
+            run += counts[b];
+        }
+    }
+}
+
+// Same grid as k_queue_count: each workgroup reserves a range per bucket with
+// one global add and places its items in it.
+__global__ __launch_bounds__(256) void k_queue_scatter(const RasterView* __restrict__ views,
+                                                       uint32_t* __restrict__ cursor, uint2* __restrict__ queue) {
+    __shared__ uint32_t s_count[kQueueBuckets];
+    __shared__ uint32_t s_base[kQueueBuckets];
+    if (threadIdx.x < kQueueBuckets) s_count[threadIdx.x] = 0u;
+    __syncthreads();
+    const RasterView& v = views[blockIdx.y];
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    uint32_t b = 0, rank = 0;
+    const bool live = t < v.tiles;
+    if (live) {
+        b = list_bucket(v.tile_offsets, t);
+        rank = atomicAdd(&s_count[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kQueueBuckets)
+        s_base[threadIdx.x] = s_count[threadIdx.x] ? atomicAdd(&cursor[threadIdx.x], s_count[threadIdx.x]) : 0u;
+    __syncthreads();
+    if (live) queue[s_base[b] + rank] = make_uint2(blockIdx.y, t);
 }
 
 __global__ void k_f16_to_f32(const uint16_t* in, float* out, size_t n) {
@@ -366,6 +501,79 @@ hipError_t launch_rasterize_records(const float4* p, uint32_t rec, const uint32_
         case 321: launch_raster_variant<32, 1>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
         case 322: launch_raster_variant<32, 2>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
         case 324: launch_raster_variant<32, 4>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_queue_build(const RasterView* views, uint32_t n_views, uint32_t max_tiles,
+                                     uint32_t* ctl, uint2* queue, hipStream_t s) {
+    if (n_views == 0 || max_tiles == 0) return hipSuccess;
+    uint32_t* counts = ctl + kRasterQueueCounts;
+    uint32_t* cursor = ctl + kRasterQueueCursor;
+    const dim3 grid((max_tiles + 255u) / 256u, n_views);
+    hipLaunchKernelGGL(k_queue_count, grid, dim3(256), 0, s, views, counts);
+    hipLaunchKernelGGL(k_queue_place, dim3(1), dim3(64), 0, s, counts, cursor);
+    hipLaunchKernelGGL(k_queue_scatter, grid, dim3(256), 0, s, views, cursor, queue);
+    return hipGetLastError();
+}
+
+template <int TW, int PPL>
+static hipError_t queue_grid_variant(uint32_t wg_per_cu, bool wave_items, uint32_t* grid) {
+    int per_cu = 0, cus = 0, dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = wave_items ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rasterize_queue_waves<TW, PPL>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rasterize_queue<TW, PPL>,
+                                                                      TW * TW / PPL, 0);
+    if (e != hipSuccess) return e;
+    if (wg_per_cu > 0) per_cu = std::min(per_cu, (int)wg_per_cu);
+    *grid = (uint32_t)std::max(1, per_cu * cus);
+    return hipSuccess;
+}
+
+hipError_t rasterize_queue_grid(uint32_t tile, uint32_t pixels_per_lane, uint32_t wg_per_cu, bool wave_items,
+                                uint32_t* grid) {
+    if (!raster_variant_exists(tile, pixels_per_lane)) return hipErrorInvalidValue;
+    switch (tile * 10 + pixels_per_lane) {
+        case 81: return queue_grid_variant<8, 1>(wg_per_cu, wave_items, grid);
+        case 161: return queue_grid_variant<16, 1>(wg_per_cu, wave_items, grid);
+        case 162: return queue_grid_variant<16, 2>(wg_per_cu, wave_items, grid);
+        case 164: return queue_grid_variant<16, 4>(wg_per_cu, wave_items, grid);
+        case 321: return queue_grid_variant<32, 1>(wg_per_cu, wave_items, grid);
+        case 322: return queue_grid_variant<32, 2>(wg_per_cu, wave_items, grid);
+        case 324: return queue_grid_variant<32, 4>(wg_per_cu, wave_items, grid);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int TW, int PPL>
+static void launch_queue_variant(uint32_t grid, bool wave_items, const RasterView* views, const uint2* queue,
+                                 uint32_t n_items, uint32_t* ticket, hipStream_t s) {
+    if (wave_items)
+        hipLaunchKernelGGL((k_rasterize_queue_waves<TW, PPL>), dim3(grid), dim3(256), 0, s, views, queue, n_items,
+                           ticket);
+    else
+        hipLaunchKernelGGL((k_rasterize_queue<TW, PPL>), dim3(grid), dim3(TW * TW / PPL), 0, s, views, queue,
+                           n_items, ticket);
+}
+
+hipError_t launch_rasterize_queue(const RasterView* views, const uint2* queue, uint32_t n_items,
+                                  uint32_t* ctl, uint32_t tile, uint32_t pixels_per_lane, uint32_t grid,
+                                  bool wave_items, hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    if (!raster_variant_exists(tile, pixels_per_lane) || grid == 0) return hipErrorInvalidValue;
+    if (!wave_items) grid = std::min(grid, n_items);
+    uint32_t* ticket = ctl + kRasterQueueTicket;
+    switch (tile * 10 + pixels_per_lane) {
+        case 81: launch_queue_variant<8, 1>(grid, wave_items, views, queue, n_items, ticket, s); break;
+        case 161: launch_queue_variant<16, 1>(grid, wave_items, views, queue, n_items, ticket, s); break;
+        case 162: launch_queue_variant<16, 2>(grid, wave_items, views, queue, n_items, ticket, s); break;
+        case 164: launch_queue_variant<16, 4>(grid, wave_items, views, queue, n_items, ticket, s); break;
+        case 321: launch_queue_variant<32, 1>(grid, wave_items, views, queue, n_items, ticket, s); break;
+        case 322: launch_queue_variant<32, 2>(grid, wave_items, views, queue, n_items, ticket, s); break;
+        case 324: launch_queue_variant<32, 4>(grid, wave_items, views, queue, n_items, ticket, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

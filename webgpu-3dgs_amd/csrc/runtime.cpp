@@ -68,6 +68,10 @@ struct gs3d_ctx {
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
     uint32_t raster_ppl = 1;
+    uint32_t batch_raster_groups = 1;     // persistent raster launches per batch (view groups); 0: one
+                                          // raster launch per view on its branch stream
+    uint32_t batch_raster_wg_per_cu = 0;  // their workgroups per CU (0: as many as fit)
+    uint32_t batch_raster_wave_items = 0; // 0: a workgroup takes a tile per ticket, 1: a wavefront a pixel block
     uint32_t sort_key_tile = 0;    // the frame's tile-id passes (and stage sorts of 64-bit keys): 0, 8192, 16384
     uint32_t sort_key_tile32 = 0;  // 32-bit key passes: 0 = 8192, or 4096
     uint64_t key_capacity = 0;
@@ -570,6 +574,8 @@ int settle(gs3d_ctx* c) {
 // in device memory, so a new set of cameras replays the same graph.
 
 constexpr int kBatchBranches = 4;
+// at most this many view groups, each rasterized by one persistent launch
+constexpr uint32_t kMaxRasterGroups = 16;
 // a batch view's record per splat: the first three float4s of a gs3d_projected_splat
 constexpr size_t kBatchRecordBytes = 48;
 // Per-view stride (elements) of the batch's depth-key and tile-rectangle
@@ -581,6 +587,7 @@ struct BatchView {
     gs3d_camera cam{};
     uint32_t W = 0, H = 0, tiles_x = 0, tiles_y = 0;
     size_t img_off = 0, to_off = 0;  // offsets into the batch image / tile-range arrays (elements)
+    size_t val_off = 0;              // offset of the view's sorted splat ids (elements)
     int branch = 0;
 };
 
@@ -597,6 +604,12 @@ struct BatchState {
     hipStream_t streams[kBatchBranches] = {};
     hipEvent_t fork = nullptr;
     hipEvent_t join[kBatchBranches] = {};
+    hipStream_t raster_stream = nullptr;  // the persistent rasterizer's launches
+    hipEvent_t group_done[kMaxRasterGroups][kBatchBranches] = {};
+    hipEvent_t raster_join = nullptr;
+    uint32_t groups = 1;  // view groups of the captured graph
+    uint32_t wg_per_cu = 0;  // and the rasterizer's workgroups per CU
+    uint32_t wave_items = 0;  // and its work-item size
     uint32_t* pinned_k = nullptr;  // host-mapped, one K per view (written by identify)
     uint32_t pinned_words = 0;
     std::vector<uint64_t> K;
@@ -605,6 +618,10 @@ struct BatchState {
     bool learned = false;  // the captured graph computes learned colours
     uint64_t splat_gen = 0;  // the splats the last batch projected
     uint32_t epoch_bumps = 0;  // launches since the replay base was last zeroed
+    size_t val_total = 0;      // entries of batch_sorted_values (each view's branch capacity)
+    uint32_t n_items = 0;      // raster queue items: the tiles of every view
+    uint32_t max_tiles = 0;    // tiles of the largest view
+    uint32_t raster_grid = 0;  // workgroups of the persistent rasterizer
 };
 
 void drop_graph(BatchState* b) {
@@ -621,7 +638,13 @@ void destroy_batch(gs3d_ctx* c) {
     for (int k = 0; k < kBatchBranches; k++) {
         if (b->streams[k]) (void)hipStreamSynchronize(b->streams[k]);
     }
+    if (b->raster_stream) (void)hipStreamSynchronize(b->raster_stream);
     drop_graph(b);
+    for (auto& g : b->group_done)
+        for (auto& e : g)
+            if (e) (void)hipEventDestroy(e);
+    if (b->raster_join) (void)hipEventDestroy(b->raster_join);
+    if (b->raster_stream) (void)hipStreamDestroy(b->raster_stream);
     for (int k = 0; k < kBatchBranches; k++) {
         if (b->join[k]) (void)hipEventDestroy(b->join[k]);
         if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
@@ -641,6 +664,10 @@ int batch_init(gs3d_ctx* c) {
         HIP_TRY(c, hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
     }
     HIP_TRY(c, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    HIP_TRY(c, hipStreamCreateWithFlags(&b->raster_stream, hipStreamNonBlocking));
+    HIP_TRY(c, hipEventCreateWithFlags(&b->raster_join, hipEventDisableTiming));
+    for (auto& g : b->group_done)
+        for (auto& e : g) HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return GS3D_OK;
 }
 
@@ -668,8 +695,23 @@ int batch_buffers(gs3d_ctx* c, bool* moved) {
         to = std::max(to, v.to_off + (size_t)v.tiles_x * v.tiles_y + 1);
     }
     const uint32_t radix = 1u << c->digit_bits;
+    // each view's sorted splat ids persist until the batch's one raster launch;
+    // a view's slice holds its branch's key capacity
+    b->val_total = 0;
+    b->n_items = 0;
+    b->max_tiles = 0;
+    for (auto& v : b->views) {
+        v.val_off = b->val_total;
+        b->val_total += b->cap[v.branch];
+        b->n_items += v.tiles_x * v.tiles_y;
+        b->max_tiles = std::max(b->max_tiles, v.tiles_x * v.tiles_y);
+    }
     std::vector<std::pair<std::string, size_t>> want = {
         {"batch_views", V * sizeof(ViewParams)},
+        {"batch_sorted_values", b->val_total * 4},
+        {"batch_raster_views", V * sizeof(RasterView)},
+        {"batch_raster_queue", (size_t)b->n_items * sizeof(uint2)},
+        {"batch_raster_ctl", (size_t)kMaxRasterGroups * kRasterQueueWords * 4},
         {"batch_projected", V * (size_t)n * kBatchRecordBytes},
         {"batch_depth_keys", V * batch_view_stride(n) * 4},
         {"batch_rects", V * batch_view_stride(n) * 8},
@@ -746,11 +788,15 @@ int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, u
                  uint32_t key_bytes, bool first_iota, const uint32_t* n_dev, uint64_t n_max,
                  const uint32_t* shifts, uint32_t passes, const uint32_t* hist, uint32_t* tickets,
                  uint64_t* status, const uint32_t* epoch_base, uint32_t* epoch_ctr, bool* in_b,
-                 uint32_t key_bits, uint32_t key_tile, uint32_t* carry_a = nullptr, uint32_t* carry_b = nullptr) {
+                 uint32_t key_bits, uint32_t key_tile, uint32_t* carry_a = nullptr, uint32_t* carry_b = nullptr,
+                 uint32_t* last_values_out = nullptr) {
+    // last_values_out (or null): where the last pass writes its values instead
+    // of the ping-pong buffer
     const uint32_t radix = 1u << c->digit_bits;
     for (uint32_t p = 0; p < passes; p++) {
         const bool even = (p % 2) == 0;
-        HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, even ? vb : va, key_bytes,
+        uint32_t* vout = (p + 1 == passes && last_values_out) ? last_values_out : (even ? vb : va);
+        HIP_TRY(c, launch_radix_pass(even ? ka : kb, even ? va : vb, even ? kb : ka, vout, key_bytes,
                                      first_iota && p == 0, n_dev, (uint32_t)n_max, hist + p * radix, status,
                                      tickets + p, shifts[p], c->digit_bits, ++*epoch_ctr,
                                      carry_a ? (even ? carry_a : carry_b) : nullptr,
@@ -758,6 +804,30 @@ int batch_passes(gs3d_ctx* c, hipStream_t s, void* ka, uint32_t* va, void* kb, u
                                      pass_match_bits(shifts[p], c->digit_bits, key_bits), carry_a != nullptr));
     }
     *in_b = (passes % 2) == 1;
+    return GS3D_OK;
+}
+
+// The persistent rasterizer of views [v0, v1) on raster_stream: their tiles
+// queued longest list first, then one launch that drains the queue (a fill
+// kernel clears the group's counters: no memset node).
+int batch_raster_group(gs3d_ctx* c, uint32_t group, uint32_t v0, uint32_t v1) {
+    BatchState* b = c->batch;
+    size_t item0 = 0;
+    for (uint32_t i = 0; i < v0; i++) item0 += (size_t)b->views[i].tiles_x * b->views[i].tiles_y;
+    uint32_t items = 0, max_tiles = 0;
+    for (uint32_t i = v0; i < v1; i++) {
+        const uint32_t t = b->views[i].tiles_x * b->views[i].tiles_y;
+        items += t;
+        max_tiles = std::max(max_tiles, t);
+    }
+    const RasterView* rviews = buf<RasterView>(c, "batch_raster_views") + v0;
+    uint2* queue = buf<uint2>(c, "batch_raster_queue") + item0;
+    uint32_t* qctl = buf<uint32_t>(c, "batch_raster_ctl") + (size_t)group * kRasterQueueWords;
+    hipStream_t s = b->raster_stream;
+    HIP_TRY(c, launch_fill_u32(qctl, 0u, kRasterQueueWords, s));
+    HIP_TRY(c, launch_raster_queue_build(rviews, v1 - v0, max_tiles, qctl, queue, s));
+    HIP_TRY(c, launch_rasterize_queue(rviews, queue, items, qctl, c->tile_x, c->raster_ppl, b->raster_grid,
+                                      b->wave_items != 0, s));
     return GS3D_OK;
 }
 
@@ -775,6 +845,7 @@ int batch_enqueue(gs3d_ctx* c) {
     uint32_t* tile_offsets = buf<uint32_t>(c, "batch_tile_offsets");
     uint16_t* images = buf<uint16_t>(c, "batch_images");
     uint32_t* epoch_base = buf<uint32_t>(c, "batch_epoch");
+    uint32_t* sorted_values = buf<uint32_t>(c, "batch_sorted_values");
 
     HIP_TRY(c, launch_bump_epoch(epoch_base, c->stream));
     MultiPreprocessArgs ma{};
@@ -805,6 +876,28 @@ int batch_enqueue(gs3d_ctx* c) {
     HIP_TRY(c, hipEventRecord(b->fork, c->stream));
     for (int k = 0; k < kBatchBranches; k++) HIP_TRY(c, hipStreamWaitEvent(b->streams[k], b->fork, 0));
 
+    // view groups: contiguous view ranges of about equal pixel count, each
+    // rasterized by one persistent launch on raster_stream once its views'
+    // chains are done, while later views' chains still run
+    uint32_t group_begin[kMaxRasterGroups], group_end[kMaxRasterGroups];
+    {
+        uint64_t total_px = 0;
+        for (const auto& v : b->views) total_px += (uint64_t)v.W * v.H;
+        uint32_t g = 0;
+        uint64_t run = 0;
+        group_begin[0] = 0;
+        for (uint32_t vi = 0; vi < V; vi++) {
+            run += (uint64_t)b->views[vi].W * b->views[vi].H;
+            if (g + 1 < b->groups && run * b->groups >= total_px * (g + 1) && vi + 1 < V) {  // groups >= 2
+                group_end[g] = vi + 1;
+                group_begin[++g] = vi + 1;
+            }
+        }
+        group_end[g] = V;
+        for (uint32_t k = g + 1; k < kMaxRasterGroups; k++) group_begin[k] = group_end[k] = V + 1;
+    }
+    uint32_t group = 0;
+    HIP_TRY(c, hipStreamWaitEvent(b->raster_stream, b->fork, 0));
     uint32_t epoch_ctr[kBatchBranches] = {};
     for (uint32_t vi = 0; vi < V; vi++) {
         const BatchView& v = b->views[vi];
@@ -831,7 +924,6 @@ int batch_enqueue(gs3d_ctx* c) {
         uint32_t* vb = buf<uint32_t>(c, branch_name(k, "values_b").c_str());
         uint64_t* scan_status = buf<uint64_t>(c, branch_name(k, "scan_status").c_str());
         uint64_t* radix_status = buf<uint64_t>(c, branch_name(k, "radix_status").c_str());
-        const float4* proj = projected + (size_t)vi * n * 3;
 
         HIP_TRY(c, launch_key_histogram(dka, 4, ctr + kCtrSplatCount, n, hist_depth, hist_partial, bits,
                                         depth_passes, 0, s));
@@ -862,7 +954,9 @@ int batch_enqueue(gs3d_ctx* c) {
         ea.order = sa.order;
         ea.recs = recs;
         ea.tiles = reinterpret_cast<uint32_t*>(ka);
-        ea.values = va;
+        // with no tile-id pass (one tile) the emit order is the sorted order
+        uint32_t* view_values = sorted_values + v.val_off;
+        ea.values = tile_passes ? va : view_values;
         ea.hist = tile_passes ? hist_tile : nullptr;
         ea.hist_partial = tile_passes ? hist_partial : nullptr;
         ea.n = n;
@@ -877,12 +971,26 @@ int batch_enqueue(gs3d_ctx* c) {
         for (uint32_t p = 0; p < tile_passes; p++) tile_shifts[p] = shifts[depth_passes + p] - 32u;
         GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, 4, false, ctr + kCtrRefCounter, cap, tile_shifts,
                               tile_passes, hist_tile, ctr + kCtrSortTicket + depth_passes, radix_status,
-                              epoch_base, &epoch_ctr[k], &in_b, T > 1 ? bit_length(T - 1) : 0u, tile_key_tile(c)));
+                              epoch_base, &epoch_ctr[k], &in_b, T > 1 ? bit_length(T - 1) : 0u, tile_key_tile(c),
+                              nullptr, nullptr, view_values));
         uint32_t* to = tile_offsets + v.to_off;
         HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, 4, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
                                          b->pinned_k + vi, s));
-        HIP_TRY(c, launch_rasterize_records(proj, 3, in_b ? vb : va, to, c->tile_x, c->tile_y, v.tiles_x,
-                                            v.tiles_y, v.W, v.H, c->raster_ppl, images + v.img_off, s));
+        if (b->groups == 0) {
+            // per-view raster launches: each view's raster overlaps other views' chains
+            HIP_TRY(c, launch_rasterize_records(projected + (size_t)vi * n * 3, 3, view_values, to, c->tile_x,
+                                                c->tile_y, v.tiles_x, v.tiles_y, v.W, v.H, c->raster_ppl,
+                                                images + v.img_off, s));
+        } else if (vi + 1 == group_end[group]) {
+            // views [group_begin, vi] are queued on their branches: the group's
+            // persistent raster waits for the branches up to here
+            for (int kk = 0; kk < kBatchBranches; kk++) {
+                HIP_TRY(c, hipEventRecord(b->group_done[group][kk], b->streams[kk]));
+                HIP_TRY(c, hipStreamWaitEvent(b->raster_stream, b->group_done[group][kk], 0));
+            }
+            GS3D_TRY(batch_raster_group(c, group, group_begin[group], vi + 1));
+            group++;
+        }
     }
     for (int k = 0; k < kBatchBranches; k++) {
         if (epoch_ctr[k] >= kEpochStride)  // rebin_batch checks this before anything is queued
@@ -890,6 +998,8 @@ int batch_enqueue(gs3d_ctx* c) {
         HIP_TRY(c, hipEventRecord(b->join[k], b->streams[k]));
         HIP_TRY(c, hipStreamWaitEvent(c->stream, b->join[k], 0));
     }
+    HIP_TRY(c, hipEventRecord(b->raster_join, b->raster_stream));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, b->raster_join, 0));
     return GS3D_OK;
 }
 
@@ -960,6 +1070,8 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
                          (c->learned_enabled || c->learned_force);
     const bool same = b->captured && shape == b->shape && b->n == c->n && b->deg == c->sh_degree &&
                       b->tile == c->tile_x && b->bits == c->digit_bits && b->ppl == c->raster_ppl &&
+                      b->groups == std::min<uint32_t>(c->batch_raster_groups, n_views) &&
+                      b->wg_per_cu == c->batch_raster_wg_per_cu && b->wave_items == c->batch_raster_wave_items &&
                       b->learned == learned;
     if (!same) {
         drop_graph(b);
@@ -972,10 +1084,15 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
     b->tile = c->tile_x;
     b->bits = c->digit_bits;
     b->ppl = c->raster_ppl;
+    b->groups = std::min<uint32_t>(c->batch_raster_groups, n_views);
+    b->wg_per_cu = c->batch_raster_wg_per_cu;
+    b->wave_items = c->batch_raster_wave_items;
     b->learned = learned;
     b->have = false;
     b->splat_gen = c->splat_gen;
     const bool eager = (flags & GS3D_BATCH_EAGER) != 0;
+    HIP_TRY(c, rasterize_queue_grid(c->tile_x, c->raster_ppl, c->batch_raster_wg_per_cu,
+                                    c->batch_raster_wave_items != 0, &b->raster_grid));
     for (int attempt = 0; attempt < 3; attempt++) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         bool moved = false;
@@ -1004,6 +1121,23 @@ int rebin_batch(gs3d_ctx* c, const gs3d_view* views, uint32_t n_views, uint32_t 
             vp[i].counters = counters + (size_t)i * kCtrTotalWords;
         }
         HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_views"), vp.data(), vp.size() * sizeof(ViewParams),
+                                  hipMemcpyHostToDevice, c->stream));
+        std::vector<RasterView> rv(n_views);
+        uint32_t* sorted_values = buf<uint32_t>(c, "batch_sorted_values");
+        uint32_t* tile_offsets = buf<uint32_t>(c, "batch_tile_offsets");
+        uint16_t* images = buf<uint16_t>(c, "batch_images");
+        for (uint32_t i = 0; i < n_views; i++) {
+            const BatchView& v = b->views[i];
+            rv[i].proj = projected + (size_t)i * c->n * 3;
+            rv[i].values = sorted_values + v.val_off;
+            rv[i].tile_offsets = tile_offsets + v.to_off;
+            rv[i].image = reinterpret_cast<uint2*>(images + v.img_off);
+            rv[i].tiles_x = v.tiles_x;
+            rv[i].tiles = v.tiles_x * v.tiles_y;
+            rv[i].width = v.W;
+            rv[i].height = v.H;
+        }
+        HIP_TRY(c, hipMemcpyAsync(buf<void>(c, "batch_raster_views"), rv.data(), rv.size() * sizeof(RasterView),
                                   hipMemcpyHostToDevice, c->stream));
         std::vector<float> eyes;
         if (learned) {
@@ -1365,6 +1499,25 @@ int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
         c->raster_ppl = (uint32_t)value;
         return GS3D_OK;
     }
+    if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "GROUPS")) {
+        if (value < 0 || value > (int)kMaxRasterGroups)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles-batch GROUPS %d not in 0..%u", value,
+                        kMaxRasterGroups);
+        c->batch_raster_groups = (uint32_t)value;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "WAVE_ITEMS")) {
+        if (value != 0 && value != 1)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles-batch WAVE_ITEMS %d not 0 or 1", value);
+        c->batch_raster_wave_items = (uint32_t)value;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "WG_PER_CU")) {
+        if (value < 0 || value > 16)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles-batch WG_PER_CU %d not in 0..16", value);
+        c->batch_raster_wg_per_cu = (uint32_t)value;
+        return GS3D_OK;
+    }
     if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "RADIX_BITS")) {
         if (value != 4 && value != 8)
             return fail(c, GS3D_E_UNKNOWN_NAME, "no radix-sort variant with %d-bit digits", value);
@@ -1396,6 +1549,18 @@ int gs3d_get_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
     }
     if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "TILE_X")) {
         *value = (int)c->tile_x;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "GROUPS")) {
+        *value = (int)c->batch_raster_groups;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "WG_PER_CU")) {
+        *value = (int)c->batch_raster_wg_per_cu;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "WAVE_ITEMS")) {
+        *value = (int)c->batch_raster_wave_items;
         return GS3D_OK;
     }
     if (!std::strcmp(kernel, "radix-sort") && !std::strcmp(param, "RADIX_BITS")) {
@@ -1989,6 +2154,41 @@ int batch_view_records(gs3d_ctx* c, uint32_t view, const void** out) {
     return GS3D_OK;
 }
 
+// A batch view's sorted 64-bit keys, rebuilt on the host from what the batch
+// keeps: the sorted splat ids, the tile ranges (position p of tile t's range
+// holds tile id t) and the view's depths (its records, projected again).
+int batch_view_keys(gs3d_ctx* c, uint32_t view, uint64_t* dst, size_t* written) {
+    BatchState* b = c->batch;
+    const BatchView& v = b->views[view];
+    const size_t K = b->K[view], T = (size_t)v.tiles_x * v.tiles_y;
+    std::vector<uint32_t> vals(K), to(T + 1);
+    std::vector<gs3d_projected_splat> rec(b->n);
+    const void* rsrc = nullptr;
+    if (b->n) GS3D_TRY(batch_view_records(c, view, &rsrc));
+    if (K) HIP_TRY(c, hipMemcpyAsync(vals.data(), buf<uint32_t>(c, "batch_sorted_values") + v.val_off, K * 4,
+                                     hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(to.data(), buf<uint32_t>(c, "batch_tile_offsets") + v.to_off, (T + 1) * 4,
+                              hipMemcpyDeviceToHost, c->stream));
+    if (b->n) HIP_TRY(c, hipMemcpyAsync(rec.data(), rsrc, (size_t)b->n * sizeof(gs3d_projected_splat),
+                                        hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    size_t filled = 0;
+    for (size_t t = 0; t < T; t++) {
+        const size_t lo = to[t], hi = to[t + 1];
+        for (size_t p = lo; p < hi && p < K; p++) {
+            float depth = rec[vals[p]].depth;
+            depth = -depth;
+            uint32_t bits;
+            std::memcpy(&bits, &depth, 4);
+            dst[p] = ((uint64_t)t << 32) | bits;
+            filled++;
+        }
+    }
+    if (filled != K) return fail(c, GS3D_E_STATE, "tile ranges of view %u cover %zu of %zu keys", view, filled, K);
+    if (written) *written = K * 8;
+    return GS3D_OK;
+}
+
 int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes, size_t* written) {
     if (!c) return GS3D_E_INVALID_ARG;
     BatchState* b = c->batch;
@@ -2012,6 +2212,12 @@ int gs3d_batch_read(gs3d_ctx* c, uint32_t view, int id, void* dst, size_t bytes,
             break;
         case GS3D_BUF_REF_COUNTER:
             src = buf<uint32_t>(c, "batch_counters") + (size_t)view * kCtrTotalWords + kCtrRefCounter; size = 4; break;
+        case GS3D_BUF_SORT_VALUES:
+            src = buf<uint32_t>(c, "batch_sorted_values") + v.val_off; size = (size_t)b->K[view] * 4; break;
+        case GS3D_BUF_SORT_KEYS:
+            size = (size_t)b->K[view] * 8;
+            if (dst && bytes >= size) return batch_view_keys(c, view, static_cast<uint64_t*>(dst), written);
+            break;
         default: return fail(c, GS3D_E_UNKNOWN_NAME, "buffer %d is not kept per batch view", id);
     }
     if (written) *written = size;

@@ -2,7 +2,10 @@
 
 Knobs (each a compiled kernel variant): binning/raster tile size {8, 16, 32},
 rasterizer pixels per lane {1, 2, 4} (a wavefront covers 64*ppl pixels of one
-tile) and radix digit width {4, 8}. `tune` times a candidate as the mean GPU
+tile) and radix digit width {4, 8}; for batches also the raster mode: one
+persistent rasterizer over every view's tiles after all views are binned
+(raster_groups 1), or one raster launch per view on its binning stream
+(raster_groups 0). `tune` times a candidate as the mean GPU
 time of whole single-view rebins (hipEvent pairs inside the library) over the
 given views after a warm-up; `tune_batch` times whole batched calls (one
 replayed hipGraph over all the views, wall clock). The fastest wins. Tile size
@@ -69,16 +72,18 @@ class BatchChoice:
     radix_bits: int
     views_per_s: float
     ms_per_batch: float
+    raster_groups: int = 1
     extra: dict = field(default_factory=dict)
 
 
-def tune_batch(renderer, cams, *, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8, 4), warmup=2,
-               reps=4, learned=False) -> tuple[BatchChoice, list[BatchChoice]]:
+def tune_batch(renderer, cams, *, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8, 4), raster_groups=(1, 0),
+               warmup=2, reps=4, learned=False) -> tuple[BatchChoice, list[BatchChoice]]:
     """Time every candidate on gs3d_rebin_batch over `cams` [(camera uniform, W, H), ...]
     (a re-capture per candidate, then `reps` timed replays); returns (best, all)."""
     results = []
-    for t, p, b in candidates(tiles, ppls, radix_bits):
+    for (t, p, b), g in itertools.product(candidates(tiles, ppls, radix_bits), raster_groups):
         apply(renderer, Choice(t, p, b, 0.0, {}))
+        renderer.set_kernel_variant("rasterize-tiles-batch", "GROUPS", g)
         for _ in range(warmup):
             renderer.rebin_batch(cams, learned=learned)
         renderer.sync()
@@ -87,9 +92,10 @@ def tune_batch(renderer, cams, *, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8,
             renderer.rebin_batch(cams, learned=learned)
         renderer.sync()
         per = (time.perf_counter() - t0) / reps
-        results.append(BatchChoice(t, p, b, len(cams) / per, per * 1e3))
+        results.append(BatchChoice(t, p, b, len(cams) / per, per * 1e3, g))
     best = max(results, key=lambda c: c.views_per_s)
     apply(renderer, Choice(best.tile, best.pixels_per_lane, best.radix_bits, 0.0, {}))
+    renderer.set_kernel_variant("rasterize-tiles-batch", "GROUPS", best.raster_groups)
     return best, results
 
 
