@@ -341,7 +341,7 @@ def main() -> int:
         return 0
 
     from gs3d import camera, scene
-    from gs3d._lib import STAGES
+    from gs3d._lib import REBIN_STAGES, STAGES, reference_pass_table
     from gs3d.renderer import GaussianSplatRenderer
 
     t = time.perf_counter()
@@ -370,7 +370,7 @@ def main() -> int:
     for _ in range(20):
         r.rebin(cu, W, H, 1)  # GS3D_REBIN_PROFILE
     stages, _ = r.stage_times_avg_us()
-    dominant = max(STAGES[:-1], key=lambda n: stages[n])
+    dominant = max(REBIN_STAGES, key=lambda n: stages[n])
 
     # Timed region: K back-to-back rebins; every 8th one brackets the dominant
     # kernel with a hipEvent pair on the context's stream (the stream every
@@ -508,6 +508,8 @@ def main() -> int:
         "mkeys_per_s": round(k / ((stages["radix-sort"] + stages["radix-sort-splats"]) * 1e-6) / 1e6, 1)
         if stages["radix-sort"] else None,
         "stages_us": {n: round(v, 2) for n, v in stages.items()},
+        "reference_passes": [[labels, stage, round(us, 2)] for labels, stage, us
+                             in reference_pass_table(stages) if stage != "composite-splats"],
         "stages_note": "stages_us from 20 frames with an event pair per stage (each pair adds a "
                        "dispatch gap); ms_per_step from frames without them",
         "dominant_sampled_frames": frames,

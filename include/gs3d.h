@@ -116,7 +116,10 @@ enum gs3d_stage {
     GS3D_STAGE_IDENTIFY   = 5,  /* "identify-tile-ranges"                         */
     GS3D_STAGE_RASTER     = 6,  /* "rasterize-tiles"                              */
     GS3D_STAGE_TOTAL      = 7,  /* whole rebin, first launch to last              */
-    GS3D_STAGE__COUNT     = 8
+    GS3D_STAGE_COMPOSITE  = 8,  /* "composite-splats": the display pass of
+                                   gs3d_composite ("tile-overlay" fused in, in
+                                   debug mode 1), timed after a profiled rebin   */
+    GS3D_STAGE__COUNT     = 9
 };
 
 typedef struct gs3d_ctx gs3d_ctx;
@@ -199,6 +202,15 @@ int gs3d_read_tile_counts(gs3d_ctx* ctx, uint32_t* counts, size_t n, size_t* wri
 int gs3d_stage_times_us(gs3d_ctx* ctx, float* out, int n);
 const char* gs3d_stage_label(int stage);
 
+/* Which stages do the work of one pass of the reference's Profiler table,
+ * by its label (GaussianSplatRenderer.ts:1017-1260): a bit mask, bit i = enum
+ * gs3d_stage i; 0 for a label the reference does not time. Several reference
+ * passes map to one fused stage here ("prefix-scan-local/-blocks/-add" ->
+ * prefix-scan; the histogram, table scans and scatter of the radix passes ->
+ * radix-sort-splats for the 32 depth bits and radix-sort for the tile bits;
+ * "tile-overlay" and "composite-splats" -> composite-splats). */
+uint32_t gs3d_reference_pass_stages(const char* reference_label);
+
 /* Average per-stage times over every profiled rebin since the last reset
  * (the Profiler's averaging window, Profiler.ts:231-251); `frames` gets the
  * number of frames averaged. */
@@ -208,6 +220,24 @@ int gs3d_profiler_reset(gs3d_ctx* ctx);
  * timing whole frames profiles only what it needs. Unrecorded stages read 0. */
 int gs3d_set_profile_mask(gs3d_ctx* ctx, uint32_t stage_mask);
 int gs3d_stage_times_avg_us(gs3d_ctx* ctx, float* out, int n, uint64_t* frames);
+
+/* Launch geometry (WorkgroupManager.getLayout, src/WorkgroupManager.ts:58-147:
+ * the reference computes workgroup and dispatch sizes from device limits;
+ * here the library owns its launches and reports them). For one launch of
+ * `kernel` over a problem of problem[0] x problem[1] x problem[2]: threads per
+ * workgroup in workgroup[0..2] and workgroups in grid[0..2], exactly as the
+ * library launches it. Kernels and their problem sizes:
+ *   "preprocess-splats", "radix-histogram" (the depth-key digit histogram),
+ *   "radix-sort-splats" (one depth pass), "prefix-scan", "emit-tile-refs":
+ *       problem[0] = splats
+ *   "radix-sort" (one tile-id pass): problem[0] = keys
+ *   "identify-tile-ranges", "rasterize-tiles", "composite-splats":
+ *       problem[0], problem[1] = viewport width, height
+ * ctx may be NULL (the defaults: 16x16 tiles, 1 pixel per lane, default key
+ * tiles); otherwise the context's tile size and kernel variants count.
+ * GS3D_E_UNKNOWN_NAME for another kernel name. */
+int gs3d_launch_geometry(const gs3d_ctx* ctx, const char* kernel, const uint64_t problem[3],
+                         uint32_t workgroup[3], uint64_t grid[3]);
 
 /* Number of radix passes the last rebin's sort ran and the bit offset of each
  * (passes whose digit is constant over all keys are skipped). */

@@ -83,19 +83,20 @@ function camera(req) {
   return { ok: true, cameras: out };
 }
 
+// Launch layouts of the library's passes (no context: the default variants).
 function workgroups(req) {
-  const wm = new host.WorkgroupManager(req.limits);
+  const wm = new host.WorkgroupManager(host.loadAddon(), null);
   const out = [];
   for (const c of req.cases) {
-    const fn = host[c.strategy];
     try {
-      wm.register({ name: c.name, problemSize: c.problemSize, strategyFn: fn, strategyArgs: c.args });
+      if (c.update) wm.update(c.name, { problemSize: c.problemSize });
+      else wm.register({ name: c.name, kernel: c.kernel, problemSize: c.problemSize });
       out.push(wm.getLayout(c.name));
     } catch (e) {
       out.push({ error: String(e.message) });
     }
   }
-  return { ok: true, layouts: out };
+  return { ok: true, layouts: out, names: [...wm.layouts().keys()] };
 }
 
 function pipelines(req) {
@@ -130,7 +131,7 @@ async function render(req) {
     const cam = new host.Camera(req.eye, [0, 0, 0], [0, 1, 0], Math.PI / 3, req.width / req.height, 0.1, 10000);
     const frame = { width: req.width, height: req.height, camera: cam };
     const decisions = [r.render(frame), r.render(frame)];
-    cam.rotate(0.0, 0.0);
+    cam.setTarget([0, 0, 0]);  // same view, new camera version
     decisions.push(r.render(frame));
     const image = r.readImage(false);
     const counts = await r.readTileCounts();
@@ -146,6 +147,36 @@ async function render(req) {
       uniform: b64(cam.getUniformData(req.width, req.height)), variant,
       framebuffer: b64(framebuffer), compositeRebinned: composited, debugRef: r.debugRef,
     };
+  } finally {
+    r.destroy();
+  }
+}
+
+// GPU: profiled frames (rebin + display pass) through GaussianSplatRenderer and
+// the Profiler: the averaged stage table, the reference's pass table and the
+// launch layouts of the last frame.
+async function profile(req) {
+  const scene = new host.Scene();
+  const { splats, bounds } = await new host.SceneLoader({ native: true }).loadFile(req.path);
+  scene.splats = splats;
+  scene.bounds = bounds;
+  scene.tiles = [16, 16];
+  const r = new host.GaussianSplatRenderer(scene, { profiler: true });
+  const profiler = new host.Profiler(r);
+  r.profiler = profiler;
+  try {
+    r.init();
+    r.setAlwaysRebin(true);
+    const cam = new host.Camera(req.eye, [0, 0, 0], [0, 1, 0], Math.PI / 3, req.width / req.height, 0.1, 10000);
+    const framebuffer = new Uint8Array(req.width * req.height * 4);
+    const frame = { width: req.width, height: req.height, camera: cam, colorTexture: framebuffer };
+    r.render(frame);
+    profiler.resetWindow();
+    for (let i = 0; i < req.frames; i++) r.render(frame);
+    const timings = Object.fromEntries(profiler.timings());
+    const reference = profiler.referenceTimings();
+    const layouts = Object.fromEntries(r.launchLayouts());
+    return { ok: true, timings, reference, layouts, refCount: r.refCount() };
   } finally {
     r.destroy();
   }
@@ -196,7 +227,7 @@ function argErrors(req) {
   return { ok: true, results: out };
 }
 
-const HANDLERS = { ply, plyTime, batch, camera, workgroups, pipelines, render, addonExports, argErrors };
+const HANDLERS = { ply, plyTime, batch, camera, workgroups, pipelines, render, profile, addonExports, argErrors };
 
 (async () => {
   const req = JSON.parse(fs.readFileSync(process.argv[2], 'utf8'));

@@ -28,7 +28,7 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_abi_version_and_labels():
     lib = L.lib()
     assert lib.gs3d_abi_version() == 1
-    assert [lib.gs3d_stage_label(i).decode() for i in range(8)] == list(L.STAGES)
+    assert [lib.gs3d_stage_label(i).decode() for i in range(9)] == list(L.STAGES)
     assert lib.gs3d_stage_label(99) == b""
 
 
@@ -67,3 +67,33 @@ def test_struct_sizes():
     import oracle_lib
 
     oracle_lib.lib()  # built with _Static_assert(sizeof(projected) == 64)
+
+
+def test_launch_geometry_without_device():
+    """gs3d_launch_geometry (what WorkgroupManager reports) with no context: the default
+    variants' launch shapes; an unknown pass is GS3D_E_UNKNOWN_NAME."""
+    assert L.launch_geometry("preprocess-splats", [100_000]) == ((256, 1, 1), (391, 1, 1))
+    assert L.launch_geometry("prefix-scan", [0]) == ((1024, 1, 1), (1, 1, 1))
+    assert L.launch_geometry("radix-sort", [16_384]) == ((1024, 1, 1), (1, 1, 1))
+    assert L.launch_geometry("radix-sort", [16_385]) == ((1024, 1, 1), (2, 1, 1))
+    assert L.launch_geometry("rasterize-tiles", [1281, 720]) == ((256, 1, 1), (81 * 45, 1, 1))
+    assert L.launch_geometry("identify-tile-ranges", [1280, 720]) == ((256, 1, 1), (15, 1, 1))
+    assert L.launch_geometry("composite-splats", [1920, 1080])[1][0] == -(-1920 * 1080 // 1024)
+    with pytest.raises(L.Gs3dError) as e:
+        L.launch_geometry("rasterize", [64, 64])
+    assert e.value.code == -6
+
+
+def test_reference_pass_stages():
+    """Every pass label of the reference's Profiler table maps to the fused stage(s)
+    doing its work; an unknown label maps to none."""
+    assert L.reference_pass_stages("prefix-scan-blocks") == ("prefix-scan",)
+    assert L.reference_pass_stages("radix-scatter") == ("radix-sort-splats", "radix-sort")
+    assert L.reference_pass_stages("tile-overlay") == ("composite-splats",)
+    assert L.reference_pass_stages("grid") == ()
+    table = L.reference_pass_table({s: float(i + 1) for i, s in enumerate(L.STAGES)})
+    assert [lab for labels, _, _ in table for lab in labels] == list(L.REFERENCE_PASSES)
+    assert [stage for _, stage, _ in table] == [
+        "preprocess-splats", "prefix-scan", "emit-tile-refs", "radix-sort-splats+radix-sort",
+        "identify-tile-ranges", "rasterize-tiles", "composite-splats"]
+    assert table[3][2] == 2.0 + 5.0

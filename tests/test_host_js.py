@@ -3,10 +3,11 @@
 CPU tests: the PLY loader (ascii, binary little/big endian, extra elements,
 f_rest ordering, SH degree, header errors) against the Python restatement of
 the loader's activations (SceneWorker.ts:407-464); the Camera uniform against
-the textbook look-at / perspective closed forms (Camera.ts:69-191); the
-WorkgroupManager strategies (WorkgroupManager.ts:10-147) against their closed
-forms; PipelineManager variant validation. The GPU test renders through
-GaussianSplatRenderer.render and compares with the oracle pipeline.
+the textbook look-at / perspective closed forms (Camera.ts:69-191) and the
+Python mirror; WorkgroupManager layouts against the library's launch geometry;
+PipelineManager variant validation. The GPU tests render through
+GaussianSplatRenderer.render and renderBatch and compare with the oracle
+pipeline, and read the Profiler's stage and reference-pass tables.
 """
 import base64
 import json
@@ -279,65 +280,69 @@ def test_camera_matches_python_mirror(tmp_path):
         np.testing.assert_allclose(u[32:64], v[32:64], rtol=1e-6, atol=1e-6)
 
 
-def test_camera_controls(tmp_path):
-    """rotate keeps the distance to the target, pan moves position and target by
-    the same offset, zoom stops at the near plane; each bumps the version."""
-    base = dict(CAMS[1])
-    cams = [dict(base, ops=[["rotate", 30.0, -12.0]]), dict(base, ops=[["pan", 5.0, -3.0]]),
-            dict(base, ops=[["zoom", 1e6]]), dict(base, ops=[["setPosition", [1, 2, 3]]]),
-            dict(base, ops=[["setAspect", 2.0]])]
+def test_camera_setters(tmp_path):
+    """setOrbit puts the camera at the orbit position gs3d.scene.orbit_eye gives (the
+    workloads' camera placement) and its uniform equals the Python mirror's; setPosition,
+    setTarget and setAspect change the view or projection; each bumps the version."""
+    base = dict(CAMS[1], target=[0.0, 0.0, 0.0])
+    cams = [dict(base, ops=[["setOrbit", 8.0, 45.0, 20.0]]), dict(base, ops=[["setPosition", [1, 2, 3]]]),
+            dict(base, ops=[["setTarget", [0.5, 0.5, -1.0]]]), dict(base, ops=[["setAspect", 2.0]])]
     r = run_node({"cmd": "camera", "cameras": cams}, tmp_path)["cameras"]
-    p0, t0 = np.array(base["position"], float), np.array(base["target"], float)
-    d0 = np.linalg.norm(p0 - t0)
-    rot = r[0]
-    assert abs(np.linalg.norm(np.array(rot["position"]) - t0) - d0) < 1e-4 * d0
-    assert not np.allclose(rot["position"], p0)
-    pan = r[1]
-    np.testing.assert_allclose(np.array(pan["position"]) - p0, np.array(pan["target"]) - t0, atol=1e-5)
-    zoom = r[2]
-    assert abs(np.linalg.norm(np.array(zoom["position"]) - t0) - base["near"]) < 1e-4
-    np.testing.assert_allclose(r[3]["position"], [1, 2, 3])
+    eye = S.orbit_eye(8.0, 45.0, 20.0)
+    np.testing.assert_allclose(r[0]["position"], eye, rtol=1e-6, atol=1e-6)
+    py = C.Camera(r[0]["position"], [0, 0, 0], base["up"], base["fov"], base["aspect"], base["near"], base["far"])
+    u, v = f32(r[0]["uniform"]), py.uniform_data(base["width"], base["height"])
+    np.testing.assert_array_equal(u[:32], v[:32])
+    np.testing.assert_allclose(u[32:64], v[32:64], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(r[1]["position"], [1, 2, 3])
+    view = look_at_closed_form([1, 2, 3], [0, 0, 0], base["up"])
+    np.testing.assert_allclose(f32(r[1]["uniform"])[0:16], view, rtol=0, atol=2e-6 * max(1, np.abs(view).max()))
+    np.testing.assert_allclose(r[2]["target"], [0.5, 0.5, -1.0])
     for out in r:
         assert out["versionDelta"] >= 1
-    u = f32(r[4]["uniform"]).astype(np.float64)
+    u = f32(r[3]["uniform"]).astype(np.float64)
     np.testing.assert_allclose(u[16:32], perspective_closed_form(base["fov"], 2.0, base["near"], base["far"]),
                                rtol=3e-7, atol=1e-12)
 
 
 # ---------------------------------------------------------------- managers
 
-LIMITS = dict(maxTotalThreads=1024, maxSizeX=1024, maxSizeY=1024, maxSizeZ=1024, maxDispatch=2147483647,
-              maxSharedMemory=65536)
+@pytest.mark.skipif(not ADDON.exists(), reason="addon not built")
+def test_workgroup_layouts_are_the_library_launches(tmp_path):
+    """WorkgroupManager reports what libgs3d launches (gs3d_launch_geometry, the same
+    shape functions the launchers size their grids with): the JS layouts equal the C
+    ABI's for every pass, follow problem-size updates, and an unknown pass throws."""
+    from gs3d import _lib as L
 
-
-def test_workgroup_strategies(tmp_path):
     cases = [
-        dict(name="a", strategy="linear1D", args=[256], problemSize=[1000, 1, 1]),
-        dict(name="b", strategy="linear1D", args=[256, 16], problemSize=[16_000_000, 1, 1]),
-        dict(name="c", strategy="linear1D", args=[300], problemSize=[1, 1, 1]),
-        dict(name="d", strategy="tiled1D", args=[8, 4, 256], problemSize=[1_000_000, 1, 1]),
-        dict(name="e", strategy="tiled1D", args=[8, 4096, 1024], problemSize=[100, 1, 1]),
-        dict(name="f", strategy="tile2D", args=[16, 16], problemSize=[1920, 1080, 1]),
-        dict(name="g", strategy="tile2D", args=[64, 32], problemSize=[10, 10, 1]),
-        dict(name="h", strategy="linear1D", args=[256], problemSize=[0, 1, 1]),
+        dict(name="pre", kernel="preprocess-splats", problemSize=[1_000_000]),
+        dict(name="hist", kernel="radix-histogram", problemSize=[1_000_000]),
+        dict(name="depth", kernel="radix-sort-splats", problemSize=[1_000_000]),
+        dict(name="scan", kernel="prefix-scan", problemSize=[1_000_000]),
+        dict(name="emit", kernel="emit-tile-refs", problemSize=[1_000_000]),
+        dict(name="sort", kernel="radix-sort", problemSize=[3_615_659]),
+        dict(name="ident", kernel="identify-tile-ranges", problemSize=[1920, 1080]),
+        dict(name="raster", kernel="rasterize-tiles", problemSize=[1920, 1080]),
+        dict(name="comp", kernel="composite-splats", problemSize=[1920, 1080]),
+        dict(name="raster", problemSize=[640, 360], update=True),
+        dict(name="bad", kernel="no-such-pass", problemSize=[1]),
+        dict(name="never", problemSize=[5], update=True),
     ]
-    r = run_node({"cmd": "workgroups", "limits": LIMITS, "cases": cases}, tmp_path)["layouts"]
-    assert r[0] == {"workgroupSize": [256, 1, 1], "dispatchSize": [4, 1, 1]}
-    assert r[1] == {"workgroupSize": [256, 1, 1], "dispatchSize": [3907, 1, 1]}
-    assert r[2] == {"workgroupSize": [256, 1, 1], "dispatchSize": [1, 1, 1]}  # highest power of two <= 300
-    assert r[3] == {"workgroupSize": [256, 1, 1], "dispatchSize": [489, 1, 1]}
-    # 8 elements x 4096 B x x <= 65536 forces x = 2
-    assert r[4] == {"workgroupSize": [2, 1, 1], "dispatchSize": [7, 1, 1]}
-    assert r[5] == {"workgroupSize": [16, 16, 1], "dispatchSize": [120, 68, 1]}
-    assert r[6] == {"error": "Workgroup layout exceeds hardware limits."}  # 64 x 32 = 2048 threads
-    assert r[7] == {"workgroupSize": [256, 1, 1], "dispatchSize": [1, 1, 1]}
-
-
-def test_workgroup_dispatch_clamp(tmp_path):
-    lim = dict(LIMITS, maxDispatch=65535)
-    cases = [dict(name="a", strategy="linear1D", args=[256], problemSize=[256 * 70000, 1, 1])]
-    r = run_node({"cmd": "workgroups", "limits": lim, "cases": cases}, tmp_path)["layouts"]
-    assert r[0]["dispatchSize"] == [65535, 1, 1]
+    r = run_node({"cmd": "workgroups", "cases": cases}, tmp_path)
+    out = r["layouts"]
+    for c, lay in zip(cases[:9], out[:9]):
+        wg, grid = L.launch_geometry(c["kernel"], c["problemSize"])
+        assert lay == {"workgroupSize": list(wg), "dispatchSize": list(grid)}, c
+    # the 1M-splat 1080p frame: thread per splat, 8,192-key depth tiles, 8,192 counts per
+    # scan workgroup, 16,384-key tile-id tiles, one 16x16 workgroup per tile
+    assert out[0] == {"workgroupSize": [256, 1, 1], "dispatchSize": [3907, 1, 1]}
+    assert out[2]["dispatchSize"][0] == 123 and out[3]["dispatchSize"][0] == 123
+    assert out[5] == {"workgroupSize": [1024, 1, 1], "dispatchSize": [221, 1, 1]}
+    assert out[7] == {"workgroupSize": [256, 1, 1], "dispatchSize": [8160, 1, 1]}
+    assert out[9]["dispatchSize"] == [40 * 23, 1, 1]
+    assert "no pass named 'no-such-pass'" in out[10]["error"]
+    assert "No workgroup layout registered for 'never'" in out[11]["error"]
+    assert r["names"] == ["pre", "hist", "depth", "scan", "emit", "sort", "ident", "raster", "comp"]
 
 
 def test_pipeline_variants(tmp_path):
@@ -372,7 +377,8 @@ def test_addon_exports(tmp_path):
             "bufferWrite", "bufferRead", "bufferDestroy", "bufferList", "setKernelVariant",
             "getKernelVariant", "setProfileMask", "sortPasses", "setDebugMode", "composite",
             "parsePly", "loadPly", "rebinBatch", "batchRefCounts", "batchRead", "learnedInit",
-            "learnedTrain", "learnedValidate", "learnedEnable", "abiVersion"}
+            "learnedTrain", "learnedValidate", "learnedEnable", "launchGeometry", "referencePassStages",
+            "abiVersion"}
     assert set(r["names"]) == want
     assert r["abiVersion"] >= 1
 
@@ -450,3 +456,34 @@ def test_node_render_batch_equals_single_views(tmp_path):
         assert r["batchImages"][i] == sv["image"]
     assert len(r["losses"]) == 2 and r["losses"][1] < r["losses"][0]
     assert r["validation"]["enabled"] == (r["validation"]["psnr"] >= 45)
+
+
+@pytest.mark.gpu
+def test_node_profiler_table(tmp_path):
+    """The Profiler over profiled frames with their display pass: every stage label in
+    execution order with a time > 0, the stages adding up to the whole rebin (each event
+    pair adds a dispatch gap, so their sum is a little below it), the reference's 13 pass
+    labels each in one row of the reference table, and the launch layouts of the frame."""
+    raw = S.clustered(14, 200_000, 16, sh_degree=3)
+    p = tmp_path / "scene.ply"
+    write_ply(p, raw, "binary_little_endian")
+    r = run_node({"cmd": "profile", "path": str(p), "eye": [5.0, 2.0, 6.0], "width": 960, "height": 540,
+                  "frames": 12}, tmp_path, timeout=300)
+    t = r["timings"]
+    from gs3d import _lib as L
+
+    assert list(t) == list(L.STAGES)
+    assert all(t[s] > 0 for s in L.STAGES), t
+    parts = sum(t[s] for s in L.REBIN_STAGES)
+    assert 0.5 * t["rebin-total"] <= parts <= 1.02 * t["rebin-total"], t
+    ref = r["reference"]
+    assert [lab for row in ref for lab in row["passes"]] == list(L.REFERENCE_PASSES)
+    by_stage = {row["stages"]: row for row in ref}
+    assert by_stage["prefix-scan"]["passes"] == ["prefix-scan-local", "prefix-scan-blocks", "prefix-scan-add"]
+    assert by_stage["radix-sort-splats+radix-sort"]["passes"] == ["radix-histogram", "scan-local", "scan-add",
+                                                                   "radix-scatter"]
+    assert abs(by_stage["radix-sort-splats+radix-sort"]["us"] - t["radix-sort-splats"] - t["radix-sort"]) < 1e-3
+    assert by_stage["composite-splats"]["passes"] == ["tile-overlay", "composite-splats"]
+    lay = r["layouts"]
+    assert lay["rasterize-tiles"] == {"workgroupSize": [256, 1, 1], "dispatchSize": [60 * 34, 1, 1]}
+    assert lay["radix-sort"]["dispatchSize"][0] == -(-r["refCount"] // 16384)

@@ -552,6 +552,70 @@ static napi_value js_get_kernel_variant(napi_env env, napi_callback_info info) {
     return r;
 }
 
+/* launchGeometry(ctx | null, kernel, [x, y, z]) -> {workgroupSize: [3], dispatchSize: [3]}:
+ * the launch shape libgs3d.so uses for one pass of `kernel` (gs3d_launch_geometry). */
+static napi_value js_launch_geometry(napi_env env, napi_callback_info info) {
+    ARGS(3);
+    gs3d_ctx* c = NULL;
+    if (!is_nullish(env, argv[0])) {
+        c = get_ctx(env, argv[0]);
+        if (!c) return NULL;
+    }
+    char kernel[64];
+    uint64_t problem[3] = {1, 1, 1};
+    bool is_array = false;
+    uint32_t len = 0;
+    if (!get_name(env, argv[1], kernel, sizeof kernel) || napi_is_array(env, argv[2], &is_array) != napi_ok ||
+        !is_array || napi_get_array_length(env, argv[2], &len) != napi_ok || len < 1 || len > 3) {
+        napi_throw_type_error(env, NULL, "launchGeometry(ctx | null, kernel, [x, y, z])");
+        return NULL;
+    }
+    for (uint32_t i = 0; i < len; i++) {
+        napi_value e;
+        size_t v = 0;
+        if (napi_get_element(env, argv[2], i, &e) != napi_ok ||
+            !arg_size(env, e, &v, "launchGeometry(ctx | null, kernel, [x, y, z])"))
+            return NULL;
+        problem[i] = v;
+    }
+    uint32_t wg[3];
+    uint64_t grid[3];
+    const int rc = gs3d_launch_geometry(c, kernel, problem, wg, grid);
+    if (rc != GS3D_OK) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "gs3d_launch_geometry failed (%d): no pass named '%s'", rc, kernel);
+        char code[16];
+        snprintf(code, sizeof code, "%d", rc);
+        napi_throw_error(env, code, msg);
+        return NULL;
+    }
+    napi_value out, w, d;
+    napi_create_object(env, &out);
+    napi_create_array_with_length(env, 3, &w);
+    napi_create_array_with_length(env, 3, &d);
+    for (uint32_t i = 0; i < 3; i++) {
+        napi_value x;
+        napi_create_uint32(env, wg[i], &x);
+        napi_set_element(env, w, i, x);
+        napi_create_double(env, (double)grid[i], &x);
+        napi_set_element(env, d, i, x);
+    }
+    napi_set_named_property(env, out, "workgroupSize", w);
+    napi_set_named_property(env, out, "dispatchSize", d);
+    return out;
+}
+
+/* referencePassStages(label) -> bit mask of the stages doing that reference pass's work. */
+static napi_value js_reference_pass_stages(napi_env env, napi_callback_info info) {
+    ARGS(1);
+    char label[64];
+    if (!get_name(env, argv[0], label, sizeof label)) {
+        napi_throw_type_error(env, NULL, "referencePassStages(label)");
+        return NULL;
+    }
+    return make_u32(env, gs3d_reference_pass_stages(label));
+}
+
 /* setProfileMask(ctx, mask): which stages a profiled rebin brackets with events. */
 static napi_value js_set_profile_mask(napi_env env, napi_callback_info info) {
     ARGS(2);
@@ -948,6 +1012,8 @@ static napi_value init(napi_env env, napi_value exports) {
         FN("setKernelVariant", js_set_kernel_variant),
         FN("getKernelVariant", js_get_kernel_variant),
         FN("setProfileMask", js_set_profile_mask),
+        FN("launchGeometry", js_launch_geometry),
+        FN("referencePassStages", js_reference_pass_stages),
         FN("sortPasses", js_sort_passes),
         FN("setDebugMode", js_set_debug_mode),
         FN("composite", js_composite),

@@ -275,6 +275,21 @@ hipError_t launch_rasterize_records(const float4* records, uint32_t rec, const u
                                     uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
 
+// Launch shape of a pass for a problem size: threads per workgroup and
+// workgroups. The launchers size their grids with these same functions.
+struct LaunchShape {
+    uint32_t block;
+    uint64_t grid;
+};
+LaunchShape preprocess_launch_shape(uint32_t n);
+LaunchShape key_histogram_launch_shape(uint32_t n);
+LaunchShape radix_pass_launch_shape(uint32_t key_bytes, uint32_t key_tile, uint32_t n);
+LaunchShape scan_rects_launch_shape(uint32_t n);
+LaunchShape emit_launch_shape(uint32_t n);
+LaunchShape identify_launch_shape(uint32_t tiles);
+LaunchShape raster_launch_shape(uint32_t tile, uint32_t pixels_per_lane, uint32_t tiles);
+LaunchShape composite_launch_shape(uint32_t pixels);
+
 // One view of the batch's persistent rasterizer (device memory).
 struct RasterView {
     const float4* proj;            // the view's 48-B records (3 float4s per splat)

@@ -153,13 +153,17 @@ hipError_t launch_composite(const CompositeParams& p, hipStream_t s) {
     a.clear = p.clear;
     a.use_clear = p.use_clear;
     a.bgra = p.bgra;
-    const uint32_t per_block = kCompositeThreads * kCompositePixels;
-    const uint32_t blocks = (a.pixels + per_block - 1) / per_block;
+    const uint32_t blocks = (uint32_t)composite_launch_shape(a.pixels).grid;
     if (p.debug)
         hipLaunchKernelGGL(k_composite<true>, dim3(blocks), dim3(kCompositeThreads), 0, s, a);
     else
         hipLaunchKernelGGL(k_composite<false>, dim3(blocks), dim3(kCompositeThreads), 0, s, a);
     return hipGetLastError();
+}
+
+LaunchShape composite_launch_shape(uint32_t pixels) {
+    const uint32_t per_block = kCompositeThreads * kCompositePixels;
+    return {(uint32_t)kCompositeThreads, ((uint64_t)pixels + per_block - 1) / per_block};
 }
 
 }  // namespace gs3d

@@ -323,6 +323,9 @@ hipError_t launch_emit(const EmitArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+LaunchShape emit_launch_shape(uint32_t n) { return {(uint32_t)kEmitThreads, emit_grid(n)}; }
+LaunchShape key_histogram_launch_shape(uint32_t n) { return {256u, key_hist_rows(n)}; }
+
 uint32_t key_hist_rows(uint32_t n_max) {
     const uint32_t blocks = (n_max + 256u * kHistKeys - 1u) / (256u * kHistKeys);
     return blocks < 1024u ? (blocks ? blocks : 1u) : 1024u;

@@ -372,10 +372,12 @@ __global__ __launch_bounds__(256) void k_sh_relayout(const float* f_dc, const fl
 
 }  // namespace
 
+LaunchShape preprocess_launch_shape(uint32_t n) { return {256u, (n + 255ull) / 256u}; }
+
 hipError_t launch_preprocess(const PreprocessArgs& a, hipStream_t s) {
     const uint32_t threads = a.n > (a.counters ? a.counter_words : 0u) ? a.n : a.counter_words;
     if (threads == 0) return hipSuccess;
-    const uint32_t grid = (threads + 255u) / 256u;
+    const uint32_t grid = (uint32_t)preprocess_launch_shape(threads).grid;
     switch (a.sh_degree) {
         case 0: hipLaunchKernelGGL(k_preprocess<0>, dim3(grid), dim3(256), 0, s, a); break;
         case 1: hipLaunchKernelGGL(k_preprocess<1>, dim3(grid), dim3(256), 0, s, a); break;

@@ -442,11 +442,17 @@ __global__ void k_f16_to_f32(const uint16_t* in, float* out, size_t n) {
 
 }  // namespace
 
+LaunchShape identify_launch_shape(uint32_t tiles) { return {256u, ((uint64_t)tiles + 1u + 255u) / 256u}; }
+LaunchShape raster_launch_shape(uint32_t tile, uint32_t pixels_per_lane, uint32_t tiles) {
+    return {tile * tile / (pixels_per_lane ? pixels_per_lane : 1u), tiles};
+}
+
 hipError_t launch_identify_tiles(const void* keys, uint32_t key_bytes, const uint32_t* n_dev, uint32_t n_max,
                                  uint32_t* tile_offsets, uint32_t tile_count, uint32_t* k_host, hipStream_t s) {
     const uint32_t threads = tile_count + 1;
     const uint32_t stride = key_bytes == 8 ? 2u : 1u, word = key_bytes == 8 ? 1u : 0u;
-    hipLaunchKernelGGL(k_identify, dim3((threads + 255u) / 256u), dim3(256), 0, s,
+    (void)threads;
+    hipLaunchKernelGGL(k_identify, dim3((uint32_t)identify_launch_shape(tile_count).grid), dim3(256), 0, s,
                        static_cast<const uint32_t*>(keys), stride, word, n_dev, tile_offsets, tile_count, n_max,
                        k_host);
     return hipGetLastError();

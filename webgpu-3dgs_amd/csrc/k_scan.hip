@@ -289,8 +289,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
+LaunchShape scan_rects_launch_shape(uint32_t n) {
+    return {(uint32_t)kScanThreads, n == 0 ? 1ull : ((uint64_t)n + kScanTile - 1) / kScanTile};
+}
+
 hipError_t launch_scan_rects(const ScanRectsArgs& a, hipStream_t s) {
-    const uint32_t parts = a.n == 0 ? 1u : (a.n + kScanTile - 1) / kScanTile;
+    const uint32_t parts = (uint32_t)scan_rects_launch_shape(a.n).grid;
     hipLaunchKernelGGL(k_scan_rects, dim3(parts), dim3(kScanThreads), 0, s, a);
     return hipGetLastError();
 }

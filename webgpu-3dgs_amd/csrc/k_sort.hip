@@ -330,6 +330,31 @@ static void launch_shape(int shape, const void* kin, const uint32_t* vin, void* 
     }
 }
 
+// Threads and keys per thread of a pass shape (see launch_radix_pass for the
+// key_tile values).
+static void pass_shape(uint32_t key_bytes, uint32_t key_tile, uint32_t n_max, uint32_t* threads, uint32_t* items) {
+    if (key_bytes == 8) {
+        const bool small = key_tile == 8192 || (key_tile == 0 && n_max >= kSortSmallTileFrom);
+        *threads = small ? kSort64SmallThreads : kSort64Threads;
+        *items = small ? kSort64SmallItems : kSort64Items;
+    } else if (key_tile == 4096) {
+        *threads = kSort32SmallThreads;
+        *items = kSort32SmallItems;
+    } else if (key_tile == 16384) {
+        *threads = kSort32LargeThreads;
+        *items = kSort32LargeItems;
+    } else {
+        *threads = kSort32Threads;
+        *items = kSort32Items;
+    }
+}
+
+LaunchShape radix_pass_launch_shape(uint32_t key_bytes, uint32_t key_tile, uint32_t n) {
+    uint32_t threads = 0, items = 0;
+    pass_shape(key_bytes, key_tile, n, &threads, &items);
+    return {threads, ((uint64_t)n + threads * items - 1) / (threads * items)};
+}
+
 hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, uint32_t* vout,
                              uint32_t key_bytes, bool iota_values, const uint32_t* n_dev,
                              uint32_t n_max, const uint32_t* hist_pass, uint64_t* status,

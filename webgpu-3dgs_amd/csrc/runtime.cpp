@@ -48,7 +48,9 @@ struct DevBuf {
 
 const char* kStageLabels[GS3D_STAGE__COUNT] = {
     "preprocess-splats", "radix-sort-splats", "prefix-scan", "emit-tile-refs",
-    "radix-sort", "identify-tile-ranges", "rasterize-tiles", "rebin-total"};
+    "radix-sort", "identify-tile-ranges", "rasterize-tiles", "rebin-total", "composite-splats"};
+// stages a rebin brackets (the composite is timed by gs3d_composite)
+constexpr uint32_t kRebinStageMask = (1u << GS3D_STAGE_COMPOSITE) - 1u;
 
 }  // namespace
 
@@ -107,6 +109,10 @@ struct gs3d_ctx {
     bool last_profiled = false;
     double stage_sum_us[GS3D_STAGE__COUNT] = {};
     uint64_t stage_frames = 0;
+    // the display pass, timed by gs3d_composite after a profiled rebin
+    hipEvent_t composite_ev[2] = {};
+    bool composite_pending = false;
+    float composite_last_us = 0.f;
 
     struct BatchState* batch = nullptr;  // gs3d_rebin_batch state, created on first use
 
@@ -267,6 +273,19 @@ int fold_slot(gs3d_ctx* c, int slot) {
     }
     c->stage_frames++;
     c->ring_unread[slot] = false;
+    return GS3D_OK;
+}
+
+// Add the last timed composite to the running sums.
+int fold_composite(gs3d_ctx* c) {
+    if (!c->composite_pending) return GS3D_OK;
+    HIP_TRY(c, hipEventSynchronize(c->composite_ev[1]));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->composite_ev[0], c->composite_ev[1]));
+    c->composite_last_us = ms * 1000.0f;
+    c->stage_sum_us[GS3D_STAGE_COMPOSITE] += (double)ms * 1000.0;
+    c->stage_count[GS3D_STAGE_COMPOSITE]++;
+    c->composite_pending = false;
     return GS3D_OK;
 }
 
@@ -536,7 +555,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     c->k_pending = true;
     if (c->profiled) {
         c->ring_unread[c->cur_slot] = true;
-        c->ring_mask[c->cur_slot] = c->profile_mask;
+        c->ring_mask[c->cur_slot] = c->profile_mask & kRebinStageMask;
     }
     c->last_profiled = c->profiled;
     return GS3D_OK;
@@ -1356,6 +1375,33 @@ int gs3d_abi_version(void) { return GS3D_ABI_VERSION; }
 
 const char* gs3d_last_error(const gs3d_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+uint32_t gs3d_reference_pass_stages(const char* label) {
+    if (!label) return 0u;
+    struct Row {
+        const char* label;
+        uint32_t stages;
+    };
+    const uint32_t sort = (1u << GS3D_STAGE_DEPTH_SORT) | (1u << GS3D_STAGE_SORT);
+    static const Row kRows[] = {
+        {"preprocess-splats", 1u << GS3D_STAGE_PREPROCESS},
+        {"prefix-scan-local", 1u << GS3D_STAGE_SCAN},
+        {"prefix-scan-blocks", 1u << GS3D_STAGE_SCAN},
+        {"prefix-scan-add", 1u << GS3D_STAGE_SCAN},
+        {"emit-tile-refs", 1u << GS3D_STAGE_EMIT},
+        {"radix-histogram", sort},
+        {"scan-local", sort},
+        {"scan-add", sort},
+        {"radix-scatter", sort},
+        {"identify-tile-ranges", 1u << GS3D_STAGE_IDENTIFY},
+        {"rasterize-tiles", 1u << GS3D_STAGE_RASTER},
+        {"tile-overlay", 1u << GS3D_STAGE_COMPOSITE},
+        {"composite-splats", 1u << GS3D_STAGE_COMPOSITE},
+    };
+    for (const Row& r : kRows)
+        if (!std::strcmp(label, r.label)) return r.stages;
+    return 0u;
+}
+
 const char* gs3d_stage_label(int stage) {
     return (stage >= 0 && stage < GS3D_STAGE__COUNT) ? kStageLabels[stage] : "";
 }
@@ -1391,6 +1437,11 @@ int gs3d_create(int device, gs3d_ctx** out) {
                     gs3d_destroy(c);
                     return GS3D_E_HIP;
                 }
+    for (int w = 0; w < 2; w++)
+        if (hipEventCreate(&c->composite_ev[w]) != hipSuccess) {
+            gs3d_destroy(c);
+            return GS3D_E_HIP;
+        }
     *out = c;
     return GS3D_OK;
 }
@@ -1405,6 +1456,8 @@ int gs3d_destroy(gs3d_ctx* c) {
         for (int s = 0; s < GS3D_STAGE__COUNT; s++)
             for (int w = 0; w < 2; w++)
                 if (c->ring[r][s][w]) (void)hipEventDestroy(c->ring[r][s][w]);
+    for (int w = 0; w < 2; w++)
+        if (c->composite_ev[w]) (void)hipEventDestroy(c->composite_ev[w]);
     destroy_batch(c);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1616,6 +1669,47 @@ int gs3d_ref_count(gs3d_ctx* c, uint64_t* out) {
     return GS3D_OK;
 }
 
+int gs3d_launch_geometry(const gs3d_ctx* c, const char* kernel, const uint64_t problem[3], uint32_t workgroup[3],
+                         uint64_t grid[3]) {
+    if (!kernel || !problem || !workgroup || !grid) return GS3D_E_INVALID_ARG;
+    const uint32_t tile = c ? c->tile_x : 16u;
+    const uint32_t ppl = c ? c->raster_ppl : 1u;
+    const uint32_t key_tile32 = c ? c->sort_key_tile32 : 0u;
+    const uint32_t key_tile = c ? tile_key_tile(c) : 16384u;
+    const uint64_t n64 = problem[0];
+    if (n64 > 0xFFFFFFFFull || problem[1] > 0xFFFFFFFFull) return GS3D_E_INVALID_ARG;
+    const uint32_t n = (uint32_t)n64;
+    const uint64_t tiles_x = (problem[0] + tile - 1) / tile, tiles_y = (problem[1] + tile - 1) / tile;
+    LaunchShape sh{};
+    uint64_t g[3] = {0, 1, 1};
+    if (!std::strcmp(kernel, "preprocess-splats")) {
+        sh = preprocess_launch_shape(n);
+    } else if (!std::strcmp(kernel, "radix-histogram")) {
+        sh = key_histogram_launch_shape(n);
+    } else if (!std::strcmp(kernel, "radix-sort-splats")) {
+        sh = radix_pass_launch_shape(4, key_tile32, n);
+    } else if (!std::strcmp(kernel, "prefix-scan")) {
+        sh = scan_rects_launch_shape(n);
+    } else if (!std::strcmp(kernel, "emit-tile-refs")) {
+        sh = emit_launch_shape(n);
+    } else if (!std::strcmp(kernel, "radix-sort")) {
+        sh = radix_pass_launch_shape(4, key_tile, n);
+    } else if (!std::strcmp(kernel, "identify-tile-ranges")) {
+        sh = identify_launch_shape((uint32_t)(tiles_x * tiles_y));
+    } else if (!std::strcmp(kernel, "rasterize-tiles")) {
+        sh = raster_launch_shape(tile, ppl, (uint32_t)(tiles_x * tiles_y));
+    } else if (!std::strcmp(kernel, "composite-splats")) {
+        sh = composite_launch_shape((uint32_t)(problem[0] * problem[1]));
+    } else {
+        return GS3D_E_UNKNOWN_NAME;
+    }
+    g[0] = sh.grid;
+    workgroup[0] = sh.block;
+    workgroup[1] = workgroup[2] = 1;
+    for (int i = 0; i < 3; i++) grid[i] = g[i];
+    return GS3D_OK;
+}
+
 int gs3d_sort_passes(gs3d_ctx* c, uint32_t* n_passes, uint32_t bit_offsets[16]) {
     if (!c || !n_passes) return GS3D_E_INVALID_ARG;
     *n_passes = c->n_passes;
@@ -1749,7 +1843,17 @@ int gs3d_composite(gs3d_ctx* c, int format, const float* clear_rgba, uint8_t* fr
     p.clear = clear_rgba ? pack_unorm8_clear(clear_rgba, p.bgra) : 0u;
     p.debug = c->debug_mode == 1;
     if (p.debug) GS3D_TRY(ensure_t(c, "debug_target", px * 4, &p.debug_target));
+    // timed like a rebin stage when the frame's rebin was profiled
+    const bool timed = c->last_profiled && (c->profile_mask >> GS3D_STAGE_COMPOSITE & 1u);
+    if (timed) {
+        GS3D_TRY(fold_composite(c));
+        HIP_TRY(c, hipEventRecord(c->composite_ev[0], c->stream));
+    }
     HIP_TRY(c, launch_composite(p, c->stream));
+    if (timed) {
+        HIP_TRY(c, hipEventRecord(c->composite_ev[1], c->stream));
+        c->composite_pending = true;
+    }
     c->framebuffer_px = px;
     if (p.debug) c->debug_target_px = px;
     if (framebuffer) {
@@ -1765,11 +1869,12 @@ int gs3d_stage_times_us(gs3d_ctx* c, float* out, int n) {
     GS3D_TRY(set_device(c));
     GS3D_TRY(settle(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    GS3D_TRY(fold_composite(c));
     for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++) {
         float ms = 0.f;
         if (c->ring_mask[c->cur_slot] >> s & 1u)
             HIP_TRY(c, hipEventElapsedTime(&ms, c->ring[c->cur_slot][s][0], c->ring[c->cur_slot][s][1]));
-        out[s] = ms * 1000.0f;
+        out[s] = s == GS3D_STAGE_COMPOSITE ? c->composite_last_us : ms * 1000.0f;
     }
     return GS3D_OK;
 }
@@ -1779,6 +1884,8 @@ int gs3d_profiler_reset(gs3d_ctx* c) {
     GS3D_TRY(set_device(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (int r = 0; r < gs3d_ctx::kRing; r++) c->ring_unread[r] = false;
+    c->composite_pending = false;
+    c->composite_last_us = 0.f;
     for (int s = 0; s < GS3D_STAGE__COUNT; s++) {
         c->stage_sum_us[s] = 0.0;
         c->stage_count[s] = 0;
@@ -1798,6 +1905,7 @@ int gs3d_stage_times_avg_us(gs3d_ctx* c, float* out, int n, uint64_t* frames) {
     GS3D_TRY(set_device(c));
     GS3D_TRY(settle(c));
     for (int r = 0; r < gs3d_ctx::kRing; r++) GS3D_TRY(fold_slot(c, r));
+    GS3D_TRY(fold_composite(c));
     if (frames) *frames = c->stage_frames;
     for (int s = 0; s < GS3D_STAGE__COUNT && s < n; s++)
         out[s] = c->stage_count[s] ? (float)(c->stage_sum_us[s] / (double)c->stage_count[s]) : 0.0f;

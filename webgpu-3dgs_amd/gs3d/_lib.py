@@ -48,7 +48,36 @@ REBIN_PROFILE = 1
 REBIN_KEEP_UNSORTED = 2
 
 STAGES = ("preprocess-splats", "radix-sort-splats", "prefix-scan", "emit-tile-refs", "radix-sort",
-          "identify-tile-ranges", "rasterize-tiles", "rebin-total")
+          "identify-tile-ranges", "rasterize-tiles", "rebin-total", "composite-splats")
+# the stages of one rebin (without the whole-rebin bracket and the display pass)
+REBIN_STAGES = STAGES[:7]
+
+# The pass labels of the reference's Profiler table, in its execution order
+# (GaussianSplatRenderer.ts:1017-1260, Profiler.ts:135-228).
+REFERENCE_PASSES = ("preprocess-splats", "prefix-scan-local", "prefix-scan-blocks", "prefix-scan-add",
+                    "emit-tile-refs", "radix-histogram", "scan-local", "scan-add", "radix-scatter",
+                    "identify-tile-ranges", "rasterize-tiles", "tile-overlay", "composite-splats")
+
+
+def reference_pass_stages(label: str) -> tuple[str, ...]:
+    """The stages (STAGES labels) doing the work of a reference pass
+    (gs3d_reference_pass_stages)."""
+    mask = lib().gs3d_reference_pass_stages(label.encode())
+    return tuple(s for i, s in enumerate(STAGES) if mask >> i & 1)
+
+
+def reference_pass_table(stage_us: dict) -> list[tuple[list[str], str, float]]:
+    """The reference's pass labels grouped by the stage(s) doing their work, in
+    execution order: [(reference labels, "stage[+stage]", microseconds)]."""
+    rows: list[tuple[list[str], str, float]] = []
+    for label in REFERENCE_PASSES:
+        stages = reference_pass_stages(label)
+        key = "+".join(stages)
+        if rows and rows[-1][1] == key:
+            rows[-1][0].append(label)
+        else:
+            rows.append(([label], key, sum(stage_us.get(s, 0.0) for s in stages)))
+    return rows
 
 P = C.c_void_p
 U32 = C.c_uint32
@@ -92,6 +121,8 @@ _PROTOS = {
     "gs3d_stage_emit": (I32, [P, P, P, U32, U32, P, P]),
     "gs3d_stage_sort": (I32, [P, P, P, U32, U32, P, P]),
     "gs3d_sort_pairs_noalloc": (I32, [P, P, P, U32, U32, P, P]),
+    "gs3d_launch_geometry": (I32, [P, C.c_char_p, P, P, P]),
+    "gs3d_reference_pass_stages": (U32, [C.c_char_p]),
     "gs3d_stage_identify": (I32, [P, P, U32, U32, P]),
     "gs3d_stage_rasterize": (I32, [P, P, U32, P, U32, P, U32, U32, U32, U32, P]),
     "gs3d_set_debug_mode": (I32, [P, U32, U32]),
@@ -157,3 +188,18 @@ def ptr(a) -> int | None:
     if not a.flags["C_CONTIGUOUS"]:
         raise ValueError("array must be C-contiguous")
     return a.ctypes.data
+
+
+def launch_geometry(kernel: str, problem, handle=None) -> tuple[tuple[int, int, int], tuple[int, int, int]]:
+    """(threads per workgroup, workgroups) of one launch of `kernel` over `problem`
+    (gs3d_launch_geometry); handle None: the default variants."""
+    import numpy as np
+
+    p = np.ones(3, np.uint64)
+    p[:len(problem)] = problem
+    wg = np.zeros(3, np.uint32)
+    grid = np.zeros(3, np.uint64)
+    rc = lib().gs3d_launch_geometry(handle, kernel.encode(), ptr(p), ptr(wg), ptr(grid))
+    if rc != 0:
+        raise Gs3dError(rc, f"no pass named {kernel!r}")
+    return tuple(int(x) for x in wg), tuple(int(x) for x in grid)

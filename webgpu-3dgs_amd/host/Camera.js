@@ -1,49 +1,48 @@
 'use strict';
-// Camera with the reference's API and 288-byte uniform block
-// (src/Camera.ts: constructor :33-67, updateView/setProjection :69-79,
-// setPosition/setTarget :81-91, rotate/pan/zoom :93-170, getUniformData :172-191).
+// The host camera: the view and projection one rebin needs, and the 288-byte
+// CameraUniforms block that preprocess reads (the block's layout is the
+// reference's Camera.getUniformData, src/Camera.ts:172-191, read by
+// preprocess_splats.wgsl:1-8). The arithmetic is gs3d/camera.py's: a look-at
+// view, an OpenGL-style perspective (NDC z in [-1, 1]) and both inverses,
+// computed in doubles and stored as float32. `version` changes with every
+// change of view or projection; GaussianSplatRenderer.render compares it to
+// skip re-binning a still camera.
+//
+// Mouse-driven orbit / pan / zoom belongs to the application around the
+// renderer; setOrbit places the camera on a sphere around its target, the
+// camera placement the workloads and the bench use.
 
-const { vec3, quat, mat4 } = require('./glmath');
+const { mat4 } = require('./glmath');
+
+const vec = (v) => Float32Array.of(v[0], v[1], v[2]);
 
 class Camera {
   constructor(position, target, up, fov, aspect, near, far) {
-    this.position = vec3.clone(position);
-    this.target = vec3.clone(target);
-    this.worldUp = vec3.clone(up);
-    this.up = vec3.clone(up);
-    this.right = vec3.create();
+    this.position = vec(position);
+    this.target = vec(target);
+    this.up = vec(up);
     this.fov = fov;
     this.aspect = aspect;
     this.near = near;
     this.far = far;
-    this.projectionMatrix = mat4.create();
-    this.viewMatrix = mat4.create();
-    this.inverseProjectionMatrix = mat4.create();
-    this.inverseViewMatrix = mat4.create();
-    // bumped on every view/projection change; renderers skip work while it is unchanged
+    this.viewMatrix = new Float32Array(16);
+    this.inverseViewMatrix = new Float32Array(16);
+    this.projectionMatrix = new Float32Array(16);
+    this.inverseProjectionMatrix = new Float32Array(16);
     this.version = 0;
-    this.forward = vec3.create();
-    vec3.sub(this.forward, this.target, this.position);
-    vec3.normalize(this.forward, this.forward);
-    this.orientation = quat.create();
-    const dir = vec3.normalize(vec3.create(), vec3.sub(vec3.create(), this.target, this.position));
-    quat.rotationTo(this.orientation, vec3.fromValues(0, 0, -1), dir);
-    this.rotationSpeed = 0.01;
-    this.panSpeed = 0.02;
-    this.zoomSpeed = 0.01;
     this.updateView();
     this.setProjection();
   }
 
   updateView() {
     mat4.lookAt(this.viewMatrix, this.position, this.target, this.up);
-    mat4.invert(this.inverseViewMatrix, this.viewMatrix);
+    if (!mat4.invert(this.inverseViewMatrix, this.viewMatrix)) this.inverseViewMatrix.fill(0);
     this.version++;
   }
 
   setProjection() {
     mat4.perspective(this.projectionMatrix, this.fov, this.aspect, this.near, this.far);
-    mat4.invert(this.inverseProjectionMatrix, this.projectionMatrix);
+    if (!mat4.invert(this.inverseProjectionMatrix, this.projectionMatrix)) this.inverseProjectionMatrix.fill(0);
     this.version++;
   }
 
@@ -53,80 +52,42 @@ class Camera {
   }
 
   setPosition(position) {
-    vec3.copy(this.position, position);
-    this.updateVectors();
+    this.position = vec(position);
     this.updateView();
   }
 
   setTarget(target) {
-    vec3.copy(this.target, target);
-    this.updateVectors();
+    this.target = vec(target);
     this.updateView();
   }
 
-  rotate(deltaX, deltaY) {
-    const yaw = quat.setAxisAngle(quat.create(), vec3.fromValues(0, 1, 0), deltaX * this.rotationSpeed);
-    const right = vec3.normalize(vec3.create(), vec3.cross(vec3.create(), this.up, this.forward));
-    const pitch = quat.setAxisAngle(quat.create(), right, -deltaY * this.rotationSpeed);
-    const rotation = quat.create();
-    quat.multiply(rotation, rotation, pitch);
-    quat.multiply(rotation, yaw, rotation);
-    quat.normalize(this.orientation, rotation);
-    const offset = vec3.sub(vec3.create(), this.position, this.target);
-    const rotated = vec3.transformQuat(vec3.create(), offset, this.orientation);
-    vec3.add(this.position, this.target, rotated);
-    vec3.transformQuat(this.up, this.up, this.orientation);
-    vec3.transformQuat(this.forward, this.forward, this.orientation);
+  // On the sphere of `radius` around the target: azimuth about +y measured
+  // from +z towards +x, elevation above the xz-plane (degrees); the same
+  // placement as gs3d.scene.orbit_eye around the origin.
+  setOrbit(radius, azimuthDeg, elevationDeg) {
+    const az = (azimuthDeg * Math.PI) / 180;
+    const el = (elevationDeg * Math.PI) / 180;
+    const t = this.target;
+    this.position = Float32Array.of(t[0] + radius * Math.cos(el) * Math.sin(az), t[1] + radius * Math.sin(el),
+      t[2] + radius * Math.cos(el) * Math.cos(az));
     this.updateView();
   }
 
-  pan(deltaX, deltaY) {
-    const dir = vec3.normalize(vec3.create(), vec3.sub(vec3.create(), this.target, this.position));
-    const right = vec3.normalize(vec3.create(), vec3.cross(vec3.create(), this.worldUp, dir));
-    const up = vec3.normalize(vec3.create(), vec3.cross(vec3.create(), right, dir));
-    const offset = vec3.create();
-    vec3.scaleAndAdd(offset, offset, right, -deltaX * this.panSpeed);
-    vec3.scaleAndAdd(offset, offset, up, deltaY * this.panSpeed);
-    vec3.add(this.position, this.position, offset);
-    vec3.add(this.target, this.target, offset);
-    this.updateView();
-  }
-
-  zoom(deltaZoom) {
-    const dir = vec3.normalize(vec3.create(), vec3.sub(vec3.create(), this.target, this.position));
-    const distance = vec3.distance(this.position, this.target);
-    let amount = deltaZoom * this.zoomSpeed * (distance / 10);
-    if (distance - amount < this.near) amount = distance - this.near;
-    vec3.scale(dir, dir, amount);
-    vec3.add(this.position, this.position, dir);
-    this.updateView();
-  }
-
-  // 72 floats: view, projection, inverse view, inverse projection, position.xyz1, viewport
+  // CameraUniforms, 72 floats: view, projection, inverse view, inverse
+  // projection (column-major), position.xyz1, viewport (w, h, w / h, 0).
   getUniformData(viewportWidth, viewportHeight) {
-    const data = new Float32Array(72);
-    data.set(this.viewMatrix, 0);
-    data.set(this.projectionMatrix, 16);
-    data.set(this.inverseViewMatrix, 32);
-    data.set(this.inverseProjectionMatrix, 48);
-    data[64] = this.position[0];
-    data[65] = this.position[1];
-    data[66] = this.position[2];
-    data[67] = 1.0;
-    data[68] = viewportWidth;
-    data[69] = viewportHeight;
-    data[70] = viewportWidth / viewportHeight;
-    data[71] = 0.0;
-    return data;
-  }
-
-  updateVectors() {
-    vec3.sub(this.forward, this.target, this.position);
-    vec3.normalize(this.forward, this.forward);
-    vec3.cross(this.right, this.forward, this.worldUp);
-    vec3.normalize(this.right, this.right);
-    vec3.cross(this.up, this.right, this.forward);
-    vec3.normalize(this.up, this.up);
+    const u = new Float32Array(72);
+    u.set(this.viewMatrix, 0);
+    u.set(this.projectionMatrix, 16);
+    u.set(this.inverseViewMatrix, 32);
+    u.set(this.inverseProjectionMatrix, 48);
+    u.set(this.position, 64);
+    u[67] = 1;
+    u[68] = viewportWidth;
+    u[69] = viewportHeight;
+    u[70] = viewportWidth / viewportHeight;
+    u[71] = 0;
+    return u;
   }
 }
 

@@ -7,7 +7,7 @@
 // composite, :1075-1097). The passes run in libgs3d.so on one HIP stream;
 // this class keeps the host-side state machine and the resource registries.
 
-const { WorkgroupManager, linear1D, tiled1D, tile2D, BufferManager, PipelineManager } = require('./managers');
+const { WorkgroupManager, BufferManager, PipelineManager } = require('./managers');
 
 const REBIN_PROFILE = 1;
 const BATCH_EAGER = 1;
@@ -20,6 +20,9 @@ const BUF = {
 // Framebuffer byte orders of composite (the swapchain format handed to init).
 const FORMATS = { rgba8unorm: 0, bgra8unorm: 1 };
 const MAX_TILES_PER_SPLAT = 16;  // Config.MAX_TILES_PER_SPLAT: the reference's key budget per splat
+// passes sized by splat count, by key count and by viewport
+const SPLAT_PASSES = ['preprocess-splats', 'radix-histogram', 'radix-sort-splats', 'prefix-scan', 'emit-tile-refs'];
+const PIXEL_PASSES = ['identify-tile-ranges', 'rasterize-tiles', 'composite-splats'];
 
 function loadAddon() {
   try {
@@ -52,14 +55,22 @@ class GaussianSplatRenderer {
     this.rebinCount = 0;
   }
 
-  // Launch-geometry layouts of the passes (registerBinningLayouts, :127-158).
+  // The passes' launch layouts (registerBinningLayouts, :127-158): the
+  // library's own launch geometry for the current problem sizes. The key
+  // count starts at the reference's 16-per-splat budget and follows the last
+  // rebin's K when launchLayouts() is read.
   registerBinningLayouts(splatCount) {
-    const maxRefs = Math.max(1, Math.max(1, splatCount) * MAX_TILES_PER_SPLAT);
-    this.workgroups.register({ name: 'splat-1d', problemSize: [splatCount, 1, 1], strategyFn: linear1D, strategyArgs: [256] });
-    this.workgroups.register({ name: 'radix', problemSize: [maxRefs, 1, 1], strategyFn: linear1D, strategyArgs: [256, 16] });
-    this.workgroups.register({ name: 'scan', problemSize: [splatCount, 1, 1], strategyFn: tiled1D, strategyArgs: [8, 4, 256] });
+    for (const name of SPLAT_PASSES) this.workgroups.register({ name, problemSize: [splatCount] });
+    this.workgroups.register({ name: 'radix-sort', problemSize: [Math.max(1, splatCount) * MAX_TILES_PER_SPLAT] });
     [this.tileSizeX, this.tileSizeY] = this.clampTileSize(this.scene.tiles[0], this.scene.tiles[1]);
-    this.workgroups.register({ name: 'rasterize', problemSize: [1, 1, 1], strategyFn: tile2D, strategyArgs: [this.tileSizeX, this.tileSizeY] });
+    for (const name of PIXEL_PASSES) this.workgroups.register({ name, problemSize: [1, 1] });
+  }
+
+  // name -> {workgroupSize, dispatchSize} of every pass, as the library launches
+  // them for the last frame (waits for the last rebin to read its key count).
+  launchLayouts() {
+    if (this.rebinCount > 0) this.workgroups.update('radix-sort', { problemSize: [this.refCount()] });
+    return this.workgroups.layouts();
   }
 
   clampTileSize(x, y) {
@@ -103,7 +114,6 @@ class GaussianSplatRenderer {
     this.tileSizeX = tx;
     this.tileSizeY = ty;
     this.scene.tiles = [tx, ty];
-    this.workgroups.update('rasterize', { strategyArgs: [tx, ty] });
     this.lastViewport = [-1, -1];  // force the tile grid + target to be recomputed
     return [tx, ty];
   }
@@ -136,11 +146,14 @@ class GaussianSplatRenderer {
     this.tileCountY = tilesY;
     const splatCountChanged = splatCount !== this.lastSplatCount;
     if (splatCountChanged) {
-      this.workgroups.update('splat-1d', { problemSize: [splatCount, 1, 1] });
+      for (const name of SPLAT_PASSES) this.workgroups.update(name, { problemSize: [splatCount] });
       this.lastSplatCount = splatCount;
     }
     const viewportChanged = width !== this.lastViewport[0] || height !== this.lastViewport[1];
-    if (viewportChanged) this.lastViewport = [width, height];
+    if (viewportChanged) {
+      this.lastViewport = [width, height];
+      for (const name of PIXEL_PASSES) this.workgroups.update(name, { problemSize: [width, height] });
+    }
     const version = frame.cameraVersion !== undefined ? frame.cameraVersion : frame.camera.version;
     const cameraMoved = version !== this.lastCameraVersion;
     this.lastCameraVersion = version;

@@ -1,86 +1,12 @@
 'use strict';
-// Small column-major matrix / vector / quaternion helpers for the host Camera.
+// Column-major 4x4 matrix helpers for the host Camera.
 // The reference builds its camera with gl-matrix 3.4.4 (src/Camera.ts:1); the
 // functions here are written from the textbook definitions of the same
 // operations (look-at view matrix, OpenGL-style perspective with NDC z in
-// [-1, 1], cofactor inverse, Hamilton quaternion product). Arithmetic is in
+// [-1, 1], cofactor inverse). Arithmetic is in
 // JS doubles; results are stored in Float32Array like gl-matrix's defaults.
 
 const EPSILON = 1e-6;
-
-const vec3 = {
-  create: () => new Float32Array(3),
-  clone: (a) => Float32Array.from(a),
-  fromValues: (x, y, z) => Float32Array.of(x, y, z),
-  copy: (o, a) => { o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; return o; },
-  add: (o, a, b) => { o[0] = a[0] + b[0]; o[1] = a[1] + b[1]; o[2] = a[2] + b[2]; return o; },
-  sub: (o, a, b) => { o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2]; return o; },
-  scale: (o, a, s) => { o[0] = a[0] * s; o[1] = a[1] * s; o[2] = a[2] * s; return o; },
-  scaleAndAdd: (o, a, b, s) => {
-    o[0] = a[0] + b[0] * s; o[1] = a[1] + b[1] * s; o[2] = a[2] + b[2] * s; return o;
-  },
-  dot: (a, b) => a[0] * b[0] + a[1] * b[1] + a[2] * b[2],
-  cross: (o, a, b) => {
-    const ax = a[0], ay = a[1], az = a[2], bx = b[0], by = b[1], bz = b[2];
-    o[0] = ay * bz - az * by; o[1] = az * bx - ax * bz; o[2] = ax * by - ay * bx; return o;
-  },
-  length: (a) => Math.hypot(a[0], a[1], a[2]),
-  distance: (a, b) => Math.hypot(b[0] - a[0], b[1] - a[1], b[2] - a[2]),
-  normalize: (o, a) => {
-    const l2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-    const inv = l2 > 0 ? 1 / Math.sqrt(l2) : 0;
-    o[0] = a[0] * inv; o[1] = a[1] * inv; o[2] = a[2] * inv; return o;
-  },
-  // rotate v by unit quaternion q = (x, y, z, w): v + 2w(u x v) + 2 u x (u x v)
-  transformQuat: (o, v, q) => {
-    const qx = q[0], qy = q[1], qz = q[2], qw = q[3];
-    const x = v[0], y = v[1], z = v[2];
-    let tx = qy * z - qz * y, ty = qz * x - qx * z, tz = qx * y - qy * x;
-    tx *= 2; ty *= 2; tz *= 2;
-    o[0] = x + qw * tx + (qy * tz - qz * ty);
-    o[1] = y + qw * ty + (qz * tx - qx * tz);
-    o[2] = z + qw * tz + (qx * ty - qy * tx);
-    return o;
-  },
-};
-
-const quat = {
-  create: () => Float32Array.of(0, 0, 0, 1),
-  setAxisAngle: (o, axis, rad) => {
-    const s = Math.sin(rad / 2);
-    o[0] = s * axis[0]; o[1] = s * axis[1]; o[2] = s * axis[2]; o[3] = Math.cos(rad / 2);
-    return o;
-  },
-  multiply: (o, a, b) => {  // Hamilton product a * b
-    const ax = a[0], ay = a[1], az = a[2], aw = a[3];
-    const bx = b[0], by = b[1], bz = b[2], bw = b[3];
-    o[0] = aw * bx + ax * bw + ay * bz - az * by;
-    o[1] = aw * by + ay * bw + az * bx - ax * bz;
-    o[2] = aw * bz + az * bw + ax * by - ay * bx;
-    o[3] = aw * bw - ax * bx - ay * by - az * bz;
-    return o;
-  },
-  normalize: (o, a) => {
-    const l = Math.hypot(a[0], a[1], a[2], a[3]);
-    const inv = l > 0 ? 1 / l : 0;
-    for (let i = 0; i < 4; i++) o[i] = a[i] * inv;
-    return o;
-  },
-  // shortest-arc rotation taking unit vector a onto unit vector b
-  rotationTo: (o, a, b) => {
-    const d = vec3.dot(a, b);
-    if (d < -1 + EPSILON) {
-      const axis = vec3.cross(new Float64Array(3), [1, 0, 0], a);
-      if (vec3.length(axis) < EPSILON) vec3.cross(axis, [0, 1, 0], a);
-      vec3.normalize(axis, axis);
-      return quat.setAxisAngle(o, axis, Math.PI);
-    }
-    if (d > 1 - EPSILON) { o[0] = 0; o[1] = 0; o[2] = 0; o[3] = 1; return o; }
-    const c = vec3.cross(new Float64Array(3), a, b);
-    o[0] = c[0]; o[1] = c[1]; o[2] = c[2]; o[3] = 1 + d;
-    return quat.normalize(o, o);
-  },
-};
 
 const mat4 = {
   create: () => { const m = new Float32Array(16); m[0] = m[5] = m[10] = m[15] = 1; return m; },
@@ -148,4 +74,4 @@ const mat4 = {
   },
 };
 
-module.exports = { vec3, quat, mat4, EPSILON };
+module.exports = { mat4, EPSILON };

@@ -8,6 +8,11 @@ const managers = require('./managers');
 const loader = require('./SceneLoader');
 const { GaussianSplatRenderer, loadAddon, BUF, REBIN_PROFILE } = require('./GaussianSplatRenderer');
 
+// The pass labels of the reference's Profiler table, in its execution order.
+const REFERENCE_PASSES = ['preprocess-splats', 'prefix-scan-local', 'prefix-scan-blocks', 'prefix-scan-add',
+  'emit-tile-refs', 'radix-histogram', 'scan-local', 'scan-add', 'radix-scatter', 'identify-tile-ranges',
+  'rasterize-tiles', 'tile-overlay', 'composite-splats'];
+
 // Scene (src/Scene.ts): tile pixel size, splats, bounds.
 class Scene {
   constructor() {
@@ -56,6 +61,25 @@ class Profiler {
     return m;
   }
 
+  // The reference's table (its pass labels, GaussianSplatRenderer.ts:1017-1260):
+  // rows of [reference labels, the stage(s) doing their work, microseconds],
+  // in execution order; passes fused into one stage here share a row.
+  referenceTimings() {
+    const { addon } = this.renderer;
+    const t = this.timings();
+    const labels = [...t.keys()];
+    const rows = [];
+    for (const pass of REFERENCE_PASSES) {
+      const mask = addon.referencePassStages(pass);
+      const stages = labels.filter((_, i) => (mask >>> i) & 1);
+      const key = stages.join('+');
+      const last = rows[rows.length - 1];
+      if (last && last.stages === key) last.passes.push(pass);
+      else rows.push({ passes: [pass], stages: key, us: stages.reduce((s, l) => s + t.get(l), 0) });
+    }
+    return rows;
+  }
+
   getBuffersSortedBySize() {
     return this.renderer.addon.bufferList(this.renderer.ctx);
   }
@@ -66,6 +90,6 @@ class Profiler {
 }
 
 module.exports = {
-  Camera, Scene, Profiler, GaussianSplatRenderer, loadAddon, BUF, REBIN_PROFILE,
+  Camera, Scene, Profiler, GaussianSplatRenderer, loadAddon, BUF, REBIN_PROFILE, REFERENCE_PASSES,
   ...managers, ...loader, glmath,
 };

@@ -1,84 +1,64 @@
 'use strict';
-// Host-side resource services with the reference's names and semantics:
-//   WorkgroupManager + linear1D/tiled1D/tile2D  (src/WorkgroupManager.ts:10-147)
-//   BufferManager                               (src/BufferManager.ts:13-180)
-//   PipelineManager                             (src/PipelineManager.ts:27-117)
-// Device memory and kernels live in libgs3d.so; these classes keep the
-// reference's registry-by-name API over the N-API addon.
+// The renderer's resource registries, over libgs3d.so through the N-API addon.
+// They keep the names the reference's renderer uses for them
+// (src/WorkgroupManager.ts, src/BufferManager.ts, src/PipelineManager.ts);
+// what each one holds is this library's:
+//   WorkgroupManager  launch geometry. The library owns its launches; a
+//                     registered pass reports the workgroup and dispatch sizes
+//                     the library launches for its problem size and the
+//                     context's tile size and kernel variants.
+//   BufferManager     the context's named device buffers.
+//   PipelineManager   the choice of ahead-of-time compiled kernel variants (the
+//                     reference compiles WGSL with __NAME__ constants
+//                     substituted in; here each constant picks a variant).
 
 // ---- WorkgroupManager -------------------------------------------------------
 
-const highestPowerOfTwoAtMost = (v) => 2 ** Math.floor(Math.log2(Math.max(1, v)));
-
-const linear1D = (preferredSize = 256, elementsPerThread = 1) => ({ limits, problemSize }) => {
-  const x = highestPowerOfTwoAtMost(Math.min(preferredSize, limits.maxTotalThreads, limits.maxSizeX));
-  return { workgroupSize: [x, 1, 1], dispatchSize: [Math.max(1, Math.ceil(problemSize[0] / (elementsPerThread * x))), 1, 1] };
-};
-
-const tile2D = (tileX, tileY) => ({ problemSize }) => ({
-  workgroupSize: [tileX, tileY, 1],
-  dispatchSize: [Math.max(1, Math.ceil(problemSize[0] / tileX)), Math.max(1, Math.ceil(problemSize[1] / tileY)), 1],
-});
-
-const tiled1D = (elementsPerThread, bytesPerElement, preferredSize = Infinity) => ({ limits, problemSize }) => {
-  let x = highestPowerOfTwoAtMost(Math.min(preferredSize, limits.maxTotalThreads, limits.maxSizeX));
-  while (x > 1 && elementsPerThread * x * bytesPerElement > limits.maxSharedMemory) x /= 2;
-  return { workgroupSize: [x, 1, 1], dispatchSize: [Math.max(1, Math.ceil(problemSize[0] / (elementsPerThread * x))), 1, 1] };
-};
+const LIMIT_NAMES = ['maxTotalThreads', 'maxSizeX', 'maxSizeY', 'maxSizeZ', 'maxDispatch', 'maxSharedMemory'];
 
 class WorkgroupManager {
-  // limits: {maxTotalThreads, maxSizeX, maxSizeY, maxSizeZ, maxDispatch, maxSharedMemory}
-  constructor(limits) {
-    this.limits = Object.freeze({ ...limits });
-    this.strategies = new Map();
-    this.layouts = new Map();
+  // ctx null: the library's default variants (no device needed).
+  constructor(addon, ctx = null) {
+    this.addon = addon;
+    this.ctx = ctx;
+    this.passes = new Map();  // name -> {kernel, problemSize}
+    this.limits = null;
+    if (ctx) {
+      const l = addon.deviceLimits(ctx);
+      this.limits = Object.freeze(Object.fromEntries(LIMIT_NAMES.map((k, i) => [k, l[i]])));
+    }
   }
 
   static fromDevice(addon, ctx) {
-    const l = addon.deviceLimits(ctx);
-    return new WorkgroupManager({
-      maxTotalThreads: l[0], maxSizeX: l[1], maxSizeY: l[2], maxSizeZ: l[3], maxDispatch: l[4], maxSharedMemory: l[5],
-    });
+    return new WorkgroupManager(addon, ctx);
   }
 
-  register(config) {
-    this.strategies.set(config.name, { ...config });
-    this.layouts.set(config.name, this.computeLayout(config.problemSize, config.strategyFn(...config.strategyArgs)));
+  // A pass of the library by kernel name (gs3d_launch_geometry lists them) and
+  // its problem size: [splats] or [keys] for the 1-D passes, [width, height]
+  // for the per-pixel ones. An unknown kernel throws.
+  register({ name, kernel = name, problemSize }) {
+    const entry = { kernel, problemSize: Array.from(problemSize) };
+    this.addon.launchGeometry(this.ctx, entry.kernel, entry.problemSize);
+    this.passes.set(name, entry);
   }
 
-  update(name, updates) {
-    const config = this.strategies.get(name);
-    if (!config) throw new Error(`No strategy registered for '${name}'`);
-    if (updates.problemSize) config.problemSize = updates.problemSize;
-    if (updates.strategyArgs) config.strategyArgs = updates.strategyArgs;
-    this.layouts.set(name, this.computeLayout(config.problemSize, config.strategyFn(...config.strategyArgs)));
+  update(name, { problemSize }) {
+    const entry = this.passes.get(name);
+    if (!entry) throw new Error(`No workgroup layout registered for '${name}'`);
+    entry.problemSize = Array.from(problemSize);
   }
 
+  // {workgroupSize: [x, y, z], dispatchSize: [x, y, z]} as the library launches it now
   getLayout(name) {
-    const layout = this.layouts.get(name);
-    if (!layout) throw new Error(`No layout found for '${name}'`);
-    return layout;
+    const entry = this.passes.get(name);
+    if (!entry) throw new Error(`No workgroup layout registered for '${name}'`);
+    return this.addon.launchGeometry(this.ctx, entry.kernel, entry.problemSize);
   }
 
-  computeLayout(problemSize, strategy) {
-    let result;
-    try {
-      result = strategy({ limits: this.limits, problemSize });
-    } catch (err) {
-      if (err instanceof Error) throw new Error(`Invalid workgroup layout: ${err.message}`);
-      throw err;
-    }
-    const [x, y, z] = result.workgroupSize;
-    if (x > this.limits.maxSizeX || y > this.limits.maxSizeY || z > this.limits.maxSizeZ ||
-        x * y * z > this.limits.maxTotalThreads) {
-      throw new Error('Workgroup layout exceeds hardware limits.');
-    }
-    const d = result.dispatchSize ||
-      [Math.ceil(problemSize[0] / x), Math.ceil(problemSize[1] / y), Math.ceil(problemSize[2] / z)];
-    return {
-      workgroupSize: [x, y, z],
-      dispatchSize: d.map((v) => Math.min(v, this.limits.maxDispatch)),
-    };
+  layouts() {
+    const out = new Map();
+    for (const name of this.passes.keys()) out.set(name, this.getLayout(name));
+    return out;
   }
 }
 
@@ -160,87 +140,78 @@ class BufferManager {
 }
 
 // ---- PipelineManager --------------------------------------------------------
-// WGSL text substitution (__NAME__) becomes the choice of a kernel variant that
-// libgs3d.so compiled ahead of time. The registry lists the variants.
 
+// The compiled variants: per kernel, the values each constant may take.
 const KERNEL_VARIANTS = {
-  'preprocess-splats': { constants: {} },
-  'prefix-scan': { constants: {} },
-  'emit-tile-refs': { constants: {} },
-  'radix-sort': { constants: { RADIX_BITS: [4, 8], KEY_TILE: [0, 8192, 16384] } },
-  'identify-tile-ranges': { constants: {} },
-  'rasterize-tiles': {
-    constants: { TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4] },
-  },
+  'preprocess-splats': {},
+  'prefix-scan': {},
+  'emit-tile-refs': {},
+  'radix-sort': { RADIX_BITS: [4, 8], KEY_TILE: [0, 8192, 16384] },
+  'radix-sort-splats': { KEY_TILE: [0, 4096, 8192] },
+  'identify-tile-ranges': {},
+  'rasterize-tiles': { TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4] },
+  'rasterize-tiles-batch': { GROUPS: [...Array(17).keys()], WAVE_ITEMS: [0, 1], WG_PER_CU: [...Array(17).keys()] },
 };
 
-// Constants the library selects by gs3d_set_kernel_variant; the tile constants
-// follow gs3d_set_tile_size instead.
-const LIBRARY_CONSTANTS = new Set(['RADIX_BITS', 'KEY_TILE', 'PIXELS_PER_LANE']);
+// Constants that select a variant in the library (gs3d_set_kernel_variant);
+// the tile constants follow the renderer's tile size (gs3d_set_tile_size).
+const SELECTED_IN_LIBRARY = new Set(['RADIX_BITS', 'KEY_TILE', 'PIXELS_PER_LANE', 'GROUPS', 'WAVE_ITEMS', 'WG_PER_CU']);
 
-class PipelineManager {
-  // With an addon and context, creating or updating a pipeline also selects
-  // the matching compiled variant in the library.
-  constructor(addon = null, ctx = null) {
-    this.addon = addon;
-    this.ctx = ctx;
-    this.pipelines = new Map();
-    this.pipelineConfigs = new Map();
+function checkConstants(name, kernel, constants) {
+  const table = KERNEL_VARIANTS[kernel];
+  if (!table) throw new Error(`No compiled kernel for pipeline "${name}"`);
+  for (const [k, v] of Object.entries(constants)) {
+    if (!table[k] || !table[k].includes(v)) throw new Error(`Pipeline "${name}": no compiled variant with ${k}=${v}`);
   }
-
-  create(config) {
-    const kernel = config.kernel || config.name;
-    const variant = KERNEL_VARIANTS[kernel];
-    if (!variant) throw new Error(`No compiled kernel for pipeline "${config.name}"`);
-    const constants = { ...(config.codeConstants || {}) };
-    for (const [k, v] of Object.entries(constants)) {
-      const allowed = variant.constants[k];
-      if (!allowed || !allowed.includes(v)) {
-        throw new Error(`Pipeline "${config.name}": no compiled variant with ${k}=${v}`);
-      }
-    }
-    if ('TILE_X' in constants && constants.TILE_X !== constants.TILE_Y) {
-      throw new Error(`Pipeline "${config.name}": tiles are square (8, 16 or 32)`);
-    }
-    if (this.addon) {
-      for (const [k, v] of Object.entries(constants)) {
-        if (LIBRARY_CONSTANTS.has(k)) this.addon.setKernelVariant(this.ctx, kernel, k, v);
-      }
-    }
-    const pipeline = { label: `pipeline-${config.name}`, kernel, constants };
-    this.pipelines.set(config.name, pipeline);
-    this.pipelineConfigs.set(config.name, { ...config, codeConstants: constants });
-  }
-
-  applyShaderConstants(src, constants) {
-    let out = src;
-    for (const [k, v] of Object.entries(constants)) out = out.replace(new RegExp(`__${k}__`, 'g'), String(v));
-    return out;
-  }
-
-  update(name, updates) {
-    const old = this.pipelineConfigs.get(name);
-    if (!old) throw new Error(`No pipeline config found for "${name}"`);
-    this.create({
-      ...old, ...updates,
-      constants: { ...old.constants, ...updates.constants },
-      codeConstants: { ...old.codeConstants, ...updates.codeConstants },
-    });
-  }
-
-  get(name) {
-    return this.pipelines.get(name);
-  }
-
-  delete(name) {
-    return this.pipelines.delete(name);
-  }
-
-  clear() {
-    this.pipelines.clear();
+  if ('TILE_X' in constants && constants.TILE_X !== constants.TILE_Y) {
+    throw new Error(`Pipeline "${name}": tiles are square (8, 16 or 32)`);
   }
 }
 
-module.exports = {
-  WorkgroupManager, linear1D, tiled1D, tile2D, BufferManager, PipelineManager, KERNEL_VARIANTS,
-};
+class PipelineManager {
+  // With an addon and context, a pipeline's constants are applied to the
+  // library when it is created or updated.
+  constructor(addon = null, ctx = null) {
+    this.addon = addon;
+    this.ctx = ctx;
+    this.selections = new Map();  // name -> {label, kernel, constants}
+  }
+
+  create({ name, kernel = name, codeConstants = {} }) {
+    const constants = { ...codeConstants };
+    checkConstants(name, kernel, constants);
+    if (this.addon && this.ctx) {
+      for (const [k, v] of Object.entries(constants)) {
+        if (SELECTED_IN_LIBRARY.has(k)) this.addon.setKernelVariant(this.ctx, kernel, k, v);
+      }
+    }
+    const selection = { label: `pipeline-${name}`, kernel, constants };
+    this.selections.set(name, selection);
+    return selection;
+  }
+
+  // New constants merged over the pipeline's current ones.
+  update(name, { codeConstants = {} } = {}) {
+    const current = this.selections.get(name);
+    if (!current) throw new Error(`No pipeline config found for "${name}"`);
+    return this.create({ name, kernel: current.kernel, codeConstants: { ...current.constants, ...codeConstants } });
+  }
+
+  get(name) {
+    return this.selections.get(name);
+  }
+
+  has(name) {
+    return this.selections.has(name);
+  }
+
+  delete(name) {
+    return this.selections.delete(name);
+  }
+
+  clear() {
+    this.selections.clear();
+  }
+}
+
+module.exports = { WorkgroupManager, BufferManager, PipelineManager, KERNEL_VARIANTS };
