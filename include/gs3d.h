@@ -322,6 +322,23 @@ int gs3d_learned_grad(gs3d_ctx* ctx, uint32_t first_dir, float* grad_params,
                       float* grad_latents, float* loss);
 int gs3d_learned_get(gs3d_ctx* ctx, float* params, float* latents, uint16_t* latents_f16);
 int gs3d_learned_set(gs3d_ctx* ctx, const float* params, const float* latents);
+/* The whole training state, to checkpoint and resume training: the
+ * parameters and latents (layouts of gs3d_learned_get), the Adam first and
+ * second moments of each, and the Adam step count (bias correction). Any
+ * array pointer may be NULL to skip that array. Setting a state needs
+ * gs3d_learned_init on the same splats first; it re-derives the f16 latent
+ * copy and leaves learned colours disabled until gs3d_learned_validate. */
+typedef struct gs3d_learned_state {
+    float* params;        /* GS3D_LEARNED_PARAMS */
+    float* params_m;
+    float* params_v;
+    float* latents;       /* N x 8 */
+    float* latents_m;
+    float* latents_v;
+    uint32_t step;        /* Adam steps taken */
+} gs3d_learned_state;
+int gs3d_learned_get_state(gs3d_ctx* ctx, gs3d_learned_state* state);
+int gs3d_learned_set_state(gs3d_ctx* ctx, const gs3d_learned_state* state);
 /* The 256 training directions ([256][3]). */
 int gs3d_learned_dirs(float* dirs);
 /* Learned colours of every splat seen from `eye` (N x 3 f32). */

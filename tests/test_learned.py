@@ -301,3 +301,40 @@ def test_workload_3m_sh3_32_views_learned(renderer):
     same = all(np.array_equal(renderer.batch_read(i, L.BUF_IMAGE_RGBA16F, np.uint16), exact[i])
                for i in range(len(views)))
     assert same == (not enabled)
+
+
+@pytest.mark.gpu
+def test_checkpoint_resume_is_exact(renderer):
+    """gs3d_learned_get_state / set_state: training resumed from a checkpoint (parameters,
+    latents, both Adam moments and the step) continues exactly as uninterrupted training
+    (the training kernels are deterministic: no atomics); resuming without the moments
+    does not. Setting a state leaves the moments as set (no decay)."""
+    say("learned: checkpoint and resume")
+    s = S.activate(S.clustered(23, 3000, 6, sh_degree=3))
+    renderer.set_splat_data(s)
+    renderer.learned_init(seed=7)
+    renderer.learned_train(2, lr=3e-3, lr_latent=3e-2)
+    ckpt = renderer.learned_get_state()
+    assert ckpt["step"] == 2 * 16
+    assert np.abs(ckpt["params_m"]).max() > 0 and np.abs(ckpt["latents_v"]).max() > 0
+    renderer.learned_train(2, lr=3e-3, lr_latent=3e-2)
+    straight = renderer.learned_get_state()
+    renderer.learned_init(seed=99)  # a different state, then the checkpoint
+    renderer.learned_set_state(ckpt)
+    back = renderer.learned_get_state()
+    for k in renderer.STATE_ARRAYS:
+        assert np.array_equal(back[k], ckpt[k]), k
+    assert back["step"] == ckpt["step"]
+    _, _, z16 = renderer.learned_get()
+    assert np.array_equal(z16, ckpt["latents"].astype(np.float16).view(np.uint16))
+    renderer.learned_train(2, lr=3e-3, lr_latent=3e-2)
+    resumed = renderer.learned_get_state()
+    for k in renderer.STATE_ARRAYS:
+        assert np.array_equal(resumed[k], straight[k]), k
+    assert resumed["step"] == straight["step"] == 4 * 16
+    # the same resume without the moments and the step diverges
+    renderer.learned_init(seed=99)
+    renderer.learned_set(ckpt["params"], ckpt["latents"])
+    renderer.learned_train(2, lr=3e-3, lr_latent=3e-2)
+    p, _, _ = renderer.learned_get()
+    assert not np.array_equal(p, straight["params"])

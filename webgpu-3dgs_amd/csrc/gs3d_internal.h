@@ -353,6 +353,8 @@ hipError_t launch_learned_reduce(const float* partial, uint32_t rows, float* gra
                                  float* loss, float* scratch, hipStream_t s);
 hipError_t launch_adam(float* x, float* m, float* v, const float* grad, size_t n, float lr, float b1, float b2,
                        float eps, uint32_t step, uint16_t* f16_copy, hipStream_t s);
+// f16_copy[i] = x[i] rounded to binary16 (the latents' f16 copy)
+hipError_t launch_f16_copy(const float* x, size_t n, uint16_t* f16_copy, hipStream_t s);
 // sum over pixels of the squared RGB differences of two f16 RGBA images (accumulated into *out, f64)
 hipError_t launch_sq_diff_sum(const uint16_t* a, const uint16_t* b, size_t pixels, double* out, hipStream_t s);
 

@@ -506,6 +506,11 @@ __global__ __launch_bounds__(256) void k_adam(float* x, float* m, float* v, cons
     }
 }
 
+__global__ __launch_bounds__(256) void k_f16_copy(const float* x, size_t n, uint16_t* f16_copy) {
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u)
+        f16_copy[i] = f32_to_f16_bits(x[i]);
+}
+
 // *out += sum over pixels of (a - b)^2 over the RGB channels of f16 RGBA images.
 __global__ __launch_bounds__(256) void k_sq_diff_sum(const uint16_t* a, const uint16_t* b, size_t pixels,
                                                     double* out) {
@@ -579,6 +584,13 @@ hipError_t launch_adam(float* x, float* m, float* v, const float* grad, size_t n
     const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_adam, dim3((uint32_t)blocks), dim3(256), 0, s, x, m, v, grad, n, lr, b1, b2, eps, c1, c2,
                        f16_copy);
+    return hipGetLastError();
+}
+
+hipError_t launch_f16_copy(const float* x, size_t n, uint16_t* f16_copy, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_f16_copy, dim3((uint32_t)blocks), dim3(256), 0, s, x, n, f16_copy);
     return hipGetLastError();
 }
 

@@ -2380,11 +2380,8 @@ int gs3d_learned_init(gs3d_ctx* c, uint64_t seed) {
         HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, name), 0u, kLearnedParams, c->stream));
     for (const char* name : {"learned_lm", "learned_lv"})
         HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, name), 0u, (size_t)c->n * 8, c->stream));
-    // f16 copy of the latents: an Adam step with zero learning rate writes it
-    HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, "learned_lgrad"), 0u, (size_t)c->n * 8, c->stream));
-    HIP_TRY(c, launch_adam(buf<float>(c, "learned_latents"), buf<float>(c, "learned_lm"), buf<float>(c, "learned_lv"),
-                           buf<float>(c, "learned_lgrad"), (size_t)c->n * 8, 0.0f, 0.9f, 0.999f, 1e-8f, 1,
-                           buf<uint16_t>(c, "learned_latents_f16"), c->stream));
+    HIP_TRY(c, launch_f16_copy(buf<float>(c, "learned_latents"), (size_t)c->n * 8,
+                               buf<uint16_t>(c, "learned_latents_f16"), c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->learned_init = true;
     c->learned_enabled = false;
@@ -2461,12 +2458,51 @@ int gs3d_learned_set(gs3d_ctx* c, const float* params, const float* latents) {
     if (params) HIP_TRY(c, hipMemcpyAsync(buf<float>(c, "learned_params"), params, kLearnedParams * 4, hipMemcpyHostToDevice, c->stream));
     if (latents) {
         HIP_TRY(c, hipMemcpyAsync(buf<float>(c, "learned_latents"), latents, (size_t)c->n * 32, hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(c, launch_fill_u32(buf<uint32_t>(c, "learned_lgrad"), 0u, (size_t)c->n * 8, c->stream));
-        HIP_TRY(c, launch_adam(buf<float>(c, "learned_latents"), buf<float>(c, "learned_lm"), buf<float>(c, "learned_lv"),
-                               buf<float>(c, "learned_lgrad"), (size_t)c->n * 8, 0.0f, 0.9f, 0.999f, 1e-8f, 1,
-                               buf<uint16_t>(c, "learned_latents_f16"), c->stream));
+        HIP_TRY(c, launch_f16_copy(buf<float>(c, "learned_latents"), (size_t)c->n * 8,
+                                   buf<uint16_t>(c, "learned_latents_f16"), c->stream));
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->learned_enabled = false;
+    return GS3D_OK;
+}
+
+int gs3d_learned_get_state(gs3d_ctx* c, gs3d_learned_state* st) {
+    if (!c || !st) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    const size_t np = kLearnedParams * 4, nz = (size_t)c->n * 32;
+    const std::pair<float*, const char*> params[] = {{st->params, "learned_params"}, {st->params_m, "learned_pm"},
+                                                     {st->params_v, "learned_pv"}};
+    const std::pair<float*, const char*> lat[] = {{st->latents, "learned_latents"}, {st->latents_m, "learned_lm"},
+                                                  {st->latents_v, "learned_lv"}};
+    for (const auto& p : params)
+        if (p.first) HIP_TRY(c, hipMemcpyAsync(p.first, buf<float>(c, p.second), np, hipMemcpyDeviceToHost, c->stream));
+    for (const auto& p : lat)
+        if (p.first && nz) HIP_TRY(c, hipMemcpyAsync(p.first, buf<float>(c, p.second), nz, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    st->step = c->learned_step;
+    return GS3D_OK;
+}
+
+int gs3d_learned_set_state(gs3d_ctx* c, const gs3d_learned_state* st) {
+    if (!c || !st) return GS3D_E_INVALID_ARG;
+    GS3D_TRY(learned_require(c));
+    GS3D_TRY(set_device(c));
+    GS3D_TRY(settle(c));
+    const size_t np = kLearnedParams * 4, nz = (size_t)c->n * 32;
+    const std::pair<const float*, const char*> params[] = {{st->params, "learned_params"}, {st->params_m, "learned_pm"},
+                                                           {st->params_v, "learned_pv"}};
+    const std::pair<const float*, const char*> lat[] = {{st->latents, "learned_latents"}, {st->latents_m, "learned_lm"},
+                                                        {st->latents_v, "learned_lv"}};
+    for (const auto& p : params)
+        if (p.first) HIP_TRY(c, hipMemcpyAsync(buf<float>(c, p.second), p.first, np, hipMemcpyHostToDevice, c->stream));
+    for (const auto& p : lat)
+        if (p.first && nz) HIP_TRY(c, hipMemcpyAsync(buf<float>(c, p.second), p.first, nz, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_f16_copy(buf<float>(c, "learned_latents"), (size_t)c->n * 8,
+                               buf<uint16_t>(c, "learned_latents_f16"), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->learned_step = st->step;
     c->learned_enabled = false;
     return GS3D_OK;
 }

@@ -137,6 +137,8 @@ _PROTOS = {
     "gs3d_learned_grad": (I32, [P, U32, P, P, C.POINTER(F32)]),
     "gs3d_learned_get": (I32, [P, P, P, P]),
     "gs3d_learned_set": (I32, [P, P, P]),
+    "gs3d_learned_get_state": (I32, [P, P]),
+    "gs3d_learned_set_state": (I32, [P, P]),
     "gs3d_learned_dirs": (I32, [P]),
     "gs3d_learned_colors": (I32, [P, P, P]),
     "gs3d_learned_validate": (I32, [P, P, U32, F32, C.POINTER(F32), C.POINTER(I32)]),
@@ -203,3 +205,9 @@ def launch_geometry(kernel: str, problem, handle=None) -> tuple[tuple[int, int, 
     if rc != 0:
         raise Gs3dError(rc, f"no pass named {kernel!r}")
     return tuple(int(x) for x in wg), tuple(int(x) for x in grid)
+
+
+class LearnedState(C.Structure):
+    """gs3d_learned_state: pointers to the six state arrays and the Adam step."""
+    _fields_ = [("params", P), ("params_m", P), ("params_v", P), ("latents", P), ("latents_m", P),
+                ("latents_v", P), ("step", U32)]

@@ -209,6 +209,23 @@ class GaussianSplatRenderer:
         self._check(self._lib.gs3d_learned_set(self._h, L.ptr(p) if p is not None else None,
                                                L.ptr(z) if z is not None else None))
 
+    STATE_ARRAYS = ("params", "params_m", "params_v", "latents", "latents_m", "latents_v")
+
+    def learned_get_state(self) -> dict:
+        """The whole training state (a checkpoint): the six arrays and the Adam step."""
+        out = {k: np.zeros(L.LEARNED_PARAMS, np.float32) for k in self.STATE_ARRAYS[:3]}
+        out.update({k: np.zeros((self.splat_count, 8), np.float32) for k in self.STATE_ARRAYS[3:]})
+        st = L.LearnedState(*[L.ptr(out[k]) for k in self.STATE_ARRAYS], 0)
+        self._check(self._lib.gs3d_learned_get_state(self._h, C.byref(st)))
+        out["step"] = int(st.step)
+        return out
+
+    def learned_set_state(self, state: dict) -> None:
+        arrays = {k: np.ascontiguousarray(state[k], np.float32) for k in self.STATE_ARRAYS if k in state}
+        st = L.LearnedState(*[L.ptr(arrays[k]) if k in arrays else None for k in self.STATE_ARRAYS],
+                            int(state.get("step", 0)))
+        self._check(self._lib.gs3d_learned_set_state(self._h, C.byref(st)))
+
     @staticmethod
     def learned_dirs() -> np.ndarray:
         d = np.zeros((L.LEARNED_DIRS, 3), np.float32)
