@@ -118,17 +118,17 @@ def probe_cpu_webgpu() -> dict:
         for g in globs:
             hits += glob.glob(g, recursive=True)[:4]
         found[key] = sorted(set(hits))[:4]
-    try:
-        ld = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
-        found["ldconfig"] = sorted({l.split()[0] for l in ld.splitlines()
-                                    if any(x in l for x in ("wgpu", "dawn", "vulkan"))})[:8]
-    except (OSError, subprocess.SubprocessError):
-        found["ldconfig"] = []
+    libs = []
+    for d in ("/usr/lib", "/usr/lib/x86_64-linux-gnu", "/usr/local/lib", "/lib/x86_64-linux-gnu"):
+        for pat in ("libwgpu*", "libdawn*", "libwebgpu*", "libvulkan*"):
+            libs += glob.glob(f"{d}/{pat}")
+    found["system_libraries"] = sorted(set(libs))[:8]
     node = shutil.which("node")
     found["node"] = subprocess.run([node, "--version"], capture_output=True, text=True,
                                    timeout=10).stdout.strip() if node else None
     usable = bool(found["wgpu_native"] or found["dawn"] or found["node_webgpu"])
     found["usable_cpu_adapter"] = usable and bool(found["vulkan_cpu_icd"])
+    found["note"] = "looked for, nothing installed; run before the GPU is initialised"
     return found
 
 
@@ -323,6 +323,7 @@ def main() -> int:
               "(a multi-GPU run would shard batches by view); use --gpus 1", file=sys.stderr)
         return 2
 
+    webgpu_probe = probe_cpu_webgpu()  # before anything initialises the GPU (it starts node)
     if args.copy_calibration:
         import torch
 
@@ -522,7 +523,7 @@ def main() -> int:
                   "mean_ms": round(sum(o["ms"] for o in orbit) / len(orbit), 4) if orbit else None,
                   "cycling_ms_per_frame": round(cycle_ms, 4) if orbit else None,
                   "cycling_frames_per_s": round(1000.0 / cycle_ms, 2) if orbit else None},
-        "cpu_webgpu_probe": probe_cpu_webgpu(),
+        "cpu_webgpu_probe": webgpu_probe,
     }
     if args.composite > 0:
         # The display step (not part of frames/s): premultiplied over onto the
