@@ -115,13 +115,18 @@ __device__ __forceinline__ bool may_touch(float4 f0, float4 f1, float rx0, float
     const float rr = rad * 1.01f + 0.05f;
     if (ex * ex + ey * ey > rr * rr) return false;
     if (!(ca > 0.0f && cc > 0.0f && ca * cc - cb * cb > 0.0f)) return true;  // degenerate: keep
+    // The vertex positions and the threshold use the hardware reciprocal and
+    // log2 (about 1 ulp): a vertex off by that much moves q by far less than
+    // the margins, and q at any point of the rectangle is >= its minimum.
     // edge x = qx: vertex at dy = -cb ex / cc, clamped to the rectangle
-    const float vy = fminf(fmaxf(my - cb * ex / cc, ry0), ry1) - my;
+    const float vy = fminf(fmaxf(my - cb * ex * __builtin_amdgcn_rcpf(cc), ry0), ry1) - my;
     const float q1 = ca * ex * ex + 2.0f * cb * ex * vy + cc * vy * vy;
     // edge y = qy: vertex at dx = -cb ey / ca, clamped
-    const float vx = fminf(fmaxf(mx - cb * ey / ca, rx0), rx1) - mx;
+    const float vx = fminf(fmaxf(mx - cb * ey * __builtin_amdgcn_rcpf(ca), rx0), rx1) - mx;
     const float q2 = ca * vx * vx + 2.0f * cb * vx * ey + cc * ey * ey;
-    const float t = 2.0f * __logf(255.0f * op) * 1.02f + 0.05f;
+    // 2 ln(255 o) = 2 ln(2) log2(255 o); 255 o >= 0.99 here, a normal number
+    constexpr float kTwoLn2 = 1.3862943611198906f;
+    const float t = kTwoLn2 * __builtin_amdgcn_logf(255.0f * op) * 1.02f + 0.05f;
     return fminf(q1, q2) * 0.99f <= t;
 }
 
