@@ -130,6 +130,60 @@ __device__ __forceinline__ bool may_touch(float4 f0, float4 f1, float rx0, float
     return fminf(q1, q2) * 0.99f <= t;
 }
 
+// may_touch for the four 4 x 4 quarters of an 8 x 8 block at once. Quarter
+// q = 2 r + c covers the pixel centres x0 + 4 c + [0, 3], y0 + 4 r + [0, 3];
+// hit[q] is may_touch on that rectangle, with the same tests and margins.
+// Terms that depend on one column or one row only are formed once; a clamp
+// is v_med3 (the bounds are ordered, the mean finite).
+__device__ __forceinline__ void may_touch_quarters(float4 f0, float4 f1, float x0, float y0, bool hit[4]) {
+    const float mx = f0.x, my = f0.y, rad = f0.w;
+    const float ca = f1.x, cb = f1.y, cc = f1.z, op = f1.w;
+    const bool visible = op * 255.0f >= 0.99f;
+    const float rr = rad * 1.01f + 0.05f;
+    const float rr2 = rr * rr;
+    const bool conic_ok = ca > 0.0f && cc > 0.0f && ca * cc - cb * cb > 0.0f;
+    const float kx = cb * __builtin_amdgcn_rcpf(cc);  // edge x = const: vertex at dy = -kx dx
+    const float ky = cb * __builtin_amdgcn_rcpf(ca);  // edge y = const: vertex at dx = -ky dy
+    const float cb2 = 2.0f * cb;
+    // q_min * 0.99 <= t  as  q_min <= t / 0.99
+    constexpr float kTwoLn2 = 1.3862943611198906f;
+    const float t = (kTwoLn2 * __builtin_amdgcn_logf(255.0f * op) * 1.02f + 0.05f) * (1.0f / 0.99f);
+    float lo_x[2], hi_x[2], lo_y[2], hi_y[2];
+    float ex[2], ex2[2], ey[2], ey2[2];  // mean to the nearest column / row, squared
+    float vy0[2], ax[2], bx[2];          // per column: edge-x vertex before its clamp, ca ex^2, 2 cb ex
+    float vx0[2], cy[2], by[2];          // per row:    edge-y vertex before its clamp, cc ey^2, 2 cb ey
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        lo_x[i] = x0 + 4.0f * (float)i;
+        hi_x[i] = lo_x[i] + 3.0f;
+        lo_y[i] = y0 + 4.0f * (float)i;
+        hi_y[i] = lo_y[i] + 3.0f;
+        ex[i] = __builtin_amdgcn_fmed3f(mx, lo_x[i], hi_x[i]) - mx;
+        ey[i] = __builtin_amdgcn_fmed3f(my, lo_y[i], hi_y[i]) - my;
+        ex2[i] = ex[i] * ex[i];
+        ey2[i] = ey[i] * ey[i];
+        vy0[i] = fmaf(-kx, ex[i], my);
+        vx0[i] = fmaf(-ky, ey[i], mx);
+        ax[i] = ca * ex2[i];
+        bx[i] = cb2 * ex[i];
+        cy[i] = cc * ey2[i];
+        by[i] = cb2 * ey[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = q & 1, r = q >> 1;
+        const bool circle = ex2[c] + ey2[r] <= rr2;
+        // edge x = mean's nearest column: q = ca ex^2 + 2 cb ex vy + cc vy^2
+        const float vy = __builtin_amdgcn_fmed3f(vy0[c], lo_y[r], hi_y[r]) - my;
+        const float q1 = fmaf(cc * vy, vy, fmaf(bx[c], vy, ax[c]));
+        // edge y = mean's nearest row: q = ca vx^2 + 2 cb vx ey + cc ey^2
+        const float vx = __builtin_amdgcn_fmed3f(vx0[r], lo_x[c], hi_x[c]) - mx;
+        const float q2 = fmaf(ca * vx, vx, fmaf(by[r], vx, cy[r]));
+        // a degenerate conic keeps every quarter its circle reaches
+        hit[q] = visible & circle & (!conic_ok | (fminf(q1, q2) <= t));
+    }
+}
+
 // Pixel block of one wavefront: BW columns x BH rows, each lane owning PPL
 // pixels of one column spaced 64/BW rows apart. 8-pixel-wide blocks (8 x 8 at
 // one pixel per lane) touch fewer splats than full-width rows of the same area.
@@ -153,6 +207,40 @@ struct alignas(16) Survivor {
     float2 pad;
 };
 
+// One wavefront's LDS. Every form holds the batch's compacted survivors; the
+// one-pixel-per-lane form also holds a zero-opacity record (slot kWave) and,
+// per 16-lane quarter, the LDS addresses of the survivors that may touch the
+// quarter's 4 x 4 pixels, padded with the zero record's address.
+template <int PPL>
+struct WaveLds {
+    Survivor slot[kWave];
+};
+template <>
+struct WaveLds<1> {
+    Survivor slot[kWave + 1];
+    uint32_t list[4][kWave];
+};
+
+// Set bits of a wavefront mask below this lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// LDS byte address of a __shared__ object, and a survivor read from one.
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+typedef float LdsF4 __attribute__((ext_vector_type(4)));
+typedef float LdsF2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void lds_survivor(uint32_t addr, float4& a, float4& b, float2& c) {
+    const __attribute__((address_space(3))) LdsF4* p4 = (const __attribute__((address_space(3))) LdsF4*)(uintptr_t)addr;
+    const LdsF4 va = p4[0], vb = p4[1];
+    const LdsF2 vc = *(const __attribute__((address_space(3))) LdsF2*)(uintptr_t)(addr + 32u);
+    a = make_float4(va.x, va.y, va.z, va.w);
+    b = make_float4(vb.x, vb.y, vb.z, vb.w);
+    c = make_float2(vc.x, vc.y);
+}
+
 // TW x TW tile; RasterShape gives the wavefronts' pixel blocks. `proj` holds
 // one record of `rec` float4s per splat (4: gs3d_projected_splat, 3: a batch
 // view's 48-B record); only the first three float4s are read.
@@ -166,13 +254,14 @@ struct alignas(16) Survivor {
 // alpha 0, which leaves C and T bit-identical (T - T * 0 = T,
 // C + T * 0 * c = C). The all-saturated exit is checked once per 4 splats.
 // One wavefront's pixel block `wave` (0 .. kWaves-1) of `tile`, with that
-// wavefront's own LDS slots.
+// wavefront's own LDS slots. This form serves PPL > 1; one pixel per lane
+// takes raster_block_quarters below.
 template <int TW, int PPL>
-__device__ __forceinline__ void raster_block(const float4* __restrict__ proj, uint32_t rec,
-                                             const uint32_t* __restrict__ values,
-                                             const uint32_t* __restrict__ tile_offsets, uint32_t tile,
-                                             int wave, uint32_t tiles_x, uint32_t width, uint32_t height,
-                                             uint2* __restrict__ image, Survivor* slots) {
+__device__ __forceinline__ void raster_block_rows(const float4* __restrict__ proj, uint32_t rec,
+                                                  const uint32_t* __restrict__ values,
+                                                  const uint32_t* __restrict__ tile_offsets, uint32_t tile,
+                                                  int wave, uint32_t tiles_x, uint32_t width, uint32_t height,
+                                                  uint2* __restrict__ image, Survivor* slots) {
     using Sh = RasterShape<TW, PPL>;
     constexpr float kLog2e = 1.4426950408889634f;
 
@@ -300,28 +389,194 @@ __device__ __forceinline__ void raster_block(const float4* __restrict__ proj, ui
     }
 }
 
+// One pixel per lane: the 8 x 8 block of wavefront `wave`, each 16-lane
+// quarter of the wavefront owning a 4 x 4 quarter of the block (lane l: pixel
+// (l & 3, (l >> 2) & 3) of quarter l >> 4). A batch is culled per quarter
+// (may_touch_quarters); the union is compacted into the slots as in
+// raster_block_rows, and each quarter gets its own list of the slots that may
+// touch it, in list order, padded to the longest list (rounded up to 4) with
+// the zero record. The composite loop then reads, per lane, the survivors of
+// its own quarter: a splat that reaches one quarter of the block costs one
+// step instead of a step for the whole block. Every pixel still blends its
+// own tile's list in order, a splat the cull drops could not have passed the
+// per-pixel rules, and the zero record blends nothing, so each pixel's result
+// is the same as with one list per block.
+template <int TW>
+__device__ __forceinline__ void raster_block_quarters(const float4* __restrict__ proj, uint32_t rec,
+                                                      const uint32_t* __restrict__ values,
+                                                      const uint32_t* __restrict__ tile_offsets,
+                                                      uint32_t tile, int wave, uint32_t tiles_x,
+                                                      uint32_t width, uint32_t height,
+                                                      uint2* __restrict__ image, WaveLds<1>* lds) {
+    using Sh = RasterShape<TW, 1>;
+    static_assert(Sh::BW == 8 && Sh::BH == 8, "one pixel per lane: 8 x 8 blocks");
+    constexpr float kLog2e = 1.4426950408889634f;
+    constexpr float kTMin = 1.0e-4f;
+
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int quarter = lane >> 4;
+    const uint32_t bx0 = tx * TW + (wave % Sh::kBlocksX) * 8u;
+    const uint32_t by0 = ty * TW + (wave / Sh::kBlocksX) * 8u;
+    const uint32_t px = bx0 + (uint32_t)(quarter & 1) * 4u + (uint32_t)(lane & 3);
+    const uint32_t py = by0 + (uint32_t)(quarter >> 1) * 4u + (uint32_t)((lane >> 2) & 3);
+    const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+    const float x0 = (float)bx0 + 0.5f, y0 = (float)by0 + 0.5f;  // first pixel centre of the block
+
+    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    float tr = (px < width && py < height) ? 1.0f : 0.0f;  // T >= 1e-4: the pixel is live
+
+    if (lane == 0) {
+        Survivor z;
+        z.a = make_float4(0.f, 0.f, 0.f, 0.f);
+        z.b = z.a;
+        z.c = make_float2(0.f, 0.f);
+        z.pad = z.c;
+        lds->slot[kWave] = z;
+    }
+    const uint32_t zero_addr = lds_addr(&lds->slot[kWave]);
+    const uint32_t slot_addr0 = lds_addr(&lds->slot[0]);
+    const uint32_t* const my_list = lds->list[quarter];
+
+    auto blend = [&](float4 a, float4 b, float2 c) {
+        const float dx = pxf - a.x, dy = pyf - a.y;
+        const float u = fmaf(b.x, dy, a.w * dx);       // A dx + B dy
+        const float p = fmaf(b.y * dy, dy, u * dx);    // log2(e) * power
+        const float d2 = fmaf(dy, dy, dx * dx);
+        const float alpha = fminf(b.z * __builtin_amdgcn_exp2f(p), 0.99f);
+        const bool ok = (tr >= kTMin) & (d2 <= a.z) & (p <= 0.0f) & (alpha >= 1.0f / 255.0f);
+        const float w = tr * (ok ? alpha : 0.0f);
+        c0 = fmaf(w, b.w, c0);
+        c1 = fmaf(w, c.x, c1);
+        c2 = fmaf(w, c.y, c2);
+        tr = tr - w;
+    };
+
+    const uint32_t start = tile_offsets[tile];
+    const uint32_t end = tile_offsets[tile + 1];
+
+    float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
+    if (start < end && start + lane < end) {
+        const uint32_t sid = values[start + lane];
+        n0 = proj[(size_t)sid * rec + 0];
+        n1 = proj[(size_t)sid * rec + 1];
+        n2 = proj[(size_t)sid * rec + 2];
+    }
+    for (uint32_t base = start; base < end; base += kWave) {
+        const uint64_t live = __ballot(tr >= kTMin);
+        if (live == 0ull) break;
+        const float4 f0 = n0, f1 = n1, f2 = n2;
+        const bool have = base + lane < end;
+        const uint32_t nidx = base + kWave + lane;
+        if (nidx < end) {
+            const uint32_t sid = values[nidx];
+            n0 = proj[(size_t)sid * rec + 0];
+            n1 = proj[(size_t)sid * rec + 1];
+            n2 = proj[(size_t)sid * rec + 2];
+        }
+        bool qh[4];
+        may_touch_quarters(f0, f1, x0, y0, qh);
+        // a quarter whose 16 pixels are all saturated takes no survivors
+        uint64_t qm[4];
+        uint32_t jmax = 0;
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            qh[q] = have && qh[q] && ((live >> (16 * q)) & 0xffffull) != 0ull;
+            qm[q] = __ballot(qh[q]);
+            jmax = max(jmax, ((uint32_t)__popcll(qm[q]) + 3u) & ~3u);
+            any = any || qh[q];
+        }
+        jmax = __builtin_amdgcn_readfirstlane(jmax);
+        const uint32_t slot = lanes_below(qm[0] | qm[1] | qm[2] | qm[3]);
+        if (any) {
+            Survivor s;
+            s.a = make_float4(f0.x, f0.y, f0.w * f0.w, -0.5f * kLog2e * f1.x);
+            s.b = make_float4(-kLog2e * f1.y, -0.5f * kLog2e * f1.z, f1.w, f2.x);
+            s.c = make_float2(f2.y, f2.z);
+            s.pad = make_float2(0.f, 0.f);
+            lds->slot[slot] = s;
+        }
+        // every list entry first points at the zero record (64 lanes x 4
+        // entries = the four lists), then the survivors' entries overwrite the
+        // head of each list; a wavefront's LDS stores complete in order
+        reinterpret_cast<uint4*>(&lds->list[0][0])[lane] = make_uint4(zero_addr, zero_addr, zero_addr, zero_addr);
+        const uint32_t my_addr = slot_addr0 + slot * (uint32_t)sizeof(Survivor);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (qh[q]) lds->list[q][lanes_below(qm[q])] = my_addr;
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the next 4 entries are read while these 4 blend (index kept inside the list)
+        uint4 ad_next = *reinterpret_cast<const uint4*>(&my_list[0]);
+        for (uint32_t j = 0; j < jmax; j += 4) {
+            const uint4 ad = ad_next;
+            ad_next = *reinterpret_cast<const uint4*>(&my_list[min(j + 4u, (uint32_t)kWave - 4u)]);
+            float4 a0, b0, a1, b1, a2, b2, a3, b3;
+            float2 e0, e1, e2, e3;
+            lds_survivor(ad.x, a0, b0, e0);
+            lds_survivor(ad.y, a1, b1, e1);
+            lds_survivor(ad.z, a2, b2, e2);
+            lds_survivor(ad.w, a3, b3, e3);
+            blend(a0, b0, e0);
+            blend(a1, b1, e1);
+            blend(a2, b2, e2);
+            blend(a3, b3, e3);
+            if (__ballot(tr >= kTMin) == 0ull) break;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (px < width && py < height) {
+        const uint32_t h0 = __half_as_ushort(__float2half_rn(c0));
+        const uint32_t h1 = __half_as_ushort(__float2half_rn(c1));
+        const uint32_t h2 = __half_as_ushort(__float2half_rn(c2));
+        const uint32_t h3 = __half_as_ushort(__float2half_rn(1.0f - tr));
+        image[(size_t)py * width + px] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
+    }
+}
+
+template <int TW, int PPL>
+__device__ __forceinline__ void raster_block(const float4* __restrict__ proj, uint32_t rec,
+                                             const uint32_t* __restrict__ values,
+                                             const uint32_t* __restrict__ tile_offsets, uint32_t tile,
+                                             int wave, uint32_t tiles_x, uint32_t width, uint32_t height,
+                                             uint2* __restrict__ image, WaveLds<PPL>* lds) {
+    if constexpr (PPL == 1) {
+        raster_block_quarters<TW>(proj, rec, values, tile_offsets, tile, wave, tiles_x, width, height, image,
+                                  lds);
+    } else {
+        raster_block_rows<TW, PPL>(proj, rec, values, tile_offsets, tile, wave, tiles_x, width, height, image,
+                                   lds->slot);
+    }
+}
+
 // Every pixel block of `tile`, one per wavefront of the workgroup.
 template <int TW, int PPL>
 __device__ __forceinline__ void raster_tile(const float4* __restrict__ proj, uint32_t rec,
                                             const uint32_t* __restrict__ values,
                                             const uint32_t* __restrict__ tile_offsets, uint32_t tile,
                                             uint32_t tiles_x, uint32_t width, uint32_t height,
-                                            uint2* __restrict__ image, Survivor (*s_slot)[kWave]) {
+                                            uint2* __restrict__ image, WaveLds<PPL>* s_lds) {
     const int wave = threadIdx.x / kWave;
     raster_block<TW, PPL>(proj, rec, values, tile_offsets, tile, wave, tiles_x, width, height, image,
-                          s_slot[wave]);
+                          &s_lds[wave]);
 }
+
+// The one-pixel-per-lane kernels are held to 6 wavefronts per SIMD (at most
+// 80 VGPRs; the compiler spills a few values that live across the whole
+// kernel, outside the composite loop). They are bound by VALU issue and hide
+// the LDS latency of the per-lane survivor reads with the extra wavefront:
+// 1M splats SH3 1080p, 16 x 16 tiles, 160.8 us at 5 wavefronts, 156.1 at 6.
+#define GS3D_RASTER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PPL == 1 ? 6 : 1)))
 
 // One workgroup per tile (the single-view frame).
 template <int TW, int PPL>
-__global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __restrict__ proj,
-                                                             uint32_t rec,
-                                                             const uint32_t* __restrict__ values,
-                                                             const uint32_t* __restrict__ tile_offsets,
-                                                             uint32_t tiles_x, uint32_t width,
-                                                             uint32_t height, uint2* __restrict__ image) {
-    __shared__ Survivor s_slot[RasterShape<TW, PPL>::kWaves][kWave];
-    raster_tile<TW, PPL>(proj, rec, values, tile_offsets, blockIdx.x, tiles_x, width, height, image, s_slot);
+__global__ __launch_bounds__(TW * TW / PPL) GS3D_RASTER_OCCUPANCY void k_rasterize(
+    const float4* __restrict__ proj, uint32_t rec, const uint32_t* __restrict__ values,
+    const uint32_t* __restrict__ tile_offsets, uint32_t tiles_x, uint32_t width, uint32_t height,
+    uint2* __restrict__ image) {
+    __shared__ WaveLds<PPL> s_lds[RasterShape<TW, PPL>::kWaves];
+    raster_tile<TW, PPL>(proj, rec, values, tile_offsets, blockIdx.x, tiles_x, width, height, image, s_lds);
 }
 
 // The batch's persistent rasterizer: a grid sized to what the GPU holds at
@@ -334,10 +589,9 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize(const float4* __res
 // * blocks per tile + block), so no wavefront waits at a barrier for the
 // slowest block of its tile.
 template <int TW, int PPL>
-__global__ __launch_bounds__(TW * TW / PPL) void k_rasterize_queue(const RasterView* __restrict__ views,
-                                                                   const uint2* __restrict__ queue,
-                                                                   uint32_t n_items, uint32_t* ticket) {
-    __shared__ Survivor s_slot[RasterShape<TW, PPL>::kWaves][kWave];
+__global__ __launch_bounds__(TW * TW / PPL) GS3D_RASTER_OCCUPANCY void k_rasterize_queue(
+    const RasterView* __restrict__ views, const uint2* __restrict__ queue, uint32_t n_items, uint32_t* ticket) {
+    __shared__ WaveLds<PPL> s_lds[RasterShape<TW, PPL>::kWaves];
     __shared__ uint32_t s_next[2];
     if (threadIdx.x == 0) s_next[0] = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -348,7 +602,7 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize_queue(const RasterV
         const uint2 q = queue[item];
         const RasterView& v = views[q.x];
         raster_tile<TW, PPL>(v.proj, 3, v.values, v.tile_offsets, q.y, v.tiles_x, v.width, v.height, v.image,
-                             s_slot);
+                             s_lds);
         // the next ticket is visible to all; the slot read at the top of this
         // iteration is the one thread 0 overwrites in the next
         __syncthreads();
@@ -356,13 +610,12 @@ __global__ __launch_bounds__(TW * TW / PPL) void k_rasterize_queue(const RasterV
 }
 
 template <int TW, int PPL>
-__global__ __launch_bounds__(256) void k_rasterize_queue_waves(const RasterView* __restrict__ views,
-                                                               const uint2* __restrict__ queue,
-                                                               uint32_t n_items, uint32_t* ticket) {
+__global__ __launch_bounds__(256) GS3D_RASTER_OCCUPANCY void k_rasterize_queue_waves(
+    const RasterView* __restrict__ views, const uint2* __restrict__ queue, uint32_t n_items, uint32_t* ticket) {
     constexpr uint32_t kBlocks = RasterShape<TW, PPL>::kWaves;
-    __shared__ Survivor s_slot[256 / kWave][kWave];
+    __shared__ WaveLds<PPL> s_lds[256 / kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    Survivor* slots = s_slot[threadIdx.x / kWave];
+    WaveLds<PPL>* const lds = &s_lds[threadIdx.x / kWave];
     const uint32_t total = n_items * kBlocks;
     uint32_t t = 0;
     if (lane == 0) t = atomicAdd(ticket, 1u);
@@ -372,7 +625,7 @@ __global__ __launch_bounds__(256) void k_rasterize_queue_waves(const RasterView*
         const uint2 q = queue[item / kBlocks];
         const RasterView& v = views[q.x];
         raster_block<TW, PPL>(v.proj, 3, v.values, v.tile_offsets, q.y, (int)(item % kBlocks), v.tiles_x,
-                              v.width, v.height, v.image, slots);
+                              v.width, v.height, v.image, lds);
         item = __shfl(t, 0, kWave);
     }
 }
