@@ -489,11 +489,7 @@ def main() -> int:
         roofline = {"kernel": dominant, "bound": "valu", "achieved": round(tf, 2),
                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                    "hbm_frac": d["hbm_frac"],
-                    # the 157.3 TFLOP/s fp32 peak counts packed (two-wide) FMAs; this code is
-                    # built without packed fp32, which halves the reachable vector peak
-                    "peak_without_packed_fp32": FP32_PEAK_TFLOPS / 2,
-                    "frac_without_packed_fp32": round(tf / (FP32_PEAK_TFLOPS / 2), 4)}
+                    "hbm_frac": d["hbm_frac"]}
     else:
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": d["GB/s"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": d["hbm_frac"], "traffic": traffic}

@@ -526,12 +526,15 @@ __device__ __forceinline__ void raster_block_quarters(const float4* __restrict__
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (px < width && py < height) {
+    // the pixel again from its centre (x + 0.5 truncates to x), so the integer
+    // coordinates need not stay live through the loops
+    const uint32_t ox = (uint32_t)pxf, oy = (uint32_t)pyf;
+    if (ox < width && oy < height) {
         const uint32_t h0 = __half_as_ushort(__float2half_rn(c0));
         const uint32_t h1 = __half_as_ushort(__float2half_rn(c1));
         const uint32_t h2 = __half_as_ushort(__float2half_rn(c2));
         const uint32_t h3 = __half_as_ushort(__float2half_rn(1.0f - tr));
-        image[(size_t)py * width + px] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
+        image[(size_t)oy * width + ox] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
     }
 }
 
