@@ -298,6 +298,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--quarter-lists", type=int, default=1, choices=(0, 1),
+                    help="rasterizer survivor lists per 4x4 quarter (1) or per 8x8 block (0)")
     ap.add_argument("--key-tile", type=int, default=0, help="keys per workgroup of the 64-bit sort passes "
                     "(0 = chosen by key count, 8192, 16384)")
     ap.add_argument("--depth-key-tile", type=int, default=0, help="keys per workgroup of the 32-bit depth "
@@ -358,6 +360,7 @@ def main() -> int:
     r.set_tile_size(args.tile, args.tile)
     r.set_kernel_variant("radix-sort", "KEY_TILE", args.key_tile)
     r.set_kernel_variant("radix-sort-splats", "KEY_TILE", args.depth_key_tile)
+    r.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", args.quarter_lists)
     for _ in range(args.warmup):
         r.rebin(cu, W, H)
     r.sync()
@@ -501,7 +504,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl.name, "splats": splats.n, "sh_degree": splats.sh_degree,
                    "width": W, "height": H, "tile": args.tile, "keys": k, "tiles": T,
-                   "radix_passes": {"splats": depth_passes, "keys": passes}, "digit_bits": 8},
+                   "radix_passes": {"splats": depth_passes, "keys": passes}, "digit_bits": 8,
+                   "raster_lists": "per 4x4 quarter" if args.quarter_lists else "per 8x8 block"},
         "mkeys_per_s": round(k / ((stages["radix-sort"] + stages["radix-sort-splats"]) * 1e-6) / 1e6, 1)
         if stages["radix-sort"] else None,
         "stages_us": {n: round(v, 2) for n, v in stages.items()},

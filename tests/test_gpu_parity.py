@@ -340,35 +340,38 @@ def test_raster_images_identical_across_blocks(renderer):
     """A splat that can reach a pixel lies in the list of every tile that holds the pixel
     (its tile rectangle bounds its radius circle), lists are ordered by (depth, id), and
     every variant does the same per-pixel arithmetic. The variants cull with different
-    rectangles: 8 x 8 blocks and their 4 x 4 quarters at one pixel per lane, 8 x 16 and
-    16 x 16 blocks at 2 and 4. A cull that dropped a splat some pixel blends would change
-    that pixel in some variants and not others, so the f16 images of the 1M SH3 1080p frame
-    at tiles 8, 16, 32 and 1, 2, 4 pixels per lane must be bit-identical, at three camera
-    positions."""
+    rectangles: 8 x 8 blocks and their 4 x 4 quarters at one pixel per lane (or the blocks
+    alone, QUARTER_LISTS 0), 8 x 16 and 16 x 16 blocks at 2 and 4. A cull that dropped a splat
+    some pixel blends would change that pixel in some variants and not others, so the f16
+    images of the 1M SH3 1080p frame at tiles 8, 16, 32, 1, 2, 4 pixels per lane and both list
+    forms must be bit-identical, at three camera positions."""
     wl = sc.workload(1)
     s = sc.activate(wl.raw)
     renderer.set_splat_data(s)
     _, W, H = wl.views[0]
-    variants = [(16, 1), (8, 1), (32, 1), (16, 2), (16, 4), (32, 2), (32, 4)]
+    variants = [(16, 1, 1), (8, 1, 1), (32, 1, 1), (16, 1, 0), (8, 1, 0), (32, 1, 0), (16, 2, 1), (16, 4, 1),
+                (32, 2, 1), (32, 4, 1)]
     try:
         for az in (0.0, 135.0, 270.0):
             eye = wl.views[0][0] if az == 0.0 else sc.orbit_eye(wl.orbit_radius, az, 20.0)
             cu = cam_mod.view_camera(eye, W, H, wl.fov_y)
             base = None
-            for tile, ppl in variants:
+            for tile, ppl, quarters in variants:
                 renderer.set_tile_size(tile, tile)
                 renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", ppl)
+                renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", quarters)
                 renderer.rebin(cu, W, H)
                 img = renderer.read(L.BUF_IMAGE_RGBA16F, np.uint16)
                 if base is None:
                     base = img
                     assert np.count_nonzero(img) > img.size // 4  # the frame is not empty
                 else:
-                    assert np.array_equal(img, base), f"azimuth {az}: tile {tile}, {ppl} px/lane"
+                    assert np.array_equal(img, base), f"azimuth {az}: tile {tile}, {ppl} px/lane, quarters {quarters}"
             say(f"  azimuth {az}: {len(variants)} variants bit-identical")
     finally:
         renderer.set_tile_size(16, 16)
         renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 1)
+        renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", 1)
 
 
 @pytest.mark.parametrize("tile,ppl", [(16, 2), (16, 4), (32, 2), (32, 4)])
@@ -403,6 +406,9 @@ def test_kernel_variant_errors(renderer):
         renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 2)  # 8x8 is one wavefront
     with pytest.raises(L.Gs3dError):
         renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 6)
+    with pytest.raises(L.Gs3dError):
+        renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", 2)
+    assert renderer.get_kernel_variant("rasterize-tiles", "QUARTER_LISTS") == 1
     with pytest.raises(L.Gs3dError):
         renderer.set_kernel_variant("nope", "X", 1)
     renderer.set_tile_size(16, 16)

@@ -266,13 +266,17 @@ bool raster_variant_exists(uint32_t tile, uint32_t pixels_per_lane);
 hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_t* values,
                             const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                             uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
-                            uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
+                            uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s,
+                            bool quarter_lists = true);
 // The same over records of `rec` float4s (4: gs3d_projected_splat; 3: the
-// batch's 48-B records, the first three float4s of one).
+// batch's 48-B records, the first three float4s of one). quarter_lists (one
+// pixel per lane only): a survivor list per 4 x 4 quarter of each 8 x 8
+// block, or one per block.
 hipError_t launch_rasterize_records(const float4* records, uint32_t rec, const uint32_t* values,
                                     const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                                     uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
-                                    uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s);
+                                    uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s,
+                                    bool quarter_lists = true);
 hipError_t launch_f16_to_f32(const uint16_t* in, float* out, size_t n, hipStream_t s);
 
 // Launch shape of a pass for a problem size: threads per workgroup and

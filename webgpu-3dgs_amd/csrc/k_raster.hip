@@ -254,8 +254,9 @@ __device__ __forceinline__ void lds_survivor(uint32_t addr, float4& a, float4& b
 // alpha 0, which leaves C and T bit-identical (T - T * 0 = T,
 // C + T * 0 * c = C). The all-saturated exit is checked once per 4 splats.
 // One wavefront's pixel block `wave` (0 .. kWaves-1) of `tile`, with that
-// wavefront's own LDS slots. This form serves PPL > 1; one pixel per lane
-// takes raster_block_quarters below.
+// wavefront's own LDS slots. This form serves PPL > 1, and one pixel per lane
+// in k_rasterize_blocks; k_rasterize at one pixel per lane takes
+// raster_block_quarters below.
 template <int TW, int PPL>
 __device__ __forceinline__ void raster_block_rows(const float4* __restrict__ proj, uint32_t rec,
                                                   const uint32_t* __restrict__ values,
@@ -582,6 +583,24 @@ __global__ __launch_bounds__(TW * TW / PPL) GS3D_RASTER_OCCUPANCY void k_rasteri
     raster_tile<TW, PPL>(proj, rec, values, tile_offsets, blockIdx.x, tiles_x, width, height, image, s_lds);
 }
 
+// One workgroup per tile, one pixel per lane, one survivor list per 8 x 8
+// block (raster_block_rows; the "QUARTER_LISTS" 0 variant). Per-quarter lists
+// pay when splats are small against a block; when most splats span all four
+// quarters the quarter test is cost without a saving, and this form is
+// faster: 1M SH3 1080p (median radius 6 px) 177 us here against 164 with
+// quarter lists; 6M SH3 2160p (median 21 px) 499 us against 553.
+template <int TW>
+__global__ __launch_bounds__(TW * TW) void k_rasterize_blocks(const float4* __restrict__ proj, uint32_t rec,
+                                                              const uint32_t* __restrict__ values,
+                                                              const uint32_t* __restrict__ tile_offsets,
+                                                              uint32_t tiles_x, uint32_t width, uint32_t height,
+                                                              uint2* __restrict__ image) {
+    __shared__ Survivor s_slot[RasterShape<TW, 1>::kWaves][kWave];
+    const int wave = threadIdx.x / kWave;
+    raster_block_rows<TW, 1>(proj, rec, values, tile_offsets, blockIdx.x, wave, tiles_x, width, height, image,
+                             s_slot[wave]);
+}
+
 // The batch's persistent rasterizer: a grid sized to what the GPU holds at
 // once, every workgroup taking (view, tile) items from `queue` in order with
 // a ticket until the queue is exhausted. Tickets are taken one item ahead
@@ -736,6 +755,14 @@ static void launch_raster_variant(uint32_t tiles, const float4* p, uint32_t rec,
                        tile_offsets, tiles_x, width, height, img);
 }
 
+template <int TW>
+static void launch_raster_blocks(uint32_t tiles, const float4* p, uint32_t rec, const uint32_t* values,
+                                 const uint32_t* tile_offsets, uint32_t tiles_x, uint32_t width, uint32_t height,
+                                 uint2* img, hipStream_t s) {
+    hipLaunchKernelGGL((k_rasterize_blocks<TW>), dim3(tiles), dim3(TW * TW), 0, s, p, rec, values, tile_offsets,
+                       tiles_x, width, height, img);
+}
+
 bool raster_variant_exists(uint32_t tile, uint32_t ppl) {
     if (tile != 8 && tile != 16 && tile != 32) return false;
     if (ppl != 1 && ppl != 2 && ppl != 4) return false;
@@ -745,20 +772,32 @@ bool raster_variant_exists(uint32_t tile, uint32_t ppl) {
 hipError_t launch_rasterize(const gs3d_projected_splat* projected, const uint32_t* values,
                             const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                             uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
-                            uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s) {
+                            uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s,
+                            bool quarter_lists) {
     return launch_rasterize_records(reinterpret_cast<const float4*>(projected), 4, values, tile_offsets, tile_x,
-                                    tile_y, tiles_x, tiles_y, width, height, pixels_per_lane, image_rgba16f, s);
+                                    tile_y, tiles_x, tiles_y, width, height, pixels_per_lane, image_rgba16f, s,
+                                    quarter_lists);
 }
 
 hipError_t launch_rasterize_records(const float4* p, uint32_t rec, const uint32_t* values,
                                     const uint32_t* tile_offsets, uint32_t tile_x, uint32_t tile_y,
                                     uint32_t tiles_x, uint32_t tiles_y, uint32_t width, uint32_t height,
-                                    uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s) {
+                                    uint32_t pixels_per_lane, uint16_t* image_rgba16f, hipStream_t s,
+                                    bool quarter_lists) {
     const uint32_t tiles = tiles_x * tiles_y;
     if (tiles == 0) return hipSuccess;
     if (tile_x != tile_y || !raster_variant_exists(tile_x, pixels_per_lane)) return hipErrorInvalidValue;
     if (rec != 3 && rec != 4) return hipErrorInvalidValue;
     uint2* img = reinterpret_cast<uint2*>(image_rgba16f);
+    if (pixels_per_lane == 1 && !quarter_lists) {
+        switch (tile_x) {
+            case 8: launch_raster_blocks<8>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+            case 16: launch_raster_blocks<16>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+            case 32: launch_raster_blocks<32>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const int key = tile_x * 10 + pixels_per_lane;
     switch (key) {
         case 81: launch_raster_variant<8, 1>(tiles, p, rec, values, tile_offsets, tiles_x, width, height, img, s); break;

@@ -70,6 +70,7 @@ struct gs3d_ctx {
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
     uint32_t raster_ppl = 1;
+    uint32_t raster_quarters = 1;  // one pixel per lane: survivor lists per 4x4 quarter (1) or per 8x8 block (0)
     uint32_t batch_raster_groups = 1;     // persistent raster launches per batch (view groups); 0: one
                                           // raster launch per view on its branch stream
     uint32_t batch_raster_wg_per_cu = 0;  // their workgroups per CU (0: as many as fit)
@@ -527,7 +528,8 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
 
     record(c, GS3D_STAGE_RASTER, 0);
     HIP_TRY(c, launch_rasterize(projected, sorted_vals, tile_offsets, c->tile_x, c->tile_y,
-                                c->tiles_x, c->tiles_y, W, H, c->raster_ppl, image, c->stream));
+                                c->tiles_x, c->tiles_y, W, H, c->raster_ppl, image, c->stream,
+                                c->raster_quarters != 0));
     record(c, GS3D_STAGE_RASTER, 1);
     record(c, GS3D_STAGE_TOTAL, 1);
 
@@ -999,7 +1001,7 @@ int batch_enqueue(gs3d_ctx* c) {
             // per-view raster launches: each view's raster overlaps other views' chains
             HIP_TRY(c, launch_rasterize_records(projected + (size_t)vi * n * 3, 3, view_values, to, c->tile_x,
                                                 c->tile_y, v.tiles_x, v.tiles_y, v.W, v.H, c->raster_ppl,
-                                                images + v.img_off, s));
+                                                images + v.img_off, s, c->raster_quarters != 0));
         } else if (vi + 1 == group_end[group]) {
             // views [group_begin, vi] are queued on their branches: the group's
             // persistent raster waits for the branches up to here
@@ -1552,6 +1554,13 @@ int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
         c->raster_ppl = (uint32_t)value;
         return GS3D_OK;
     }
+    if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "QUARTER_LISTS")) {
+        if (value != 0 && value != 1)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles QUARTER_LISTS %d not 0 or 1", value);
+        if (c->raster_quarters != (uint32_t)value && c->batch) drop_graph(c->batch);
+        c->raster_quarters = (uint32_t)value;
+        return GS3D_OK;
+    }
     if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "GROUPS")) {
         if (value < 0 || value > (int)kMaxRasterGroups)
             return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles-batch GROUPS %d not in 0..%u", value,
@@ -1602,6 +1611,10 @@ int gs3d_get_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
     }
     if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "TILE_X")) {
         *value = (int)c->tile_x;
+        return GS3D_OK;
+    }
+    if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "QUARTER_LISTS")) {
+        *value = (int)c->raster_quarters;
         return GS3D_OK;
     }
     if (!std::strcmp(kernel, "rasterize-tiles-batch") && !std::strcmp(param, "GROUPS")) {
@@ -2196,7 +2209,7 @@ int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uin
     HIP_TRY(c, hipMemcpyAsync(dt, tile_offsets, (T + 1) * 4, hipMemcpyHostToDevice, c->stream));
     const uint32_t ppl = raster_variant_exists(tile_x, c->raster_ppl) ? c->raster_ppl : 1u;
     HIP_TRY(c, launch_rasterize(dp, dv, dt, tile_x, tile_y, tiles_x, tiles_y, width, height, ppl, dimg,
-                                c->stream));
+                                c->stream, c->raster_quarters != 0));
     HIP_TRY(c, launch_f16_to_f32(dimg, dimg32, (size_t)width * height * 4, c->stream));
     HIP_TRY(c, hipMemcpyAsync(image, dimg32, (size_t)width * height * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -149,13 +149,17 @@ const KERNEL_VARIANTS = {
   'radix-sort': { RADIX_BITS: [4, 8], KEY_TILE: [0, 8192, 16384] },
   'radix-sort-splats': { KEY_TILE: [0, 4096, 8192] },
   'identify-tile-ranges': {},
-  'rasterize-tiles': { TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4] },
+  'rasterize-tiles': {
+    TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4], QUARTER_LISTS: [0, 1],
+  },
   'rasterize-tiles-batch': { GROUPS: [...Array(17).keys()], WAVE_ITEMS: [0, 1], WG_PER_CU: [...Array(17).keys()] },
 };
 
 // Constants that select a variant in the library (gs3d_set_kernel_variant);
 // the tile constants follow the renderer's tile size (gs3d_set_tile_size).
-const SELECTED_IN_LIBRARY = new Set(['RADIX_BITS', 'KEY_TILE', 'PIXELS_PER_LANE', 'GROUPS', 'WAVE_ITEMS', 'WG_PER_CU']);
+const SELECTED_IN_LIBRARY = new Set([
+  'RADIX_BITS', 'KEY_TILE', 'PIXELS_PER_LANE', 'QUARTER_LISTS', 'GROUPS', 'WAVE_ITEMS', 'WG_PER_CU',
+]);
 
 function checkConstants(name, kernel, constants) {
   const table = KERNEL_VARIANTS[kernel];
