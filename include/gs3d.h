@@ -177,6 +177,17 @@ int gs3d_get_tile_size(const gs3d_ctx* ctx, uint32_t out_xy[2]);
  *                                        gs3d_stage_sort's 64-bit passes)
  *   "radix-sort-splats" "KEY_TILE"       0 (8192), 4096: keys per workgroup of
  *                                        the 32-bit depth passes
+ *   "rasterize-tiles-batch" (gs3d_rebin_batch):
+ *                     "GROUPS"           1 (default): one persistent rasterizer
+ *                                        over every view's tiles after the
+ *                                        views' binning; g > 1: g such launches
+ *                                        over view groups; 0: a raster launch per
+ *                                        view on its binning stream
+ *                     "WAVE_ITEMS"       0 (default), 1: the persistent
+ *                                        rasterizer takes pixel blocks instead
+ *                                        of tiles per ticket
+ *                     "WG_PER_CU"        0 (as many as fit) .. 16: its
+ *                                        workgroups per CU
  * get_kernel_variant also reads "rasterize-tiles" "TILE_X". Anything else:
  * GS3D_E_UNKNOWN_NAME. A tile-size change resets an impossible PIXELS_PER_LANE
  * to 1. */
