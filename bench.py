@@ -189,18 +189,22 @@ def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
 
     for _ in range(warmup):
         r.rebin_batch(views)
-    per = timed(lambda: r.rebin_batch(views), steps)
+    per_p = timed(lambda: r.rebin_batch(views), steps)
     ks = r.batch_ref_counts()
-    out.update({"value": round(len(views) / per, 2), "unit": "views/s", "ms_per_batch": round(per * 1e3, 3),
-                "mode": "hipGraph replay, one persistent rasterizer over every view's tiles",
-                "keys_total": int(ks.sum()), "keys_max_view": int(ks.max())})
     # the same batch with one raster launch per view on its binning stream
     r.set_kernel_variant("rasterize-tiles-batch", "GROUPS", 0)
     for _ in range(warmup):
         r.rebin_batch(views)
     per_v = timed(lambda: r.rebin_batch(views), steps)
-    out["per_view_raster_views_per_s"] = round(len(views) / per_v, 2)
     r.set_kernel_variant("rasterize-tiles-batch", "GROUPS", 1)
+    # value: the faster of the two raster schedules, both replayed from a captured graph
+    per = min(per_p, per_v)
+    out.update({"value": round(len(views) / per, 2), "unit": "views/s", "ms_per_batch": round(per * 1e3, 3),
+                "mode": ("hipGraph replay, one persistent rasterizer over every view's tiles" if per_p <= per_v
+                         else "hipGraph replay, a raster launch per view on its binning stream"),
+                "persistent_raster_views_per_s": round(len(views) / per_p, 2),
+                "per_view_raster_views_per_s": round(len(views) / per_v, 2),
+                "keys_total": int(ks.sum()), "keys_max_view": int(ks.max())})
     r.rebin_batch(views)
     r.rebin_batch(views, eager=True)
     per_e = timed(lambda: r.rebin_batch(views, eager=True), max(1, steps // 2))
@@ -229,8 +233,9 @@ def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
         log(f"[bench] batch autotune: best tile {best.tile}, ppl {best.pixels_per_lane}, "
             f"{best.radix_bits}-bit digits, raster groups {best.raster_groups}: "
             f"{out['tuned_views_per_s']} views/s")
-    log(f"[bench] batched: {out['value']} views/s (graph, persistent raster), "
-        f"{out['per_view_raster_views_per_s']} with a raster launch per view, {out['eager_views_per_s']} eager, "
+    log(f"[bench] batched: {out['value']} views/s; graph with the persistent rasterizer "
+        f"{out['persistent_raster_views_per_s']}, with a raster launch per view "
+        f"{out['per_view_raster_views_per_s']}, persistent form eager {out['eager_views_per_s']}, "
         f"{out['single_view_rebins_views_per_s']} one rebin per view; K total {out['keys_total']}")
     r.close()
     return out
