@@ -585,7 +585,7 @@ int settle(gs3d_ctx* c) {
 //
 // gs3d_rebin_batch renders many views of the resident splats. One launch of
 // k_preprocess_views projects every splat into every view (each splat's
-// attributes and SH coefficients are read once per 8 views). Then each view
+// attributes and SH coefficients are read once per 64 views). Then each view
 // runs the frame's chain (depth LSD, scan, emit, tile LSD, identify,
 // rasterize) on one of kBatchBranches streams: views are dealt to branches
 // longest-first by pixel count, so the latency-bound tails of one view's
