@@ -8,10 +8,13 @@ already resident in HBM. Prints ONE JSON line (rank 0).
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1]
                   [--no-cpu-baseline] [--cpu-seconds S] [--no-batched]
 
-The same line carries "batched": the 2M-splat SH3 scene rendered for 64
-seeded orbit views of uneven resolution (640x360 / 1280x720 / 1920x1080) per
-gs3d_rebin_batch call, as views/s from one replayed hipGraph, beside the
-same views launched eagerly and one gs3d_rebin per view.
+The same line carries "stress": the 6M-splat SH3 frame at 3840x2160 (about
+119M references), with the sort's HBM fraction on its own bytes there;
+"batched": the 2M-splat SH3 scene rendered for 64 seeded orbit views of uneven
+resolution (640x360 / 1280x720 / 1920x1080) per gs3d_rebin_batch call, as
+views/s from one replayed hipGraph, beside the same views launched eagerly and
+one gs3d_rebin per view; and "learned": the learned-colour batch of the
+3M-splat scene against exact SH.
 """
 from __future__ import annotations
 
@@ -159,6 +162,72 @@ def cpu_baseline(cam, splats, W, H, tile, seconds, gpu_image=None):
                "vs_render_f64": float(np.abs(gpu_image - f64).max()),
                "oracle_keys": int(k), "f64_keys": int(k64), "tolerance": 1.0 / 255.0}
     return out, visited, blended, err
+
+
+def sort_roofline_of(algo: dict, stages: dict) -> dict | None:
+    """The sort's HBM fraction on its own algorithmic bytes: the depth passes over
+    the splats and the tile-id passes over the references, together and each."""
+    sort_us = stages.get("radix-sort", 0.0) + stages.get("radix-sort-splats", 0.0)
+    if sort_us <= 0:
+        return None
+
+    def frac(b, us):
+        return round(b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if us > 0 else None
+
+    own = algo["radix-sort-splats"] + algo["radix-sort"]
+    own_gbs = own / (sort_us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(own_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(own_gbs / HBM_PEAK_GBS, 4), "algo_bytes": int(own), "us": round(sort_us, 2),
+            "depth_passes_frac": frac(algo["radix-sort-splats"], stages.get("radix-sort-splats", 0.0)),
+            "tile_passes_frac": frac(algo["radix-sort"], stages.get("radix-sort", 0.0))}
+
+
+def bench_stress(steps: int, warmup: int) -> dict:
+    """The 6M-splat SH3 frame at 3840x2160 (about 119M references): frames/s and
+    the sort's own-bytes HBM fraction, the size where the sort is bandwidth-bound
+    rather than latency-bound."""
+    from gs3d import camera, scene
+    from gs3d._lib import STAGES
+    from gs3d.renderer import GaussianSplatRenderer
+
+    t = time.perf_counter()
+    wl = scene.workload(2)
+    splats = scene.activate(wl.raw)
+    eye, W, H = wl.views[0]
+    cu = camera.view_camera(eye, W, H, wl.fov_y)
+    log(f"[bench] stress workload {wl.name}: {splats.n} splats, {W}x{H} (generated in "
+        f"{time.perf_counter() - t:.1f} s)")
+    r = GaussianSplatRenderer(0)
+    r.set_splat_data(splats)
+    r.set_tile_size(wl.tile, wl.tile)
+    for _ in range(warmup):
+        r.rebin(cu, W, H)  # the first frame grows the key buffers and runs again
+    r.sync()
+    k = r.ref_count()
+    r.set_profile_mask(STAGES)
+    r.profiler_reset()
+    for _ in range(10):
+        r.rebin(cu, W, H, 1)  # GS3D_REBIN_PROFILE
+    stages, _ = r.stage_times_avg_us()
+    r.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.rebin(cu, W, H)
+    r.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    tiles_x, tiles_y = r.get_tile_count()
+    shifts = r.sort_passes()
+    depth_passes = sum(1 for b in shifts if b < 32)
+    algo = algorithmic_bytes(splats.n, k, tiles_x * tiles_y, W * H, depth_passes, len(shifts) - depth_passes,
+                             splats.sh_degree, packed=tiles_x <= 256 and tiles_y <= 256)
+    r.close()
+    out = {"workload": wl.name, "splats": splats.n, "width": W, "height": H, "keys": k,
+           "value": round(1000.0 / ms, 2), "unit": "frames/s", "ms_per_frame": round(ms, 4), "steps": steps,
+           "stages_us": {n: round(v, 2) for n, v in stages.items()},
+           "sort_roofline": sort_roofline_of(algo, stages)}
+    log(f"[bench] stress: {out['value']} frames/s ({ms:.3f} ms), K={k}; sort {out['sort_roofline']['us']} us, "
+        f"{out['sort_roofline']['frac']:.3f} of HBM peak on its own bytes")
+    return out
 
 
 def bench_batched(steps: int, warmup: int, tune: bool = True) -> dict:
@@ -312,6 +381,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-view batch measurement")
+    ap.add_argument("--no-stress", action="store_true",
+                    help="skip the 6M-splat 3840x2160 frame (frames/s and the sort's HBM fraction there)")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the 64-view batch measurement (for kernel traces of it)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-colour measurement")
@@ -477,13 +548,6 @@ def main() -> int:
                                               "reference_layout_equivalent_frac": round(gbs / HBM_PEAK_GBS, 4),
                                               "note": "the reference's sort bytes over this sort's time: "
                                                       "not this sort's own traffic, not a roofline fraction"}
-        own = algo["radix-sort-splats"] + algo["radix-sort"]
-        own_gbs = own / (sort_us * 1e-6) / 1e9
-        sort_roofline = {"bound": "hbm", "achieved": round(own_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(own_gbs / HBM_PEAK_GBS, 4), "algo_bytes": int(own),
-                         "us": round(sort_us, 2),
-                         "depth_passes_frac": per_stage["radix-sort-splats"]["hbm_frac"],
-                         "tile_passes_frac": per_stage["radix-sort"]["hbm_frac"]}
     d = per_stage[dominant]
     d["us_timed_region"] = round(dominant_us, 2)
     d["GB/s"] = round(algo[dominant] / (dominant_us * 1e-6) / 1e9, 1)
@@ -521,7 +585,7 @@ def main() -> int:
         "dominant_sampled_frames": frames,
         "per_stage": per_stage,
         "roofline": roofline,
-        "sort_roofline": sort_roofline if sort_us > 0 else None,
+        "sort_roofline": sort_roofline_of(algo, stages),
         "cpu_baseline": cpu,
         "image_error": image_error,
         "orbit": {"positions": orbit,
@@ -549,6 +613,8 @@ def main() -> int:
         r.set_debug_mode(0, 256)
         out["display"] = disp
     r.close()
+    if not args.no_stress and args.workload == 1:
+        out["stress"] = bench_stress(30, 3)
     if not args.no_batched:
         out["batched"] = bench_batched(args.batch_steps, 2)
     if not args.no_learned:
