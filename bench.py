@@ -372,8 +372,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--tile", type=int, default=16)
-    ap.add_argument("--quarter-lists", type=int, default=1, choices=(0, 1),
-                    help="rasterizer survivor lists per 4x4 quarter (1) or per 8x8 block (0)")
+    ap.add_argument("--quarter-lists", type=int, default=2, choices=(0, 1, 2),
+                    help="rasterizer survivor lists per 4x4 quarter (1), per 8x8 block (0), or chosen per "
+                         "frame from the previous frame's references per splat (2)")
     ap.add_argument("--key-tile", type=int, default=0, help="keys per workgroup of the 64-bit sort passes "
                     "(0 = chosen by key count, 8192, 16384)")
     ap.add_argument("--depth-key-tile", type=int, default=0, help="keys per workgroup of the 32-bit depth "
@@ -574,7 +575,9 @@ def main() -> int:
         "config": {"workload": wl.name, "splats": splats.n, "sh_degree": splats.sh_degree,
                    "width": W, "height": H, "tile": args.tile, "keys": k, "tiles": T,
                    "radix_passes": {"splats": depth_passes, "keys": passes}, "digit_bits": 8,
-                   "raster_lists": "per 4x4 quarter" if args.quarter_lists else "per 8x8 block"},
+                   "raster_lists": {0: "per 8x8 block", 1: "per 4x4 quarter",
+                                    2: "per 4x4 quarter up to 8 references per splat, else per 8x8 block, "
+                                       "chosen per frame"}[args.quarter_lists]},
         "mkeys_per_s": round(k / ((stages["radix-sort"] + stages["radix-sort-splats"]) * 1e-6) / 1e6, 1)
         if stages["radix-sort"] else None,
         "stages_us": {n: round(v, 2) for n, v in stages.items()},

@@ -166,11 +166,16 @@ int gs3d_get_tile_size(const gs3d_ctx* ctx, uint32_t out_xy[2]);
  * src/PipelineManager.ts:34-105; the WGSL is specialised by text substitution,
  * here a precompiled variant is selected):
  *   "rasterize-tiles" "PIXELS_PER_LANE"  1, 2, 4 (workgroup tile^2/ppl >= 64)
- *   "rasterize-tiles" "QUARTER_LISTS"    1 (default), 0: at one pixel per lane,
- *                                        a survivor list per 4x4 quarter of
- *                                        each 8x8 pixel block, or one per
- *                                        block (faster when most splats span
- *                                        the whole block); same image
+ *   "rasterize-tiles" "QUARTER_LISTS"    2 (default), 1, 0: at one pixel per
+ *                                        lane, a survivor list per 4x4 quarter
+ *                                        of each 8x8 pixel block (1), or one per
+ *                                        block (0, faster when most splats span
+ *                                        the whole block), or (2) per frame:
+ *                                        one per block when the previous frame
+ *                                        had more than 8 references per splat
+ *                                        at 16x16 tiles (scaled by tile area),
+ *                                        else per quarter (batches: per
+ *                                        quarter); the image is the same
  *   "radix-sort"      "RADIX_BITS"       4, 8
  *   "radix-sort"      "KEY_TILE"         0 (16384), 8192, 16384: keys per
  *                                        workgroup of the tile-id passes (and of

@@ -341,7 +341,7 @@ def test_raster_images_identical_across_blocks(renderer):
     (its tile rectangle bounds its radius circle), lists are ordered by (depth, id), and
     every variant does the same per-pixel arithmetic. The variants cull with different
     rectangles: 8 x 8 blocks and their 4 x 4 quarters at one pixel per lane (or the blocks
-    alone, QUARTER_LISTS 0), 8 x 16 and 16 x 16 blocks at 2 and 4. A cull that dropped a splat
+    alone, QUARTER_LISTS 0, or either chosen per frame, 2), 8 x 16 and 16 x 16 blocks at 2 and 4. A cull that dropped a splat
     some pixel blends would change that pixel in some variants and not others, so the f16
     images of the 1M SH3 1080p frame at tiles 8, 16, 32, 1, 2, 4 pixels per lane and both list
     forms must be bit-identical, at three camera positions."""
@@ -349,8 +349,8 @@ def test_raster_images_identical_across_blocks(renderer):
     s = sc.activate(wl.raw)
     renderer.set_splat_data(s)
     _, W, H = wl.views[0]
-    variants = [(16, 1, 1), (8, 1, 1), (32, 1, 1), (16, 1, 0), (8, 1, 0), (32, 1, 0), (16, 2, 1), (16, 4, 1),
-                (32, 2, 1), (32, 4, 1)]
+    variants = [(16, 1, 1), (8, 1, 1), (32, 1, 1), (16, 1, 0), (8, 1, 0), (32, 1, 0), (16, 1, 2), (8, 1, 2),
+                (16, 2, 1), (16, 4, 1), (32, 2, 1), (32, 4, 1)]
     try:
         for az in (0.0, 135.0, 270.0):
             eye = wl.views[0][0] if az == 0.0 else sc.orbit_eye(wl.orbit_radius, az, 20.0)
@@ -371,7 +371,7 @@ def test_raster_images_identical_across_blocks(renderer):
     finally:
         renderer.set_tile_size(16, 16)
         renderer.set_kernel_variant("rasterize-tiles", "PIXELS_PER_LANE", 1)
-        renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", 1)
+        renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", 2)
 
 
 @pytest.mark.parametrize("tile,ppl", [(16, 2), (16, 4), (32, 2), (32, 4)])
@@ -407,8 +407,8 @@ def test_kernel_variant_errors(renderer):
     with pytest.raises(L.Gs3dError):
         renderer.set_kernel_variant("radix-sort", "RADIX_BITS", 6)
     with pytest.raises(L.Gs3dError):
-        renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", 2)
-    assert renderer.get_kernel_variant("rasterize-tiles", "QUARTER_LISTS") == 1
+        renderer.set_kernel_variant("rasterize-tiles", "QUARTER_LISTS", 3)
+    assert renderer.get_kernel_variant("rasterize-tiles", "QUARTER_LISTS") == 2
     with pytest.raises(L.Gs3dError):
         renderer.set_kernel_variant("nope", "X", 1)
     renderer.set_tile_size(16, 16)

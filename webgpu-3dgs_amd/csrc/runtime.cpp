@@ -70,7 +70,10 @@ struct gs3d_ctx {
     uint32_t width = 0, height = 0, tiles_x = 0, tiles_y = 0;
     uint32_t digit_bits = 8;
     uint32_t raster_ppl = 1;
-    uint32_t raster_quarters = 1;  // one pixel per lane: survivor lists per 4x4 quarter (1) or per 8x8 block (0)
+    uint32_t raster_quarters = 2;  // one pixel per lane: survivor lists per 4x4 quarter (1) or per 8x8 block
+                                   // (0), or (2) chosen for each frame from the previous frame's
+                                   // references per splat (quarter_lists_auto)
+    bool auto_quarters = true;     // the choice for the next frame in mode 2
     uint32_t batch_raster_groups = 1;     // persistent raster launches per batch (view groups); 0: one
                                           // raster launch per view on its branch stream
     uint32_t batch_raster_wg_per_cu = 0;  // their workgroups per CU (0: as many as fit)
@@ -254,6 +257,22 @@ void plan_passes(gs3d_ctx* c, uint32_t tile_count, uint32_t bits) {
     const uint32_t tile_bits = tile_count > 1 ? bit_length(tile_count - 1) : 0;
     c->n_passes = 0;
     for (uint32_t s = 0; s < 32u + tile_bits; s += bits) c->shifts[c->n_passes++] = s;
+}
+
+// List form of the next frame in QUARTER_LISTS mode 2. Per-quarter lists pay
+// when splats are small against an 8 x 8 block; when most splats span the
+// block, the quarter test costs VALU and saves no blend steps (1M SH3 1080p:
+// 163 us with quarter lists, 178 with one list per block; 6M SH3 2160p: 550
+// against 503). References per splat, scaled to 16 x 16 tiles, measure the
+// splats' size: 3.6 and 19.8 on those two frames; above 8, one list per block.
+// Both forms give the same image bit for bit, so only the time depends on
+// the choice.
+bool quarter_lists_auto(uint64_t K, uint32_t n, uint32_t tile_x, uint32_t tile_y) {
+    return (double)K * tile_x * tile_y <= 8.0 * 256.0 * (double)n;
+}
+
+bool frame_quarter_lists(const gs3d_ctx* c) {
+    return c->raster_quarters == 2 ? c->auto_quarters : c->raster_quarters != 0;
 }
 
 void record(gs3d_ctx* c, int stage, int which) {
@@ -529,7 +548,7 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     record(c, GS3D_STAGE_RASTER, 0);
     HIP_TRY(c, launch_rasterize(projected, sorted_vals, tile_offsets, c->tile_x, c->tile_y,
                                 c->tiles_x, c->tiles_y, W, H, c->raster_ppl, image, c->stream,
-                                c->raster_quarters != 0));
+                                frame_quarter_lists(c)));
     record(c, GS3D_STAGE_RASTER, 1);
     record(c, GS3D_STAGE_TOTAL, 1);
 
@@ -572,6 +591,7 @@ int settle(gs3d_ctx* c) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         c->k_pending = false;
         c->K = c->pinned[0];
+        c->auto_quarters = quarter_lists_auto(c->K, c->n, c->tile_x, c->tile_y);
         if (c->K <= c->key_capacity) return GS3D_OK;
         c->key_capacity = key_capacity_for(c->K + c->K / 4 + kSortTile);  // the next ensure() reallocates
         GS3D_TRY(enqueue_rebin(c, &c->last_cam, c->last_w, c->last_h, c->last_flags));
@@ -999,6 +1019,7 @@ int batch_enqueue(gs3d_ctx* c) {
                                          b->pinned_k + vi, s));
         if (b->groups == 0) {
             // per-view raster launches: each view's raster overlaps other views' chains
+            // (mode 2 takes quarter lists: a batch's views are captured before their K is known)
             HIP_TRY(c, launch_rasterize_records(projected + (size_t)vi * n * 3, 3, view_values, to, c->tile_x,
                                                 c->tile_y, v.tiles_x, v.tiles_y, v.W, v.H, c->raster_ppl,
                                                 images + v.img_off, s, c->raster_quarters != 0));
@@ -1555,8 +1576,8 @@ int gs3d_set_kernel_variant(gs3d_ctx* c, const char* kernel, const char* param, 
         return GS3D_OK;
     }
     if (!std::strcmp(kernel, "rasterize-tiles") && !std::strcmp(param, "QUARTER_LISTS")) {
-        if (value != 0 && value != 1)
-            return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles QUARTER_LISTS %d not 0 or 1", value);
+        if (value != 0 && value != 1 && value != 2)
+            return fail(c, GS3D_E_UNKNOWN_NAME, "rasterize-tiles QUARTER_LISTS %d not 0, 1 or 2", value);
         if (c->raster_quarters != (uint32_t)value && c->batch) drop_graph(c->batch);
         c->raster_quarters = (uint32_t)value;
         return GS3D_OK;
@@ -2208,8 +2229,10 @@ int gs3d_stage_rasterize(gs3d_ctx* c, const gs3d_projected_splat* projected, uin
         HIP_TRY(c, hipMemcpyAsync(dv, sorted_values, (size_t)n_refs * 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(dt, tile_offsets, (T + 1) * 4, hipMemcpyHostToDevice, c->stream));
     const uint32_t ppl = raster_variant_exists(tile_x, c->raster_ppl) ? c->raster_ppl : 1u;
+    const bool quarters = c->raster_quarters == 2 ? quarter_lists_auto(n_refs, n_splats, tile_x, tile_y)
+                                                   : c->raster_quarters != 0;
     HIP_TRY(c, launch_rasterize(dp, dv, dt, tile_x, tile_y, tiles_x, tiles_y, width, height, ppl, dimg,
-                                c->stream, c->raster_quarters != 0));
+                                c->stream, quarters));
     HIP_TRY(c, launch_f16_to_f32(dimg, dimg32, (size_t)width * height * 4, c->stream));
     HIP_TRY(c, hipMemcpyAsync(image, dimg32, (size_t)width * height * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

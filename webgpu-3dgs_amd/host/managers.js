@@ -150,7 +150,7 @@ const KERNEL_VARIANTS = {
   'radix-sort-splats': { KEY_TILE: [0, 4096, 8192] },
   'identify-tile-ranges': {},
   'rasterize-tiles': {
-    TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4], QUARTER_LISTS: [0, 1],
+    TILE_X: [8, 16, 32], TILE_Y: [8, 16, 32], TILE_AREA: [64, 256, 1024], PIXELS_PER_LANE: [1, 2, 4], QUARTER_LISTS: [0, 1, 2],
   },
   'rasterize-tiles-batch': { GROUPS: [...Array(17).keys()], WAVE_ITEMS: [0, 1], WG_PER_CU: [...Array(17).keys()] },
 };
