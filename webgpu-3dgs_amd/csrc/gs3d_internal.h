@@ -203,6 +203,33 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, uint32_t
     return ((uint64_t)hi << 32) | lo;
 }
 
+// match_digit for two items at once: their ballot steps interleave, so each
+// step's ballot result is read two instructions after it is written (no
+// wait states) and the uniform test of a bit against match_bits is taken once
+// per pair. FULL: every digit bit can be set (no test at all).
+template <int BITS, bool FULL>
+__device__ __forceinline__ void match_digit2(uint32_t d0, bool valid0, uint32_t d1, bool valid1, uint32_t match_bits,
+                                             uint64_t& peers0, uint64_t& peers1) {
+    const uint64_t vm0 = __ballot(valid0), vm1 = __ballot(valid1);
+    uint32_t lo0 = (uint32_t)vm0, hi0 = (uint32_t)(vm0 >> 32);
+    uint32_t lo1 = (uint32_t)vm1, hi1 = (uint32_t)(vm1 >> 32);
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        if (FULL || (uint32_t)b < match_bits) {
+            const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d0, b, 1);  // all ones if bit b set
+            const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d1, b, 1);
+            const uint64_t bal0 = __ballot(m0 != 0u);
+            const uint64_t bal1 = __ballot(m1 != 0u);
+            lo0 = __builtin_amdgcn_bitop3_b32(lo0, (uint32_t)bal0, m0, 0x90);
+            hi0 = __builtin_amdgcn_bitop3_b32(hi0, (uint32_t)(bal0 >> 32), m0, 0x90);
+            lo1 = __builtin_amdgcn_bitop3_b32(lo1, (uint32_t)bal1, m1, 0x90);
+            hi1 = __builtin_amdgcn_bitop3_b32(hi1, (uint32_t)(bal1 >> 32), m1, 0x90);
+        }
+    }
+    peers0 = ((uint64_t)hi0 << 32) | lo0;
+    peers1 = ((uint64_t)hi1 << 32) | lo1;
+}
+
 __device__ __forceinline__ uint64_t match_digit_rt(uint32_t d, bool valid, uint32_t bits) {
     return bits == 8 ? match_digit<8>(d, valid) : match_digit<4>(d, valid);
 }

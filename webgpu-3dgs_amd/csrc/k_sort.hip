@@ -113,26 +113,29 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     // uniform over the block: past the keys, or more keys than the buffers hold
     // (the host re-runs the frame with larger buffers in that case)
     if (tile_base >= n || n > n_max) return;
-
-    const size_t wave_base = tile_base + (size_t)wave * (kWave * ITEMS);
+    // Positions in the tile are u32 (< TILE): item it of this thread is tile
+    // position lpos + 64 it, valid below tile_n.
+    const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - tile_base);
+    auto in_tile = [&](uint32_t j) -> bool { return j < tile_n; };
+    const uint32_t lpos = (uint32_t)(wave * (kWave * ITEMS) + lane);
+    const KeyT* __restrict__ kin_t = kin + tile_base;
     KeyT k[ITEMS];
     uint32_t rank[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const size_t idx = wave_base + (size_t)it * kWave + lane;
-        k[it] = idx < n ? kin[idx] : (KeyT)~(KeyT)0;
+        const uint32_t j = lpos + (uint32_t)(it * kWave);
+        k[it] = in_tile(j) ? kin_t[j] : (KeyT)~(KeyT)0;
     }
 
     // Wave-level multisplit: per item, BITS ballots narrow each lane's group of
     // equal digits; rank = prior count of the digit in this wavefront + lanes
     // of the group below this one; the group's lowest lane bumps the count.
+    // Two items share each ballot step (match_digit2); their counts are then
+    // taken in item order.
+    static_assert(ITEMS % 2 == 0, "items are matched in pairs");
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        const size_t idx = wave_base + (size_t)it * kWave + lane;
-        const bool valid = idx < n;
-        const uint32_t d = (uint32_t)(k[it] >> shift) & MASK;
-        const uint64_t peers = match_digit<BITS>(d, valid, match_bits);
+    const uint32_t mb = __builtin_amdgcn_readfirstlane(match_bits);
+    auto count = [&](int it, uint64_t peers, bool valid, uint32_t d) {
         uint32_t r = 0;
         if (valid) {
             const uint64_t below = peers & lt_mask;
@@ -142,6 +145,20 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         }
         rank[it] = r;
         __builtin_amdgcn_wave_barrier();
+    };
+#pragma unroll
+    for (int it = 0; it < ITEMS; it += 2) {
+        const uint32_t j0 = lpos + (uint32_t)(it * kWave);
+        const bool v0 = in_tile(j0), v1 = in_tile(j0 + (uint32_t)kWave);
+        const uint32_t d0 = (uint32_t)(k[it] >> shift) & MASK;
+        const uint32_t d1 = (uint32_t)(k[it + 1] >> shift) & MASK;
+        uint64_t p0, p1;
+        if (mb >= (uint32_t)BITS)
+            match_digit2<BITS, true>(d0, v0, d1, v1, mb, p0, p1);
+        else
+            match_digit2<BITS, false>(d0, v0, d1, v1, mb, p0, p1);
+        count(it, p0, v0, d0);
+        count(it + 1, p1, v1, d1);
     }
     __syncthreads();
 
@@ -181,12 +198,12 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     uint32_t val[ITEMS], aux[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const size_t idx = wave_base + (size_t)it * kWave + lane;
-        if (idx < n) {
+        const uint32_t j = lpos + (uint32_t)(it * kWave);
+        if (in_tile(j)) {
             s_stage[rank[it]] = k[it];
             s_digit[rank[it]] = (uint8_t)dig[it];
-            val[it] = IOTA ? (uint32_t)idx : vin[idx];
-            if constexpr (AUX == kAuxCarry) aux[it] = aux_in[idx];
+            val[it] = IOTA ? (uint32_t)(tile_base + j) : vin[tile_base + j];
+            if constexpr (AUX == kAuxCarry) aux[it] = aux_in[tile_base + j];
         }
     }
 
@@ -227,43 +244,39 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         // gather would hold up the lookback's status loads
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
-            const size_t idx = wave_base + (size_t)it * kWave + lane;
-            if (idx < n) aux[it] = aux_in[val[it]];
+            if (in_tile(lpos + (uint32_t)(it * kWave))) aux[it] = aux_in[val[it]];
         }
     }
     // keys out in LDS order (runs of equal digits are contiguous stores)
-    const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - tile_base);
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
-        if (j < tile_n) kout[scatter_index(s_delta[s_digit[j]], j)] = s_stage[j];
+        if (in_tile(j)) kout[scatter_index(s_delta[s_digit[j]], j)] = s_stage[j];
     }
     __syncthreads();
     // values: same permutation, reusing the staging memory
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const size_t idx = wave_base + (size_t)it * kWave + lane;
-        if (idx < n) s_vals[rank[it]] = val[it];
+        if (in_tile(lpos + (uint32_t)(it * kWave))) s_vals[rank[it]] = val[it];
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
-        if (j < tile_n) vout[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
+        if (in_tile(j)) vout[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
     }
     if constexpr (AUX != kAuxNone) {
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
-            const size_t idx = wave_base + (size_t)it * kWave + lane;
-            if (idx < n) s_vals[rank[it]] = aux[it];
+            if (in_tile(lpos + (uint32_t)(it * kWave))) s_vals[rank[it]] = aux[it];
         }
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
             const uint32_t j = (uint32_t)(it * THREADS + tid);
-            if (j < tile_n) aux_out[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
+            if (in_tile(j)) aux_out[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
         }
     }
 }
