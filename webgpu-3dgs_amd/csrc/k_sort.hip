@@ -247,11 +247,16 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
             if (in_tile(lpos + (uint32_t)(it * kWave))) aux[it] = aux_in[val[it]];
         }
     }
-    // keys out in LDS order (runs of equal digits are contiguous stores)
+    // keys out in LDS order (runs of equal digits are contiguous stores); the
+    // output index of each sorted position is kept for the values (and aux)
+    size_t dst[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
-        if (in_tile(j)) kout[scatter_index(s_delta[s_digit[j]], j)] = s_stage[j];
+        if (in_tile(j)) {
+            dst[it] = scatter_index(s_delta[s_digit[j]], j);
+            kout[dst[it]] = s_stage[j];
+        }
     }
     __syncthreads();
     // values: same permutation, reusing the staging memory
@@ -263,8 +268,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const uint32_t j = (uint32_t)(it * THREADS + tid);
-        if (in_tile(j)) vout[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
+        if (in_tile((uint32_t)(it * THREADS + tid))) vout[dst[it]] = s_vals[it * THREADS + tid];
     }
     if constexpr (AUX != kAuxNone) {
         __syncthreads();
@@ -275,8 +279,7 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < ITEMS; it++) {
-            const uint32_t j = (uint32_t)(it * THREADS + tid);
-            if (in_tile(j)) aux_out[scatter_index(s_delta[s_digit[j]], j)] = s_vals[j];
+            if (in_tile((uint32_t)(it * THREADS + tid))) aux_out[dst[it]] = s_vals[it * THREADS + tid];
         }
     }
 }
