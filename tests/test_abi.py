@@ -97,3 +97,17 @@ def test_reference_pass_stages():
         "preprocess-splats", "prefix-scan", "emit-tile-refs", "radix-sort-splats+radix-sort",
         "identify-tile-ranges", "rasterize-tiles", "composite-splats"]
     assert table[3][2] == 2.0 + 5.0
+
+
+def test_autotuner_candidates():
+    """Each candidate keeps at least one wavefront per tile; list forms vary only at
+    one pixel per lane (2 = picked per frame, 1 = quarter lists, 0 = block lists)."""
+    from gs3d import autotune
+
+    c = list(autotune.candidates(tiles=(8, 16), ppls=(1, 2, 4), radix_bits=(8,)))
+    assert (8, 2, 8, 2) not in c and (8, 4, 8, 2) not in c
+    assert [q for t, p, b, q in c if (t, p) == (8, 1)] == [2, 1, 0]
+    assert [q for t, p, b, q in c if (t, p) == (16, 1)] == [2, 1, 0]
+    assert [q for t, p, b, q in c if t == 16 and p > 1] == [2, 2]
+    assert len(c) == 8
+    assert autotune.Choice(16, 1, 8, 0.0, {}).quarter_lists == 2

@@ -189,6 +189,7 @@ def test_batch_autotuner_picks_the_fastest_candidate(renderer, scene_sh3):
         assert best.views_per_s == max(r.views_per_s for r in results)
         assert renderer.get_tile_size() == (best.tile, best.tile)
         assert renderer.get_kernel_variant("radix-sort", "RADIX_BITS") == best.radix_bits
+        assert renderer.get_kernel_variant("rasterize-tiles", "QUARTER_LISTS") == 2
         check_batch(renderer, views)
     finally:
         renderer.set_tile_size(16, 16)

@@ -32,18 +32,18 @@ class Choice:
     radix_bits: int
     frame_us: float
     stages_us: dict
-    quarter_lists: int = 1
+    quarter_lists: int = 2
 
 
-def candidates(tiles=(8, 16, 32), ppls=(1, 2, 4), radix_bits=(4, 8), quarter_lists=(1, 0)):
+def candidates(tiles=(8, 16, 32), ppls=(1, 2, 4), radix_bits=(4, 8), quarter_lists=(2, 1, 0)):
     """(tile, pixels per lane, digit bits, quarter lists); quarter lists only vary at one pixel per lane."""
     for t, p, b in itertools.product(tiles, ppls, radix_bits):
         if t * t // p >= 64:
-            for q in (quarter_lists if p == 1 else (1,)):
+            for q in (quarter_lists if p == 1 else (2,)):
                 yield t, p, b, q
 
 
-def tune(renderer, views, *, tiles=(8, 16, 32), ppls=(1, 2, 4), radix_bits=(8, 4), quarter_lists=(1, 0),
+def tune(renderer, views, *, tiles=(8, 16, 32), ppls=(1, 2, 4), radix_bits=(8, 4), quarter_lists=(2, 1, 0),
          warmup=3, reps=10, fov_y=None) -> tuple[Choice, list[Choice]]:
     """Time every candidate over `views` [(eye, W, H), ...]; returns (best, all)."""
     import math
@@ -87,8 +87,9 @@ def tune_batch(renderer, cams, *, tiles=(8, 16, 32), ppls=(1, 2), radix_bits=(8,
     """Time every candidate on gs3d_rebin_batch over `cams` [(camera uniform, W, H), ...]
     (a re-capture per candidate, then `reps` timed replays); returns (best, all)."""
     results = []
-    # the persistent rasterizer keeps the quarter lists; the batch tunes the rest
-    for (t, p, b, _), g in itertools.product(candidates(tiles, ppls, radix_bits, quarter_lists=(1,)),
+    # the persistent rasterizer keeps the quarter lists; the batch tunes the rest, and single
+    # frames keep the per-frame choice of list form
+    for (t, p, b, _), g in itertools.product(candidates(tiles, ppls, radix_bits, quarter_lists=(2,)),
                                              raster_groups):
         apply(renderer, Choice(t, p, b, 0.0, {}))
         renderer.set_kernel_variant("rasterize-tiles-batch", "GROUPS", g)
