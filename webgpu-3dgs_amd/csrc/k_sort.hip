@@ -15,9 +15,10 @@
 //      rank, the lowest lane bumps the wave's private LDS histogram;
 //   3. per digit: the tile's count is published as an aggregate, the workgroup
 //      looks back over predecessor tiles' status words (decoupled lookback,
-//      up to 8 predecessors per round trip) for its exclusive prefix, then
-//      publishes the inclusive prefix; the digit's global start comes from
-//      the digit histograms fused into emit;
+//      8 predecessors per thread and up to 4 threads per digit, so up to 32
+//      per round trip) for
+//      its exclusive prefix, then publishes the inclusive prefix; the digit's
+//      global start comes from the digit histograms fused into emit;
 //   4. keys, then values, are placed digit-sorted in LDS and written out in
 //      LDS order, so runs of equal digits leave as contiguous stores.
 // HBM traffic per pass: 12 B read + 12 B written per 64-bit key (8 + 8 for
@@ -37,36 +38,58 @@ __device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan over the THREADS threads of the block; all threads must call.
+// Two exclusive scans over the block with one pair of barriers; s_tmp holds
+// 2 * (THREADS / 64) words.
 template <int THREADS>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp) {
+__device__ __forceinline__ void block_exclusive_scan2(uint32_t x, uint32_t y, uint32_t* s_tmp, uint32_t& ex,
+                                                      uint32_t& ey) {
+    constexpr int WAVES = THREADS / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    uint32_t v = x;
+    uint32_t a = x, b = y;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, kWave);
-        if (lane >= d) v += t;
+        const uint32_t ta = __shfl_up(a, d, kWave);
+        const uint32_t tb = __shfl_up(b, d, kWave);
+        if (lane >= d) {
+            a += ta;
+            b += tb;
+        }
     }
-    if (lane == kWave - 1) s_tmp[wave] = v;
+    if (lane == kWave - 1) {
+        s_tmp[wave] = a;
+        s_tmp[WAVES + wave] = b;
+    }
     __syncthreads();
-    uint32_t prefix = 0;
+    uint32_t pa = 0, pb = 0;
 #pragma unroll
-    for (int w = 0; w < THREADS / kWave; w++)
-        if (w < wave) prefix += s_tmp[w];
+    for (int w = 0; w < WAVES; w++) {
+        if (w < wave) {
+            pa += s_tmp[w];
+            pb += s_tmp[WAVES + w];
+        }
+    }
     __syncthreads();
-    return prefix + v - x;
+    ex = pa + a - x;
+    ey = pb + b - y;
 }
 
-// Lookback reads the status words of up to 8 predecessor tiles with
-// independent loads (one latency per 8 tiles) and spins only on a word that is
-// not published yet. Status words are tile-major, status[tile * RADIX + d], so
-// a tile publishes all its digits with one coalesced store.
-constexpr int kLookbackGroup = 8;
+// Lookback: per round trip each of up to 4 threads per digit reads the status
+// words of kLookbackEach predecessor tiles with independent loads (32 tiles
+// per round trip at 256 digits and 1,024 threads), and spins only on a word
+// that is not published yet.
+// The tiles that have published an inclusive prefix form a frontier that
+// advances by at most one span per round trip (a tile finds the frontier,
+// then publishes its own prefix), so the span bounds the pass at span x tile
+// keys per status round trip: at 8 predecessors and 16,384 keys a 118.9M-key
+// pass was measured at that bound (208 G keys/s, 4.9 us of a 19.5 us tile in
+// the lookback). Status words are tile-major, status[tile * RADIX + d], so a
+// tile publishes all its digits with one coalesced store.
+constexpr int kLookbackEach = 8;
 
 // THREADS x ITEMS keys per tile. LDS: the staging buffer (one KeyT per key;
-// values reuse it as u32), the per-wavefront digit counts, which the digit byte
-// of every sorted position later overwrites, and one per-digit delta: global
+// values reuse it as u32), the per-wavefront digit counts, whose memory the
+// lookback's per-thread partial sums reuse, and one per-digit delta: global
 // output index = delta[d] + position in the tile, both u32 and added modulo
 // 2^32 (scatter_index). The delta alone may wrap below zero; the sum is the
 // true index, which is below n <= 2^32 - 1, so key counts of 2^31 and more
@@ -91,21 +114,29 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     constexpr int TILE = THREADS * ITEMS;
     constexpr uint32_t MASK = RADIX - 1;
     static_assert(THREADS >= RADIX && TILE >= kSortTile, "one thread per digit; status sized for kSortTile");
+    // lookback: LB_SUB threads per digit, each reading LB_EACH predecessors
+    constexpr int LB_SUB = THREADS / RADIX < 4 ? THREADS / RADIX : 4;
+    constexpr int LB_EACH = kLookbackEach;
+    constexpr int LB_SPAN = LB_SUB * LB_EACH;
     constexpr int kCountBytes = WAVES * RADIX * 4;
-    constexpr int kShared = kCountBytes > TILE ? kCountBytes : TILE;
+    static_assert(LB_SUB * RADIX * 8 <= kCountBytes, "lookback partials fit the count memory");
     __shared__ KeyT s_stage[TILE];
-    __shared__ __align__(16) uint8_t s_shared[kShared];
+    __shared__ __align__(16) uint32_t s_shared[kCountBytes / 4];
     __shared__ uint32_t s_delta[RADIX];
-    __shared__ uint32_t s_tmp[WAVES];
+    __shared__ uint32_t s_tmp[2 * WAVES];
     __shared__ uint32_t s_part;
+    __shared__ uint32_t s_lb_open[2];
     uint32_t (*s_wave_hist)[RADIX] = reinterpret_cast<uint32_t (*)[RADIX]>(s_shared);
-    uint8_t* s_digit = s_shared;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
-    if (tid == 0) s_part = atomicAdd(ticket, 1u);
-    for (int k = tid; k < WAVES * RADIX; k += THREADS) (&s_wave_hist[0][0])[k] = 0u;
+    if (tid == 0) {
+        s_part = atomicAdd(ticket, 1u);
+        s_lb_open[0] = 0u;
+        s_lb_open[1] = 0u;
+    }
+    for (int k = tid; k < WAVES * RADIX; k += THREADS) s_shared[k] = 0u;
     __syncthreads();
     const uint32_t part = s_part;
     const uint32_t n = *n_dev;
@@ -113,18 +144,37 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     // uniform over the block: past the keys, or more keys than the buffers hold
     // (the host re-runs the frame with larger buffers in that case)
     if (tile_base >= n || n > n_max) return;
+    // the digit's global start (same for every tile) is loaded with the keys
+    const uint32_t hist_d = tid < RADIX ? hist_pass[tid] : 0u;
     // Positions in the tile are u32 (< TILE): item it of this thread is tile
     // position lpos + 64 it, valid below tile_n.
     const uint32_t tile_n = (uint32_t)min((size_t)TILE, (size_t)n - tile_base);
     auto in_tile = [&](uint32_t j) -> bool { return j < tile_n; };
     const uint32_t lpos = (uint32_t)(wave * (kWave * ITEMS) + lane);
     const KeyT* __restrict__ kin_t = kin + tile_base;
+    // Keys, values (and a carried array) are loaded together, the values
+    // arriving while the keys are ranked. The loads are branch-free (a
+    // position past the tile's keys reads the tile's last key, then takes the
+    // maximum key), so the ranking waits for the keys alone.
     KeyT k[ITEMS];
-    uint32_t rank[ITEMS];
+    uint32_t rank[ITEMS], val[ITEMS], aux[ITEMS];
+    const uint32_t* __restrict__ vin_t = vin + tile_base;
+    const uint32_t* __restrict__ aux_t = aux_in + tile_base;
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = lpos + (uint32_t)(it * kWave);
-        k[it] = in_tile(j) ? kin_t[j] : (KeyT)~(KeyT)0;
+        k[it] = kin_t[min(j, tile_n - 1u)];
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t j = lpos + (uint32_t)(it * kWave);
+        const uint32_t jc = min(j, tile_n - 1u);
+        val[it] = IOTA ? (uint32_t)(tile_base + j) : vin_t[jc];
+        if constexpr (AUX == kAuxCarry) aux[it] = aux_t[jc];
+    }
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        if (!in_tile(lpos + (uint32_t)(it * kWave))) k[it] = (KeyT)~(KeyT)0;
     }
 
     // Wave-level multisplit: per item, BITS ballots narrow each lane's group of
@@ -176,8 +226,8 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
         store_status(&status[(size_t)part * RADIX + tid],
                      status_word(epoch, part == 0 ? kFlagPrefix : kFlagAggregate, tile_count));
     }
-    const uint32_t local_start = block_exclusive_scan<THREADS>(tid < RADIX ? tile_count : 0u, s_tmp);
-    const uint32_t global_start = block_exclusive_scan<THREADS>(tid < RADIX ? hist_pass[tid] : 0u, s_tmp);
+    uint32_t local_start, global_start;
+    block_exclusive_scan2<THREADS>(tid < RADIX ? tile_count : 0u, hist_d, s_tmp, local_start, global_start);
     if (tid < RADIX) {
 #pragma unroll
         for (int w = 0; w < WAVES; w++) s_wave_hist[w][tid] += local_start;
@@ -185,53 +235,77 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     __syncthreads();
 
     // Position in the tile (tile-local, known before the lookback); then the
-    // counts' memory is free for the digit bytes. Keys go to LDS now and the
-    // value loads are issued now, so both leave the registers / hide their
-    // latency while the lookback waits on predecessor tiles.
-    uint32_t dig[ITEMS];
+    // counts' memory is free for the lookback. Keys go to LDS now, so they
+    // leave the registers while the lookback waits on predecessor tiles.
 #pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        dig[it] = (uint32_t)(k[it] >> shift) & MASK;
-        rank[it] += s_wave_hist[wave][dig[it]];
-    }
+    for (int it = 0; it < ITEMS; it++) rank[it] += s_wave_hist[wave][(uint32_t)(k[it] >> shift) & MASK];
     __syncthreads();
-    uint32_t val[ITEMS], aux[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const uint32_t j = lpos + (uint32_t)(it * kWave);
-        if (in_tile(j)) {
-            s_stage[rank[it]] = k[it];
-            s_digit[rank[it]] = (uint8_t)dig[it];
-            val[it] = IOTA ? (uint32_t)(tile_base + j) : vin[tile_base + j];
-            if constexpr (AUX == kAuxCarry) aux[it] = aux_in[tile_base + j];
-        }
+        if (in_tile(lpos + (uint32_t)(it * kWave))) s_stage[rank[it]] = k[it];
     }
 
-    if (tid < RADIX) {
-        // decoupled lookback over kLookbackGroup predecessors per round trip
-        uint32_t excl = 0;
+    // Decoupled lookback, LB_SPAN predecessors per round trip: thread
+    // (sub, d) with sub < LB_SUB folds predecessors q - sub*LB_EACH - i
+    // (i < LB_EACH) of digit d, nearest first, up to and including the first
+    // inclusive prefix; the digit's thread (sub 0) then folds the LB_SUB
+    // partial sums in order and stops at the first that met a prefix. The
+    // round repeats while a digit is open; tile 0 publishes a prefix, so every
+    // digit closes by then. (One wavefront per sub and digit range: each load
+    // instruction reads one tile's status words, 512 contiguous bytes.)
+    uint32_t excl = 0;
+    if (part != 0) {
+        uint32_t* s_lb_sum = s_shared;                      // [LB_SUB][RADIX]
+        uint32_t* s_lb_met = s_shared + LB_SUB * RADIX;     // [LB_SUB][RADIX]
+        const uint32_t lb_d = (uint32_t)tid & MASK;
+        const int lb_sub = tid / RADIX;
+        bool closed = false;  // digit lb_d met a prefix (sub-0 threads)
         int64_t q = (int64_t)part - 1;  // nearest predecessor not yet folded in
-        while (q >= 0) {
-            uint64_t w[kLookbackGroup];
+        for (int round = 0;; round++) {
+            if (lb_sub < LB_SUB) {
+                const int64_t q0 = q - (int64_t)lb_sub * LB_EACH;
+                uint64_t w[LB_EACH];
 #pragma unroll
-            for (int i = 0; i < kLookbackGroup; i++)
-                w[i] = (q - i >= 0) ? load_status(&status[(size_t)(q - i) * RADIX + tid]) : 0ull;
-            bool stop = false;
+                for (int i = 0; i < LB_EACH; i++)
+                    w[i] = (q0 - i >= 0) ? load_status(&status[(size_t)(q0 - i) * RADIX + lb_d]) : 0ull;
+                uint32_t sum = 0;
+                bool met = false;
 #pragma unroll
-            for (int i = 0; i < kLookbackGroup; i++) {
-                if (stop || q - i < 0) continue;
-                uint64_t x = w[i];
-                uint32_t flag = ((uint32_t)(x >> 34) == epoch) ? (uint32_t)((x >> 32) & 3u) : 0u;
-                while (flag == 0) {
-                    x = load_status(&status[(size_t)(q - i) * RADIX + tid]);
-                    flag = ((uint32_t)(x >> 34) == epoch) ? (uint32_t)((x >> 32) & 3u) : 0u;
+                for (int i = 0; i < LB_EACH; i++) {
+                    if (met || q0 - i < 0) continue;
+                    uint64_t x = w[i];
+                    uint32_t flag = ((uint32_t)(x >> 34) == epoch) ? (uint32_t)((x >> 32) & 3u) : 0u;
+                    while (flag == 0) {
+                        x = load_status(&status[(size_t)(q0 - i) * RADIX + lb_d]);
+                        flag = ((uint32_t)(x >> 34) == epoch) ? (uint32_t)((x >> 32) & 3u) : 0u;
+                    }
+                    sum += (uint32_t)x;
+                    met = flag == kFlagPrefix;
                 }
-                excl += (uint32_t)x;
-                stop = flag == kFlagPrefix;
+                s_lb_sum[lb_sub * RADIX + lb_d] = sum;
+                s_lb_met[lb_sub * RADIX + lb_d] = met ? 1u : 0u;
             }
-            if (stop) break;
-            q -= kLookbackGroup;
+            __syncthreads();
+            // s_lb_open[round & 1] collects "some digit still open"; the
+            // other slot is cleared for the next round (its readers are past
+            // the barrier above)
+            if (tid < RADIX) {
+#pragma unroll
+                for (int sub = 0; sub < LB_SUB; sub++) {
+                    if (!closed) {
+                        excl += s_lb_sum[sub * RADIX + tid];
+                        closed = s_lb_met[sub * RADIX + tid] != 0u;
+                    }
+                }
+                if (!closed) s_lb_open[round & 1] = 1u;
+            }
+            if (tid == 0) s_lb_open[(round + 1) & 1] = 0u;
+            q -= LB_SPAN;
+            __syncthreads();
+            if (s_lb_open[round & 1] == 0u || q < 0) break;
         }
+    }
+    if (tid < RADIX) {
         if (part != 0)
             store_status(&status[(size_t)part * RADIX + tid],
                          status_word(epoch, kFlagPrefix, excl + tile_count));
@@ -254,8 +328,9 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
         if (in_tile(j)) {
-            dst[it] = scatter_index(s_delta[s_digit[j]], j);
-            kout[dst[it]] = s_stage[j];
+            const KeyT key = s_stage[j];
+            dst[it] = scatter_index(s_delta[(uint32_t)(key >> shift) & MASK], j);
+            kout[dst[it]] = key;
         }
     }
     __syncthreads();
