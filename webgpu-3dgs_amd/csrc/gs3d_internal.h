@@ -186,6 +186,15 @@ hipError_t launch_scan_rects(const ScanRectsArgs& a, hipStream_t s);
 // each 32-bit half as one v_bitop3 (truth table 0x90 for src0 & ~(src1 ^
 // src2), with src0, src1, src2 standing for 0xF0, 0xCC, 0xAA). Bits at or
 // above match_bits (uniform) are zero in every digit and are skipped.
+// Bit b of d as an all-ones/zero mask. The empty asm hides the mask's origin
+// from the optimizer, so its ballot is one compare of the mask with zero
+// instead of a shift of d and a compare beside the bit extract.
+__device__ __forceinline__ uint32_t digit_bit_mask(uint32_t d, int b) {
+    uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);
+    asm("" : "+v"(m));
+    return m;
+}
+
 template <int BITS>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, uint32_t match_bits = BITS) {
     const uint64_t vm = __ballot(valid);
@@ -194,7 +203,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid, uint32_t
 #pragma unroll
     for (int b = 0; b < BITS; b++) {
         if ((uint32_t)b < mb) {
-            const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d, b, 1);  // all ones if bit b set
+            const uint32_t m = digit_bit_mask(d, b);  // all ones if bit b set
             const uint64_t bal = __ballot(m != 0u);
             lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)bal, m, 0x90);
             hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(bal >> 32), m, 0x90);
@@ -216,8 +225,8 @@ __device__ __forceinline__ void match_digit2(uint32_t d0, bool valid0, uint32_t 
 #pragma unroll
     for (int b = 0; b < BITS; b++) {
         if (FULL || (uint32_t)b < match_bits) {
-            const uint32_t m0 = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d0, b, 1);  // all ones if bit b set
-            const uint32_t m1 = (uint32_t)__builtin_amdgcn_sbfe((int32_t)d1, b, 1);
+            const uint32_t m0 = digit_bit_mask(d0, b);  // all ones if bit b set
+            const uint32_t m1 = digit_bit_mask(d1, b);
             const uint64_t bal0 = __ballot(m0 != 0u);
             const uint64_t bal1 = __ballot(m1 != 0u);
             lo0 = __builtin_amdgcn_bitop3_b32(lo0, (uint32_t)bal0, m0, 0x90);
