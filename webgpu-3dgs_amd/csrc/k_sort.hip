@@ -183,15 +183,16 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     // Two items share each ballot step (match_digit2); their counts are then
     // taken in item order.
     static_assert(ITEMS % 2 == 0, "items are matched in pairs");
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t mb = __builtin_amdgcn_readfirstlane(match_bits);
     auto count = [&](int it, uint64_t peers, bool valid, uint32_t d) {
         uint32_t r = 0;
         if (valid) {
-            const uint64_t below = peers & lt_mask;
+            // peers in lanes below this one (v_mbcnt: no mask of lower lanes)
+            const uint32_t below =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
             const uint32_t prior = s_wave_hist[wave][d];
-            r = prior + (uint32_t)__popcll(below);
-            if (below == 0ull) s_wave_hist[wave][d] = prior + (uint32_t)__popcll(peers);
+            r = prior + below;
+            if (below == 0u) s_wave_hist[wave][d] = prior + (uint32_t)__popcll(peers);
         }
         rank[it] = r;
         __builtin_amdgcn_wave_barrier();
