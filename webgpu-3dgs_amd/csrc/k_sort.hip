@@ -38,8 +38,8 @@ __device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Two exclusive scans over the block with one pair of barriers; s_tmp holds
-// 2 * (THREADS / 64) words.
+// Two exclusive scans over the block with one barrier; s_tmp holds
+// 2 * (THREADS / 64) words and is used once per workgroup.
 template <int THREADS>
 __device__ __forceinline__ void block_exclusive_scan2(uint32_t x, uint32_t y, uint32_t* s_tmp, uint32_t& ex,
                                                       uint32_t& ey) {
@@ -61,17 +61,21 @@ __device__ __forceinline__ void block_exclusive_scan2(uint32_t x, uint32_t y, ui
         s_tmp[WAVES + wave] = b;
     }
     __syncthreads();
+    // the totals of the wavefronts before this one: lane l < wave reads wave
+    // l's, a butterfly over the first WAVES lanes sums them, lane 0 has it
+    // (one LDS read per lane instead of all WAVES totals held in registers)
     uint32_t pa = 0, pb = 0;
-#pragma unroll
-    for (int w = 0; w < WAVES; w++) {
-        if (w < wave) {
-            pa += s_tmp[w];
-            pb += s_tmp[WAVES + w];
-        }
+    if (lane < wave) {
+        pa = s_tmp[lane];
+        pb = s_tmp[WAVES + lane];
     }
-    __syncthreads();
-    ex = pa + a - x;
-    ey = pb + b - y;
+#pragma unroll
+    for (int d = 1; d < WAVES; d <<= 1) {
+        pa += __shfl_xor(pa, d, kWave);
+        pb += __shfl_xor(pb, d, kWave);
+    }
+    ex = __builtin_amdgcn_readfirstlane(pa) + a - x;
+    ey = __builtin_amdgcn_readfirstlane(pb) + b - y;
 }
 
 // Lookback: per round trip each of up to 4 threads per digit reads the status
@@ -116,7 +120,10 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     static_assert(THREADS >= RADIX && TILE >= kSortTile, "one thread per digit; status sized for kSortTile");
     // lookback: LB_SUB threads per digit, each reading LB_EACH predecessors
     constexpr int LB_SUB = THREADS / RADIX < 4 ? THREADS / RADIX : 4;
-    constexpr int LB_EACH = kLookbackEach;
+    // the 8-key shapes read 4 words per thread and round: with the other
+    // register savings they stay at or under 64 VGPRs, so two workgroups
+    // share a CU (one's loads overlap the other's ranking)
+    constexpr int LB_EACH = ITEMS >= 16 ? kLookbackEach : kLookbackEach / 2;
     constexpr int LB_SPAN = LB_SUB * LB_EACH;
     constexpr int kCountBytes = WAVES * RADIX * 4;
     static_assert(LB_SUB * RADIX * 8 <= kCountBytes, "lookback partials fit the count memory");
@@ -154,22 +161,20 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     const KeyT* __restrict__ kin_t = kin + tile_base;
     // Keys, values (and a carried array) are loaded together, the values
     // arriving while the keys are ranked. The loads are branch-free (a
-    // position past the tile's keys reads the tile's last key, then takes the
-    // maximum key), so the ranking waits for the keys alone.
+    // position past the tile's keys reads the tile's last element; its key is
+    // then replaced by the maximum key), so the ranking waits for the keys
+    // alone. Values of an IOTA pass are positions, formed where they are
+    // staged.
     KeyT k[ITEMS];
     uint32_t rank[ITEMS], val[ITEMS], aux[ITEMS];
     const uint32_t* __restrict__ vin_t = vin + tile_base;
     const uint32_t* __restrict__ aux_t = aux_in + tile_base;
 #pragma unroll
-    for (int it = 0; it < ITEMS; it++) {
-        const uint32_t j = lpos + (uint32_t)(it * kWave);
-        k[it] = kin_t[min(j, tile_n - 1u)];
-    }
+    for (int it = 0; it < ITEMS; it++) k[it] = kin_t[min(lpos + (uint32_t)(it * kWave), tile_n - 1u)];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const uint32_t j = lpos + (uint32_t)(it * kWave);
-        const uint32_t jc = min(j, tile_n - 1u);
-        val[it] = IOTA ? (uint32_t)(tile_base + j) : vin_t[jc];
+        const uint32_t jc = min(lpos + (uint32_t)(it * kWave), tile_n - 1u);
+        if constexpr (!IOTA) val[it] = vin_t[jc];
         if constexpr (AUX == kAuxCarry) aux[it] = aux_t[jc];
     }
 #pragma unroll
@@ -324,13 +329,13 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     }
     // keys out in LDS order (runs of equal digits are contiguous stores); the
     // output index of each sorted position is kept for the values (and aux)
-    size_t dst[ITEMS];
+    uint32_t dst[ITEMS];  // below n
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
         const uint32_t j = (uint32_t)(it * THREADS + tid);
         if (in_tile(j)) {
             const KeyT key = s_stage[j];
-            dst[it] = scatter_index(s_delta[(uint32_t)(key >> shift) & MASK], j);
+            dst[it] = (uint32_t)scatter_index(s_delta[(uint32_t)(key >> shift) & MASK], j);
             kout[dst[it]] = key;
         }
     }
@@ -339,7 +344,8 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     uint32_t* s_vals = reinterpret_cast<uint32_t*>(s_stage);
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        if (in_tile(lpos + (uint32_t)(it * kWave))) s_vals[rank[it]] = val[it];
+        const uint32_t j = lpos + (uint32_t)(it * kWave);
+        if (in_tile(j)) s_vals[rank[it]] = IOTA ? (uint32_t)(tile_base + j) : val[it];
     }
     __syncthreads();
 #pragma unroll
