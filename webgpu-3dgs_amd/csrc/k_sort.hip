@@ -16,10 +16,10 @@
 //      bumps the wave's private LDS histogram;
 //   3. per digit: the tile's count is published as an aggregate, the workgroup
 //      looks back over predecessor tiles' status words (decoupled lookback,
-//      8 predecessors per thread and up to 4 threads per digit, so up to 32
-//      per round trip) for
-//      its exclusive prefix, then publishes the inclusive prefix; the digit's
-//      global start comes from the digit histograms fused into emit;
+//      up to 4 threads per digit reading 8 predecessors each, 4 in the 8-key
+//      shapes) for its exclusive prefix, then publishes the inclusive prefix;
+//      the digit's global start comes from the digit histograms fused into
+//      emit;
 //   4. keys, then values, are placed digit-sorted in LDS and written out in
 //      LDS order, so runs of equal digits leave as contiguous stores.
 // HBM traffic per pass: 12 B read + 12 B written per 64-bit key (8 + 8 for
