@@ -51,20 +51,22 @@ def algorithmic_bytes(n, k, tiles, pixels, depth_passes, tile_passes, deg, packe
     keys, and rectangles, only), plus one read of the keys to histogram;
     scan: the depth key and the rectangle (packed: read in order; else the id
     and the 8-B rectangle) of each splat in, a 16-B record per splat out; emit: the records and ids in,
-    8-B (tile id, splat id) out; key sort: 16 B per key per pass (32-bit tile id
-    + 32-bit splat id, read and written); identify: one (T+1)-entry write plus, per tile, the ceil(log2 K)
+    (tile id, 32-bit splat id) out; key sort: per key and pass the tile id and
+    the splat id, read and written (tile ids are 16-bit up to 65,536 tiles, so
+    12 B per key and pass, else 32-bit, 16 B); identify: one (T+1)-entry write plus, per tile, the ceil(log2 K)
     sorted keys its binary search reads; raster: the 48 B used of each listed
     splat, tile ranges, 8-B pixel out.
     """
     pre_in = 64 + (12 * (deg + 1) ** 2 if deg else 0)
     rect = 4 if packed else 8
     carry = 4 if packed else 0
+    tid = 2 if tiles <= 65536 else 4  # bytes per tile id in emit's output and the tile-id passes
     return {
         "preprocess-splats": n * (pre_in + 64 + 4 + 4 + rect),
         "radix-sort-splats": n * (4 + 12 + 2 * carry + (16 + 2 * carry) * (depth_passes - 1)) if depth_passes else 0,
         "prefix-scan": (8 if packed else 16) * n + 16 * (n + 1),
-        "emit-tile-refs": 16 * (n + 1) + 4 * n + 8 * k,
-        "radix-sort": tile_passes * 16 * k,
+        "emit-tile-refs": 16 * (n + 1) + 4 * n + (tid + 4) * k,
+        "radix-sort": tile_passes * 2 * (tid + 4) * k,
         "identify-tile-ranges": 8 * tiles * max(1, int(k).bit_length()) + 4 * (tiles + 1),
         "rasterize-tiles": 48 * k + 8 * (tiles + 1) + 8 * pixels,
     }

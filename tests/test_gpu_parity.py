@@ -238,13 +238,14 @@ def test_pipeline_no_splats(renderer):
 
 
 @pytest.mark.parametrize("W,H,tile", [(1, 1, 16), (16, 16, 16), (17, 9, 8), (200, 100, 32), (33, 700, 16),
-                                      (4096, 40, 16), (4200, 40, 16)])
+                                      (4096, 40, 16), (4200, 40, 16), (4096, 4096, 16), (4096, 4112, 16)])
 def test_pipeline_tiny_and_ragged_viewports(renderer, W, H, tile):
     """Viewports of one tile (no tile-digit pass at all), a few tiles (one pass, so the
     result ends in the second buffer), partial tiles at the right and bottom edges, a
-    tall, narrow one, and wide ones at and past 256 tiles per row (tile rectangles
-    packed 8 bits per coordinate up to 256, gathered at full width past it): every
-    buffer bit-exact against the oracle."""
+    tall, narrow one, wide ones at and past 256 tiles per row (tile rectangles
+    packed 8 bits per coordinate up to 256, gathered at full width past it), and
+    65,536 tiles (the last frame whose tile ids sort as 16 bits, top id 65,535) and
+    65,792 (32-bit tile ids): every buffer bit-exact against the oracle."""
     say(f"pipeline viewport {W}x{H} tile={tile}")
     s = sc.activate(sc.uniform_cube(44, 3_000, log_scale_mean=-3.0))
     img, ref, err = rebin_and_compare(renderer, s, (0.0, 0.0, 2.5), W, H, tile)

@@ -144,6 +144,7 @@ struct EmitArgs {
     const uint4* recs;        // n + 1 records of k_scan_rects, or null
     uint64_t* keys;           // (tile << 32 | depth key) per key, or null when tiles is set
     uint32_t* tiles;          // or the tile id alone (the depth order is already in the stream)
+    uint16_t* tiles16;        // or the tile id as 16 bits (frames of at most 65,536 tiles)
     uint32_t* values;
     uint32_t* hist;           // [passes][radix] digit histograms (accumulated), or null
     uint32_t* hist_partial;   // [emit_grid(n)][passes][radix] scratch rows, or null
@@ -287,15 +288,20 @@ inline uint32_t pass_match_bits(uint32_t shift, uint32_t digit_bits, uint32_t ke
 // epoch_base[0] += kEpochStride (the first node of a captured batch graph).
 hipError_t launch_bump_epoch(uint32_t* epoch_base, hipStream_t s);
 // tile_offsets[0..tile_count] from n = *n_dev sorted keys (n > n_max: all 0):
-// 64-bit (tile << 32 | depth) keys, or 32-bit tile ids when key_bytes is 4.
+// 64-bit (tile << 32 | depth) keys, or tile ids alone when key_bytes is 4
+// (32-bit) or 2 (16-bit).
 // If k_host is set (host-mapped memory), *k_host = *n_dev.
 hipError_t launch_identify_tiles(const void* keys, uint32_t key_bytes, const uint32_t* n_dev, uint32_t n_max,
                                  uint32_t* tile_offsets, uint32_t tile_count, uint32_t* k_host,
                                  hipStream_t s);
 // keys[i] = (tiles[i] << 32) | bits(-projected[values[i]].depth) for i < *n_dev
-// (the 64-bit sort keys, rebuilt from the tile-id sort for readback).
-hipError_t launch_rebuild_keys(const uint32_t* tiles, const uint32_t* values, const gs3d_projected_splat* projected,
-                               const uint32_t* n_dev, uint32_t n_max, uint64_t* keys, hipStream_t s);
+// (the 64-bit sort keys, rebuilt from the tile-id sort for readback); tile ids
+// of tile_bytes 4 or 2.
+hipError_t launch_rebuild_keys(const void* tiles, uint32_t tile_bytes, const uint32_t* values,
+                               const gs3d_projected_splat* projected, const uint32_t* n_dev, uint32_t n_max,
+                               uint64_t* keys, hipStream_t s);
+// Bytes per tile id in the frame's tile-id sort: 2 up to 65,536 tiles, else 4.
+inline uint32_t tile_id_bytes(uint32_t tiles) { return tiles <= 65536u ? 2u : 4u; }
 // Square tiles of 8, 16 or 32 px; pixels_per_lane 1, 2 or 4 (a workgroup of
 // tile*tile/ppl threads, at least one wavefront).
 bool raster_variant_exists(uint32_t tile, uint32_t pixels_per_lane);

@@ -176,7 +176,8 @@ __device__ __forceinline__ void emit_chunk(const EmitArgs& a, uint32_t i0, EmitS
                     const uint64_t key = ((uint64_t)tile << 32) | (uint64_t)keylo;
                     const uint64_t pos = (uint64_t)block_start + w0 + slot;
                     if (pos < a.capacity) {
-                        if (a.tiles) a.tiles[pos] = tile;
+                        if (a.tiles16) a.tiles16[pos] = (uint16_t)tile;
+                        else if (a.tiles) a.tiles[pos] = tile;
                         else a.keys[pos] = key;
                         a.values[pos] = sid;
                     }

@@ -38,8 +38,9 @@ namespace {
 constexpr uint32_t kWideSearchKeys = 32u << 20;
 
 // The tile id of sorted key i sits at word stride * i + word (64-bit keys:
-// stride 2, the high word; 32-bit tile ids: stride 1).
-__global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ kw, uint32_t stride,
+// stride 2, the high 32-bit word; tile ids alone, W of 32 or 16 bits: stride 1).
+template <typename W>
+__global__ __launch_bounds__(256) void k_identify(const W* __restrict__ kw, uint32_t stride,
                                                   uint32_t word, const uint32_t* __restrict__ n_dev,
                                                   uint32_t* __restrict__ tile_offsets,
                                                   uint32_t tile_count, uint32_t n_max,
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ k
     if (n > n_max) n = 0;  // keys did not fit: draw nothing, the host re-runs the frame
     uint32_t out = 0;
     if (n > 0) {
-        const uint32_t last = kw[(size_t)stride * (n - 1) + word];
+        const uint32_t last = (uint32_t)kw[(size_t)stride * (n - 1) + word];
         if (t <= last + 1u) {
             // first index whose tile id >= t, in [lo, hi]. While the range is
             // wide, 7 independent probes p_j = lo + j*step (all < hi <= n) cut it
@@ -65,14 +66,15 @@ __global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ k
                 const uint32_t step = (hi - lo) >> 3;
                 uint32_t c = 0;
 #pragma unroll
-                for (uint32_t j = 1; j < 8; j++) c += kw[(size_t)stride * (lo + j * step) + word] < t ? 1u : 0u;
+                for (uint32_t j = 1; j < 8; j++)
+                    c += (uint32_t)kw[(size_t)stride * (lo + j * step) + word] < t ? 1u : 0u;
                 const uint32_t nlo = c ? lo + c * step + 1u : lo;
                 hi = c < 7u ? lo + (c + 1u) * step : hi;
                 lo = nlo;
             }
             while (lo < hi) {
                 const uint32_t mid = lo + ((hi - lo) >> 1);
-                if (kw[(size_t)stride * mid + word] < t) lo = mid + 1; else hi = mid;
+                if ((uint32_t)kw[(size_t)stride * mid + word] < t) lo = mid + 1; else hi = mid;
             }
             out = lo;
         }
@@ -80,7 +82,8 @@ __global__ __launch_bounds__(256) void k_identify(const uint32_t* __restrict__ k
     tile_offsets[t] = out;
 }
 
-__global__ __launch_bounds__(256) void k_rebuild_keys(const uint32_t* __restrict__ tiles,
+template <typename W>
+__global__ __launch_bounds__(256) void k_rebuild_keys(const W* __restrict__ tiles,
                                                       const uint32_t* __restrict__ values,
                                                       const gs3d_projected_splat* __restrict__ projected,
                                                       const uint32_t* __restrict__ n_dev, uint32_t n_max,
@@ -732,18 +735,31 @@ hipError_t launch_identify_tiles(const void* keys, uint32_t key_bytes, const uin
     const uint32_t threads = tile_count + 1;
     const uint32_t stride = key_bytes == 8 ? 2u : 1u, word = key_bytes == 8 ? 1u : 0u;
     (void)threads;
-    hipLaunchKernelGGL(k_identify, dim3((uint32_t)identify_launch_shape(tile_count).grid), dim3(256), 0, s,
-                       static_cast<const uint32_t*>(keys), stride, word, n_dev, tile_offsets, tile_count, n_max,
-                       k_host);
+    const dim3 grid((uint32_t)identify_launch_shape(tile_count).grid);
+    if (key_bytes == 2) {
+        if (tile_count > 65536u) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_identify<uint16_t>, grid, dim3(256), 0, s, static_cast<const uint16_t*>(keys), 1u, 0u,
+                           n_dev, tile_offsets, tile_count, n_max, k_host);
+    } else {
+        hipLaunchKernelGGL(k_identify<uint32_t>, grid, dim3(256), 0, s, static_cast<const uint32_t*>(keys), stride,
+                           word, n_dev, tile_offsets, tile_count, n_max, k_host);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_rebuild_keys(const uint32_t* tiles, const uint32_t* values, const gs3d_projected_splat* projected,
-                               const uint32_t* n_dev, uint32_t n_max, uint64_t* keys, hipStream_t s) {
+hipError_t launch_rebuild_keys(const void* tiles, uint32_t tile_bytes, const uint32_t* values,
+                               const gs3d_projected_splat* projected, const uint32_t* n_dev, uint32_t n_max,
+                               uint64_t* keys, hipStream_t s) {
     if (n_max == 0) return hipSuccess;
     const uint32_t blocks = std::min<uint32_t>((n_max + 255u) / 256u, 8192u);
-    hipLaunchKernelGGL(k_rebuild_keys, dim3(blocks), dim3(256), 0, s, tiles, values, projected, n_dev, n_max,
-                       keys);
+    if (tile_bytes == 2)
+        hipLaunchKernelGGL(k_rebuild_keys<uint16_t>, dim3(blocks), dim3(256), 0, s,
+                           static_cast<const uint16_t*>(tiles), values, projected, n_dev, n_max, keys);
+    else if (tile_bytes == 4)
+        hipLaunchKernelGGL(k_rebuild_keys<uint32_t>, dim3(blocks), dim3(256), 0, s,
+                           static_cast<const uint32_t*>(tiles), values, projected, n_dev, n_max, keys);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

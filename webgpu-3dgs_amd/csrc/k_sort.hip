@@ -23,7 +23,7 @@
 //   4. keys, then values, are placed digit-sorted in LDS and written out in
 //      LDS order, so runs of equal digits leave as contiguous stores.
 // HBM traffic per pass: 12 B read + 12 B written per 64-bit key (8 + 8 for
-// 32-bit keys). Tiles beyond
+// 32-bit keys, 6 + 6 for 16-bit keys). Tiles beyond
 // n = *n_dev exit at once, so a graph-captured launch can size the grid to the
 // buffer capacity while the key count lives on the device.
 #include "gs3d_internal.h"
@@ -126,7 +126,10 @@ __global__ __launch_bounds__(THREADS) void k_radix_pass(
     constexpr int LB_SPAN = LB_SUB * LB_EACH;
     constexpr int kCountBytes = WAVES * RADIX * 4;
     static_assert(LB_SUB * RADIX * 8 <= kCountBytes, "lookback partials fit the count memory");
-    __shared__ KeyT s_stage[TILE];
+    // staging: one KeyT per key, and at least 4 B per key, since the values
+    // reuse it as u32 (16-bit tile ids stage in the low half)
+    __shared__ __align__(16) uint32_t s_stage_words[TILE * (sizeof(KeyT) > 4 ? sizeof(KeyT) : 4) / 4];
+    KeyT* const s_stage = reinterpret_cast<KeyT*>(s_stage_words);
     __shared__ __align__(16) uint32_t s_shared[kCountBytes / 4];
     __shared__ uint32_t s_delta[RADIX];
     __shared__ uint32_t s_tmp[2 * WAVES];
@@ -463,7 +466,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
     if (match_bits == 0 || match_bits > digit_bits) match_bits = digit_bits;
     // key_tile: keys per workgroup, 0 = the default shape for the key width
     // (16,384 for 64-bit keys, 8,192 for 32-bit keys); 64-bit: 8,192 or
-    // 16,384; 32-bit: 4,096, 8,192 or 16,384 (16,384 not with aux)
+    // 16,384; 32- and 16-bit: 4,096, 8,192 or 16,384 (16,384 not with aux)
     int shape;  // -1 the half-size tile, 0 the default, 1 the double-size tile
     if (key_bytes == 8) {
         if (key_tile != 0 && key_tile != 8192 && key_tile != 16384) return hipErrorInvalidValue;
@@ -473,10 +476,12 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
         shape = key_tile == 4096 ? -1 : (key_tile == 16384 ? 1 : 0);
     }
     const bool aux = aux_in != nullptr && aux_out != nullptr;
-    // aux rides on 32-bit key passes of the 4,096- and 8,192-key shapes; the
+    // 16-bit keys (tile ids of frames up to 65,536 tiles) take the 32-bit
+    // shapes; aux rides on 32-bit key passes of the 4,096- and 8,192-key shapes; the
     // gather also needs the values read from vin
     if (aux && (key_bytes != 4 || shape > 0 || (!aux_carry && iota_values))) return hipErrorInvalidValue;
-    const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) + (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : 0) +
+    const int key = (digit_bits == 8 ? 100 : digit_bits == 4 ? 200 : 0) +
+                    (key_bytes == 8 ? 10 : key_bytes == 4 ? 20 : key_bytes == 2 ? 30 : 0) +
                     (iota_values ? 1 : 0) + (aux ? (aux_carry ? 4 : 2) : 0);
 #define GS3D_PASS(B, K, I, A)                                                                      \
     launch_shape<B, K, I, A>(shape, kin, vin, kout, vout, n_dev, n_max, hist_pass, status, ticket, shift, \
@@ -489,6 +494,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
         case 122: GS3D_PASS(8, uint32_t, false, kAuxGather); break;
         case 124: GS3D_PASS(8, uint32_t, false, kAuxCarry); break;
         case 125: GS3D_PASS(8, uint32_t, true, kAuxCarry); break;
+        case 130: GS3D_PASS(8, uint16_t, false, kAuxNone); break;
         case 210: GS3D_PASS(4, uint64_t, false, kAuxNone); break;
         case 211: GS3D_PASS(4, uint64_t, true, kAuxNone); break;
         case 220: GS3D_PASS(4, uint32_t, false, kAuxNone); break;
@@ -496,6 +502,7 @@ hipError_t launch_radix_pass(const void* kin, const uint32_t* vin, void* kout, u
         case 222: GS3D_PASS(4, uint32_t, false, kAuxGather); break;
         case 224: GS3D_PASS(4, uint32_t, false, kAuxCarry); break;
         case 225: GS3D_PASS(4, uint32_t, true, kAuxCarry); break;
+        case 230: GS3D_PASS(4, uint16_t, false, kAuxNone); break;
         default: return hipErrorInvalidValue;
     }
 #undef GS3D_PASS

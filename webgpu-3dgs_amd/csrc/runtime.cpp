@@ -84,6 +84,7 @@ struct gs3d_ctx {
     uint32_t n_passes = 0;
     uint32_t shifts[kMaxPasses] = {};
     bool result_in_b = false;
+    uint32_t frame_tile_bytes = 4;  // bytes per tile id in the last frame's tile-id sort (2 or 4)
     bool keep_unsorted = false;
     uint32_t epoch = 0;
     uint32_t debug_mode = 0, debug_ref = 256;  // setDebugMode, debugRef
@@ -516,7 +517,11 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     ea.projected = projected;
     ea.order = order;
     ea.recs = recs;
-    ea.tiles = reinterpret_cast<uint32_t*>(ka);  // tile ids; the sort_keys buffers hold them as u32
+    // tile ids; the sort_keys buffers hold them as u16 up to 65,536 tiles, else u32
+    const uint32_t tb = tile_id_bytes(T);
+    c->frame_tile_bytes = tb;
+    ea.tiles = tb == 4 ? reinterpret_cast<uint32_t*>(ka) : nullptr;
+    ea.tiles16 = tb == 2 ? reinterpret_cast<uint16_t*>(ka) : nullptr;
     ea.values = va;
     ea.hist = tile_passes ? hist_tile : nullptr;
     ea.hist_partial = tile_passes ? hist_partial : nullptr;
@@ -533,15 +538,15 @@ int enqueue_rebin(gs3d_ctx* c, const gs3d_camera* cam, uint32_t W, uint32_t H, u
     // the tile digits of the keys are the digits of the 32-bit tile ids, shifted by 32
     uint32_t tile_shifts[kMaxPasses];
     for (uint32_t p = 0; p < tile_passes; p++) tile_shifts[p] = c->shifts[depth_passes + p] - 32u;
-    GS3D_TRY(run_passes(c, ka, va, kb, vb, 4, false, ref_counter, cap, tile_shifts, tile_passes, hist_tile,
+    GS3D_TRY(run_passes(c, ka, va, kb, vb, tb, false, ref_counter, cap, tile_shifts, tile_passes, hist_tile,
                         counters + kCtrSortTicket + depth_passes, &c->result_in_b, nullptr, nullptr,
                         T > 1 ? bit_length(T - 1) : 0u, tile_key_tile(c)));
     record(c, GS3D_STAGE_SORT, 1);
-    const uint32_t* sorted_tiles = reinterpret_cast<const uint32_t*>(c->result_in_b ? kb : ka);
+    const void* sorted_tiles = c->result_in_b ? kb : ka;
     const uint32_t* sorted_vals = c->result_in_b ? vb : va;
 
     record(c, GS3D_STAGE_IDENTIFY, 0);
-    HIP_TRY(c, launch_identify_tiles(sorted_tiles, 4, ref_counter, (uint32_t)cap, tile_offsets, T,
+    HIP_TRY(c, launch_identify_tiles(sorted_tiles, tb, ref_counter, (uint32_t)cap, tile_offsets, T,
                                      c->pinned, c->stream));
     record(c, GS3D_STAGE_IDENTIFY, 1);
 
@@ -994,7 +999,9 @@ int batch_enqueue(gs3d_ctx* c) {
         ea.projected = nullptr;  // emit reads the scan's records
         ea.order = sa.order;
         ea.recs = recs;
-        ea.tiles = reinterpret_cast<uint32_t*>(ka);
+        const uint32_t tb = tile_id_bytes(T);
+        ea.tiles = tb == 4 ? reinterpret_cast<uint32_t*>(ka) : nullptr;
+        ea.tiles16 = tb == 2 ? reinterpret_cast<uint16_t*>(ka) : nullptr;
         // with no tile-id pass (one tile) the emit order is the sorted order
         uint32_t* view_values = sorted_values + v.val_off;
         ea.values = tile_passes ? va : view_values;
@@ -1010,12 +1017,12 @@ int batch_enqueue(gs3d_ctx* c) {
         bool in_b = false;
         uint32_t tile_shifts[kMaxPasses];
         for (uint32_t p = 0; p < tile_passes; p++) tile_shifts[p] = shifts[depth_passes + p] - 32u;
-        GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, 4, false, ctr + kCtrRefCounter, cap, tile_shifts,
+        GS3D_TRY(batch_passes(c, s, ka, va, kb, vb, tb, false, ctr + kCtrRefCounter, cap, tile_shifts,
                               tile_passes, hist_tile, ctr + kCtrSortTicket + depth_passes, radix_status,
                               epoch_base, &epoch_ctr[k], &in_b, T > 1 ? bit_length(T - 1) : 0u, tile_key_tile(c),
                               nullptr, nullptr, view_values));
         uint32_t* to = tile_offsets + v.to_off;
-        HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, 4, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
+        HIP_TRY(c, launch_identify_tiles(in_b ? kb : ka, tb, ctr + kCtrRefCounter, (uint32_t)cap, to, T,
                                          b->pinned_k + vi, s));
         if (b->groups == 0) {
             // per-view raster launches: each view's raster overlaps other views' chains
@@ -1778,7 +1785,8 @@ int gs3d_read(gs3d_ctx* c, int id, void* dst, size_t bytes, size_t* written) {
             // from each key's splat
             uint64_t* rk;
             GS3D_TRY(ensure_t(c, "readback_keys", std::max<uint64_t>(K, 1), &rk));
-            HIP_TRY(c, launch_rebuild_keys(buf<uint32_t>(c, c->result_in_b ? "sort_keys_b" : "sort_keys_a"),
+            HIP_TRY(c, launch_rebuild_keys(buf<void>(c, c->result_in_b ? "sort_keys_b" : "sort_keys_a"),
+                                           c->frame_tile_bytes,
                                            buf<uint32_t>(c, c->result_in_b ? "sort_values_b" : "sort_values_a"),
                                            buf<gs3d_projected_splat>(c, "projected_splats"),
                                            buf<uint32_t>(c, "counters") + kCtrRefCounter, (uint32_t)K, rk, c->stream));
